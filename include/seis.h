@@ -1,0 +1,143 @@
+/*
+ * seis.h - C-ABI of libseis.so, the gfx950 (MI355X) 2-D acoustic
+ * finite-difference propagator with forward, tangent (Born) and adjoint
+ * passes, batched over many velocity models of uneven size.
+ *
+ * The reference exposes one Python entry point for this path,
+ *     vel_to_seis(vec, vec_diff=None, vec_adjoint=None, adjoint_on_residual=False)
+ *         -> (seis, seis_diff, grad)          core/seis_forward2.py:146-331
+ * backed by module globals (geometry, buffers, CUDA graphs; :13-144) and four
+ * CUDA kernels (lapg :381-421, update_p :428-462, update_p_diff :470-509,
+ * update_p_adjoint :517-579).  This header replaces that surface with a plan
+ * object (the module globals), one call per pass combination (the
+ * vec_diff / vec_adjoint switches) and integer status codes (the asserts).
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers unless marked "host".
+ *   - A plan owns every workspace; the caller passes inputs and outputs.
+ *   - A batch holds n_models models; model m's parameter vector has
+ *     nz*nx+1 floats (row-major velocities, depth = row, then min_vel), as
+ *     Velocity.to_vector (core/kaggle_support.py:243-251).  Batched vectors
+ *     are the concatenation in model order; seis_plan_layout gives offsets.
+ *   - Seismograms are float32 in (shot, t, receiver) order per model, as
+ *     Seismogram.to_vector (core/kaggle_support.py:210-216).
+ *   - Gradients are float64, length nz*nx+1 per model (last = d/d min_vel).
+ *   - Every call is asynchronous on the given stream; a NULL output pointer
+ *     skips that copy-out.  Calls on one plan must not overlap.
+ *   - Return value: 0 on success, negative on error (see seis_last_error).
+ */
+#ifndef SEIS_H
+#define SEIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct seis_plan seis_plan_t;
+typedef void* seis_stream_t; /* a hipStream_t; NULL = legacy default stream */
+
+enum {
+  SEIS_OK = 0,
+  SEIS_EINVAL = -1,  /* bad argument or shape */
+  SEIS_ENOMEM = -2,  /* device allocation failed */
+  SEIS_EHIP = -3,    /* HIP runtime error */
+  SEIS_ESTATE = -4   /* call not allowed in this plan's capabilities */
+};
+
+/* Capabilities requested at plan creation (bit mask). */
+enum {
+  SEIS_CAP_FORWARD = 1, /* forward seismograms                              */
+  SEIS_CAP_TANGENT = 2, /* + Born / tangent seismograms (J dv)               */
+  SEIS_CAP_ADJOINT = 4  /* + adjoint gradients (J^T dd) and J^T J products   */
+};
+
+/* One model of a batch.  Replaces the geometry globals of
+ * core/seis_forward2.py:69-101 (nz, nx, dx, shots at [0,17,34,52,69],
+ * receivers at every column of physical row 1).  Rows are physical
+ * (unpadded) indices, as adjust_sr (:60-68) resolves them. */
+typedef struct {
+  int32_t nz, nx;        /* physical grid (depth rows, lateral columns)      */
+  float dx;              /* grid spacing [m]                                 */
+  int32_t n_shots;       /* number of shots (<= SEIS_MAX_SHOTS)              */
+  int32_t src_z;         /* physical row of every source                     */
+  int32_t src_x[16];     /* physical column of each source                   */
+  int32_t rcv_z;         /* physical row of the receivers                    */
+  int32_t rcv_x0;        /* receiver r sits at column rcv_x0 + r*rcv_dx       */
+  int32_t rcv_dx;
+  int32_t n_rcv;
+} seis_model_desc;
+
+#define SEIS_MAX_SHOTS 16
+
+/* Time axis and absorbing pad shared by all models of a plan
+ * (core/seis_forward2.py:72-78: nbc=120, nt=999, dt=1e-3, Ricker 15 Hz). */
+typedef struct {
+  int32_t nt;            /* time steps (reference: 999)                     */
+  float dt;              /* step [s]                                         */
+  int32_t nbc;           /* absorbing pad cells on every side               */
+  const double* wavelet; /* host, nt samples: source time function s[t]     */
+} seis_time_desc;
+
+/* Create a plan for n_models models.  ckpt_every = 0 stores the full
+ * forward history for the adjoint (reference behaviour, :165-176);
+ * ckpt_every = K > 0 stores the wavefield every K steps and recomputes each
+ * segment during the adjoint pass.  caps is a mask of SEIS_CAP_*. */
+int seis_plan_create(const seis_model_desc* models /* host */, int32_t n_models,
+                     const seis_time_desc* time /* host */, int32_t ckpt_every,
+                     int32_t caps, seis_plan_t** plan_out /* host */);
+
+/* Device bytes a plan with these arguments would allocate (admission). */
+int64_t seis_plan_bytes(const seis_model_desc* models, int32_t n_models,
+                        const seis_time_desc* time, int32_t ckpt_every,
+                        int32_t caps);
+
+int seis_plan_destroy(seis_plan_t* plan);
+
+/* Host arrays of n_models+1 offsets: parameter vectors (floats / doubles),
+ * seismograms (floats).  Either pointer may be NULL. */
+int seis_plan_layout(const seis_plan_t* plan, int64_t* vec_off /* host */,
+                     int64_t* seis_off /* host */);
+
+/* Per-model activity mask (host, n_models ints; nonzero = active).  Inactive
+ * models are skipped by every kernel; their outputs are left untouched.
+ * All models are active after creation. */
+int seis_set_active(seis_plan_t* plan, const int32_t* active /* host */);
+
+/* Forward: seis = F(vec).  (vel_to_seis with vec_diff = vec_adjoint = None) */
+int seis_forward(seis_plan_t* plan, const float* vec, float* seis,
+                 seis_stream_t stream);
+
+/* Forward + tangent: seis = F(vec), dseis = J(vec) dvec.
+ * (vel_to_seis with vec_diff, :252-283) */
+int seis_tangent(seis_plan_t* plan, const float* vec, const float* dvec,
+                 float* seis, float* dseis, seis_stream_t stream);
+
+/* Forward + adjoint: seis = F(vec); grad = J^T dd, or J^T (seis - dd) when
+ * on_residual != 0 (vel_to_seis with vec_adjoint, :285-314 + :361-377).
+ * misfit (n_models doubles) receives sum((seis - dd)^2) per model when
+ * on_residual != 0. */
+int seis_gradient(seis_plan_t* plan, const float* vec, const float* dd,
+                  int32_t on_residual, float* seis, double* misfit,
+                  double* grad, seis_stream_t stream);
+
+/* Gauss-Newton product: dseis = J dvec and grad = J^T (J dvec), both at vec.
+ * Replaces the dense column-by-column J of core/seis_invert.py:84-90 with
+ * one tangent + one adjoint pass per product. */
+int seis_gn_product(seis_plan_t* plan, const float* vec, const float* dvec,
+                    float* dseis, double* grad, seis_stream_t stream);
+
+/* Number of kernel launches recorded in each captured time-loop graph so far
+ * (diagnostics) and the plan's device bytes. */
+int64_t seis_plan_device_bytes(const seis_plan_t* plan);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* seis_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEIS_H */

@@ -1,0 +1,217 @@
+"""Batched propagator plans and the drop-in ``vel_to_seis``.
+
+``Propagator`` wraps one libseis plan: a fixed list of models (each with its
+own ``Geometry``), device workspaces and the captured hipGraphs of the time
+loops.  Inputs and outputs are torch tensors on the GPU; model m's slice of a
+batched tensor is ``[vec_off[m]:vec_off[m+1]]`` (parameters) or
+``[seis_off[m]:seis_off[m+1]]`` (data).
+
+``vel_to_seis`` keeps the reference signature (core/seis_forward2.py:146-150)
+and return tuple (:331) so that callers written against the reference work
+unchanged on one GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
+from .geometry import Geometry
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def model_desc(g: Geometry) -> _lib.ModelDesc:
+    if g.n_shots > _lib.MAX_SHOTS:
+        raise ValueError(f"at most {_lib.MAX_SHOTS} shots per model")
+    d = _lib.ModelDesc()
+    d.nz, d.nx, d.dx = g.nz, g.nx, g.dx
+    d.n_shots = g.n_shots
+    d.src_z = g.src_z
+    for i, x in enumerate(g.src_x):
+        d.src_x[i] = x
+    d.rcv_z = g.rcv_z
+    d.rcv_x0 = g.rcv_x[0]
+    d.rcv_dx = g.rcv_x[1] - g.rcv_x[0] if g.n_rcv > 1 else 1
+    d.n_rcv = g.n_rcv
+    return d
+
+
+def _descs(geoms: Sequence[Geometry]):
+    keys = {g.time_key() for g in geoms}
+    if len(keys) != 1:
+        raise ValueError("all models of a plan must share nt, dt, nbc and freq")
+    arr = (_lib.ModelDesc * len(geoms))(*[model_desc(g) for g in geoms])
+    g0 = geoms[0]
+    wl = np.ascontiguousarray(g0.wavelet(), dtype=np.float64)
+    td = _lib.TimeDesc()
+    td.nt, td.dt, td.nbc = g0.nt, g0.dt, g0.nbc
+    td.wavelet = wl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return arr, td, wl
+
+
+def plan_bytes(geoms: Sequence[Geometry], ckpt_every: int = 0,
+               caps: int = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT) -> int:
+    """Device bytes a plan would allocate (runs on the host only)."""
+    arr, td, _ = _descs(geoms)
+    n = _lib.lib().seis_plan_bytes(arr, len(geoms), ctypes.byref(td), ckpt_every, caps)
+    if n < 0:
+        _lib.check(int(n), "seis_plan_bytes")
+    return int(n)
+
+
+class Propagator:
+    """A libseis plan over a fixed batch of models."""
+
+    def __init__(self, geoms: Sequence[Geometry] | Geometry, ckpt_every: int = 0,
+                 caps: int = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT, device="cuda"):
+        if isinstance(geoms, Geometry):
+            geoms = [geoms]
+        self.geoms = list(geoms)
+        self.n_models = len(self.geoms)
+        self.ckpt_every = ckpt_every
+        self.caps = caps
+        self.device = torch.device(device)
+        if self.device.type != "cuda" or not torch.cuda.is_available():
+            raise RuntimeError("Propagator needs a ROCm GPU (torch.cuda)")
+        L = _lib.lib()
+        arr, td, self._wavelet = _descs(self.geoms)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(L.seis_plan_create(arr, self.n_models, ctypes.byref(td), ckpt_every, caps,
+                                          ctypes.byref(handle)), "seis_plan_create")
+        self._h = handle
+        vo = (ctypes.c_int64 * (self.n_models + 1))()
+        so = (ctypes.c_int64 * (self.n_models + 1))()
+        _lib.check(L.seis_plan_layout(self._h, vo, so), "seis_plan_layout")
+        self.vec_off = np.array(vo[:], dtype=np.int64)
+        self.seis_off = np.array(so[:], dtype=np.int64)
+        self.n_vec = int(self.vec_off[-1])
+        self.n_seis = int(self.seis_off[-1])
+        self.device_bytes = int(L.seis_plan_device_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().seis_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- helpers
+    def _vec(self, v: torch.Tensor) -> torch.Tensor:
+        v = torch.as_tensor(v).reshape(-1)
+        if v.numel() != self.n_vec:
+            raise ValueError(f"parameter vector has {v.numel()} entries, plan expects {self.n_vec}")
+        return v.to(device=self.device, dtype=torch.float32).contiguous()
+
+    def _data(self, d: torch.Tensor) -> torch.Tensor:
+        d = torch.as_tensor(d).reshape(-1)
+        if d.numel() != self.n_seis:
+            raise ValueError(f"data vector has {d.numel()} entries, plan expects {self.n_seis}")
+        return d.to(device=self.device, dtype=torch.float32).contiguous()
+
+    def set_active(self, mask: Sequence[bool]):
+        m = (ctypes.c_int32 * self.n_models)(*[1 if a else 0 for a in mask])
+        _lib.check(_lib.lib().seis_set_active(self._h, m), "seis_set_active")
+
+    def split_vec(self, v: torch.Tensor):
+        return [v[self.vec_off[i]:self.vec_off[i + 1]] for i in range(self.n_models)]
+
+    def split_seis(self, s: torch.Tensor):
+        return [s[self.seis_off[i]:self.seis_off[i + 1]] for i in range(self.n_models)]
+
+    # ------------------------------------------------------------- passes
+    def forward(self, vec, out: torch.Tensor | None = None) -> torch.Tensor:
+        v = self._vec(vec)
+        seis = out if out is not None else torch.empty(self.n_seis, device=self.device)
+        _lib.check(_lib.lib().seis_forward(self._h, _ptr(v), _ptr(seis), _stream()), "seis_forward")
+        return seis
+
+    def tangent(self, vec, dvec):
+        v, dv = self._vec(vec), self._vec(dvec)
+        seis = torch.empty(self.n_seis, device=self.device)
+        dseis = torch.empty(self.n_seis, device=self.device)
+        _lib.check(_lib.lib().seis_tangent(self._h, _ptr(v), _ptr(dv), _ptr(seis), _ptr(dseis),
+                                           _stream()), "seis_tangent")
+        return seis, dseis
+
+    def gradient(self, vec, dd, on_residual: bool = True):
+        """(seis, misfit[n_models] float64 or None, grad float64)."""
+        v, d = self._vec(vec), self._data(dd)
+        seis = torch.empty(self.n_seis, device=self.device)
+        grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)
+        misfit = torch.empty(self.n_models, device=self.device, dtype=torch.float64) if on_residual else None
+        _lib.check(_lib.lib().seis_gradient(self._h, _ptr(v), _ptr(d), int(bool(on_residual)), _ptr(seis),
+                                            _ptr(misfit), _ptr(grad), _stream()), "seis_gradient")
+        return seis, misfit, grad
+
+    def gn_product(self, vec, dvec):
+        """(J dvec, J^T J dvec) at vec."""
+        v, dv = self._vec(vec), self._vec(dvec)
+        dseis = torch.empty(self.n_seis, device=self.device)
+        grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)
+        _lib.check(_lib.lib().seis_gn_product(self._h, _ptr(v), _ptr(dv), _ptr(dseis), _ptr(grad),
+                                              _stream()), "seis_gn_product")
+        return dseis, grad
+
+
+# ----------------------------------------------------------------- drop-in
+_default: dict = {}
+
+
+def default_propagator(geometry: Geometry | None = None, ckpt_every: int = 0) -> Propagator:
+    """One cached single-model plan per geometry (the reference's module globals)."""
+    g = geometry or Geometry()
+    key = (g, ckpt_every)
+    if key not in _default:
+        _default[key] = Propagator([g], ckpt_every=ckpt_every)
+    return _default[key]
+
+
+def vel_to_seis(vec, vec_diff=None, vec_adjoint=None, adjoint_on_residual=False,
+                geometry: Geometry | None = None):
+    """Drop-in for core/seis_forward2.py:146 vel_to_seis.
+
+    Returns (result, result_diff, result_adjoint) as float64 column tensors:
+    result = seismogram (n_data, 1); result_diff = J @ vec_diff; and
+    result_adjoint = J^T @ vec_adjoint, or J^T @ (result - vec_adjoint) when
+    ``adjoint_on_residual``.  Computation runs in float32 on the GPU.
+    """
+    g = geometry or Geometry()
+    P = default_propagator(g)
+    vec = torch.as_tensor(vec)
+    if tuple(vec.shape) != (g.n_params, 1):
+        raise ValueError(f"vec must have shape ({g.n_params}, 1), got {tuple(vec.shape)}")
+    if vec_diff is not None and tuple(torch.as_tensor(vec_diff).shape) != (g.n_params, 1):
+        raise ValueError(f"vec_diff must have shape ({g.n_params}, 1)")
+    if vec_adjoint is not None and tuple(torch.as_tensor(vec_adjoint).shape) != (g.n_data, 1):
+        raise ValueError(f"vec_adjoint must have shape ({g.n_data}, 1)")
+    result_diff = None
+    result_adjoint = None
+    if vec_adjoint is not None:
+        seis, _, grad = P.gradient(vec, vec_adjoint, on_residual=adjoint_on_residual)
+        result_adjoint = grad.reshape(-1, 1)
+        if vec_diff is not None:
+            _, dseis = P.tangent(vec, vec_diff)
+            result_diff = dseis.double().reshape(-1, 1)
+    elif vec_diff is not None:
+        seis, dseis = P.tangent(vec, vec_diff)
+        result_diff = dseis.double().reshape(-1, 1)
+    else:
+        seis = P.forward(vec)
+    torch.cuda.current_stream().synchronize()
+    return seis.double().reshape(-1, 1), result_diff, result_adjoint

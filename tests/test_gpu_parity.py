@@ -1,0 +1,187 @@
+"""GPU parity of the HIP propagator against the FP64 oracle (restatement of
+core/seis_forward2.py).  Bars: seismograms 1e-5 relative L2, gradients 1e-4
+relative L2, dot-product test 1e-6."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel
+from oracle import fd2d
+from seismic_amd import synth
+from seismic_amd.geometry import Geometry
+from seismic_amd.propagator import Propagator, vel_to_seis
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+GRAD_TOL = 1e-4
+DOT_TOL = 1e-6
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _case(g, style="curved", seed=0):
+    v = synth.make_model(style, g.nz, g.nx, seed)
+    return v, synth.to_vec(v)
+
+
+@pytest.mark.parametrize("style", ["flat", "curved", "faulted", "grf"])
+def test_forward_small(small_geoms, style):
+    g, og = small_geoms
+    _, vec = _case(g, style, seed=11)
+    ref = fd2d.vel_to_seis(vec, og)[0]
+    P = Propagator([g])
+    out = _np(P.forward(torch.tensor(vec)))
+    print(style, "forward rel", rel(out, ref))
+    assert rel(out, ref) < SEIS_TOL
+
+
+def test_tangent_small(small_geoms):
+    g, og = small_geoms
+    v, vec = _case(g, "curved", 5)
+    rng = np.random.default_rng(0)
+    dvec = np.concatenate([rng.normal(0, 10, g.nz * g.nx), [5.0]])
+    s_ref, ds_ref, _ = fd2d.vel_to_seis(vec, og, vec_diff=dvec)
+    P = Propagator([g])
+    s, ds = P.tangent(torch.tensor(vec), torch.tensor(dvec))
+    print("tangent rel", rel(_np(ds), ds_ref))
+    assert rel(_np(s), s_ref) < SEIS_TOL
+    assert rel(_np(ds), ds_ref) < SEIS_TOL
+
+
+@pytest.mark.parametrize("ckpt", [0, 7])
+def test_gradient_small(small_geoms, ckpt):
+    g, og = small_geoms
+    v, vec = _case(g, "faulted", 2)
+    target = fd2d.vel_to_seis(synth.to_vec(v * 1.02), og)[0]
+    s_ref, _, g_ref = fd2d.vel_to_seis(vec, og, vec_adjoint=target, adjoint_on_residual=True)
+    P = Propagator([g], ckpt_every=ckpt)
+    s, mis, grad = P.gradient(torch.tensor(vec), torch.tensor(target), on_residual=True)
+    grad = _np(grad)
+    print("grad rel", rel(grad, g_ref), "minvel", grad[-1], g_ref[-1])
+    assert rel(_np(s), s_ref) < SEIS_TOL
+    assert rel(grad, g_ref) < GRAD_TOL
+    assert abs(grad[-1] - g_ref[-1]) / abs(g_ref[-1]) < GRAD_TOL
+    mis_ref = np.sum((s_ref - target) ** 2)
+    assert abs(float(mis[0]) - mis_ref) / mis_ref < 1e-4
+
+
+def test_gradient_direct_source(small_geoms):
+    """on_residual=False: grad = J^T dd for an arbitrary data vector."""
+    g, og = small_geoms
+    _, vec = _case(g, "flat", 4)
+    rng = np.random.default_rng(1)
+    dd = rng.normal(0, 1, og.n_shots * og.nt * og.n_rcv)
+    _, _, g_ref = fd2d.vel_to_seis(vec, og, vec_adjoint=dd, adjoint_on_residual=False)
+    P = Propagator([g])
+    _, mis, grad = P.gradient(torch.tensor(vec), torch.tensor(dd), on_residual=False)
+    assert mis is None
+    assert rel(_np(grad), g_ref) < GRAD_TOL
+
+
+def test_checkpoint_matches_full_history(small_geoms):
+    g, og = small_geoms
+    v, vec = _case(g, "grf", 9)
+    target = fd2d.vel_to_seis(synth.to_vec(v + 40.0), og)[0]
+    outs = []
+    for ckpt in (0, 2, 13, 50, 299):
+        P = Propagator([g], ckpt_every=ckpt)
+        s, mis, grad = P.gradient(torch.tensor(vec), torch.tensor(target))
+        outs.append((_np(s), _np(grad)))
+    for s, gr in outs[1:]:
+        assert np.array_equal(s, outs[0][0])
+        assert np.array_equal(gr, outs[0][1])
+
+
+def test_dot_product(small_geoms):
+    """<J dm, r> = <dm, J^T r> on the float32 GPU path, r a model-misfit residual."""
+    g, og = small_geoms
+    v, vec = _case(g, "curved", 3)
+    rng = np.random.default_rng(7)
+    dm = np.concatenate([rng.normal(0, 20, g.nz * g.nx), [3.0]])
+    dd = fd2d.vel_to_seis(synth.to_vec(v * 0.98), og)[0]
+    P = Propagator([g])
+    s, jdm = P.tangent(torch.tensor(vec), torch.tensor(dm))
+    _, _, jt = P.gradient(torch.tensor(vec), torch.tensor(dd), on_residual=True)
+    r = _np(s) - dd
+    a = float(np.dot(r, _np(jdm)))
+    b = float(np.dot(_np(jt), dm))
+    print("dot", a, b, abs(a - b) / abs(a))
+    assert abs(a - b) / abs(a) < DOT_TOL
+
+
+def test_gn_product(small_geoms):
+    g, og = small_geoms
+    v, vec = _case(g, "curved", 8)
+    rng = np.random.default_rng(2)
+    dv = np.concatenate([rng.normal(0, 10, g.nz * g.nx), [1.0]])
+    _, jdv, _ = fd2d.vel_to_seis(vec, og, vec_diff=dv)
+    _, _, jtjdv = fd2d.vel_to_seis(vec, og, vec_adjoint=jdv)
+    for ckpt in (0, 11):
+        P = Propagator([g], ckpt_every=ckpt)
+        ds, gr = P.gn_product(torch.tensor(vec), torch.tensor(dv))
+        assert rel(_np(ds), jdv) < SEIS_TOL
+        assert rel(_np(gr), jtjdv) < GRAD_TOL
+
+
+def test_batch_invariance_mixed_sizes():
+    """Each model's outputs are bit-identical alone and inside a mixed batch."""
+    kw = dict(nbc=12, nt=200)
+    geoms = [Geometry(nz=20, nx=24, **kw), Geometry(nz=30, nx=30, **kw), Geometry(nz=16, nx=40, **kw)]
+    vecs = [synth.to_vec(synth.make_model(s, gg.nz, gg.nx, i))
+            for i, (s, gg) in enumerate(zip(["flat", "grf", "faulted"], geoms))]
+    targets = [synth.to_vec(synth.make_model("flat", gg.nz, gg.nx, 100 + i)) for i, gg in enumerate(geoms)]
+    Pb = Propagator(geoms, ckpt_every=9)
+    dd = torch.cat([Propagator([gg]).forward(torch.tensor(t)) for gg, t in zip(geoms, targets)])
+    sb, mb, gb = Pb.gradient(torch.tensor(np.concatenate(vecs)), dd)
+    sb, gb = _np(sb), _np(gb)
+    for i, gg in enumerate(geoms):
+        P1 = Propagator([gg], ckpt_every=9)
+        d1 = dd[int(Pb.seis_off[i]):int(Pb.seis_off[i + 1])]
+        s1, m1, g1 = P1.gradient(torch.tensor(vecs[i]), d1)
+        assert np.array_equal(_np(s1), sb[Pb.seis_off[i]:Pb.seis_off[i + 1]])
+        assert np.array_equal(_np(g1), gb[Pb.vec_off[i]:Pb.vec_off[i + 1]])
+        assert float(m1[0]) == float(mb[i])
+
+
+def test_inactive_models_untouched():
+    kw = dict(nbc=12, nt=160)
+    geoms = [Geometry(nz=20, nx=20, **kw), Geometry(nz=24, nx=20, **kw)]
+    vecs = np.concatenate([synth.to_vec(synth.make_model("flat", gg.nz, gg.nx, i)) for i, gg in enumerate(geoms)])
+    P = Propagator(geoms)
+    full = _np(P.forward(torch.tensor(vecs)))
+    P.set_active([False, True])
+    out = torch.full((P.n_seis,), 7.0, device="cuda")
+    P.forward(torch.tensor(vecs), out=out)
+    out = _np(out)
+    assert np.all(out[:P.seis_off[1]] == 7.0)
+    assert np.array_equal(out[P.seis_off[1]:], full[P.seis_off[1]:])
+
+
+@pytest.mark.slow
+def test_forward_full_size_reference_geometry():
+    """Reference geometry (70x70, nbc=120, nt=999, 5 shots): shots 0 and 2."""
+    g = Geometry()
+    og = fd2d.Geometry()
+    v = synth.curve_fault_model(70, 70, seed=1)
+    vec = synth.to_vec(v)
+    ref = fd2d.forward_only(vec, og, shots=[0, 2])
+    out = _np(Propagator([g]).forward(torch.tensor(vec))).reshape(5, g.nt, g.n_rcv)[[0, 2]]
+    print("full-size forward rel", rel(out, ref))
+    assert rel(out, ref) < SEIS_TOL
+
+
+def test_dropin_signature():
+    g = Geometry(nz=20, nx=24, nbc=16, nt=150)
+    v = synth.make_model("flat", g.nz, g.nx, 1)
+    vec = torch.tensor(synth.to_vec(v)).reshape(-1, 1)
+    dvec = torch.ones_like(vec)
+    dd = torch.zeros(g.n_data, 1, dtype=torch.float64)
+    s, sd, gr = vel_to_seis(vec, dvec, dd, adjoint_on_residual=True, geometry=g)
+    assert s.shape == (g.n_data, 1) and sd.shape == (g.n_data, 1) and gr.shape == (g.n_params, 1)
+    assert s.dtype == torch.float64 and gr.dtype == torch.float64
+    with pytest.raises(ValueError):
+        vel_to_seis(vec[:-1], geometry=g)
