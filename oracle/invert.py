@@ -1,0 +1,338 @@
+"""FP64 NumPy restatement of the reference inversion loops.
+
+TEST INFRASTRUCTURE ONLY (parity oracle); the product never imports it.
+
+  cost_and_gradient   core/seis_invert.py:19-49
+  gauss_newton_dense  core/seis_invert.py:63-107  (InversionModel.seis_to_vel_gn)
+  gauss_newton_cg     the same pass with the dense J^T J solve replaced by
+                      conjugate gradients on J^T J products (new; pinned by the
+                      dense solve at small size)
+  bfgs                core/seis_numerics.py:307-448 (torch 2.7 LBFGS port)
+  strong_wolfe        core/seis_numerics.py:138-277
+  cubic_interpolate   core/seis_numerics.py:279-304
+  lbfgs_invert        core/seis_invert.py:109-152 (InversionModel.seis_to_vel_lbfgs)
+
+``forward`` callbacks take a (n_params,) vector and return the flat
+seismogram; ``fev`` callbacks return (seis, grad_of_sum_sq_residual/2...)
+exactly as vel_to_seis does, so the oracle propagator (fd2d.vel_to_seis) or
+any other implementation can be plugged in.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import fd2d
+
+
+def cost_and_gradient(x, target, prior, basis, g, compute_gradient=False, v2s=None):
+    """core/seis_invert.py:19-49.  Returns (cost, grad, cost_prior, cost_res)."""
+    v2s = v2s or (lambda vec, **kw: fd2d.vel_to_seis(vec, g, **kw))
+    x = np.asarray(x, dtype=np.float64).reshape(-1)
+    cost_prior, grad_prior = prior.cost_grad(x)
+    vec = basis @ x
+    if compute_gradient:
+        s, _, s_adj = v2s(vec, vec_adjoint=target, adjoint_on_residual=True)
+    else:
+        s = v2s(vec)[0]
+    cost_res = np.mean((s - target) ** 2)
+    if compute_gradient:
+        grad = grad_prior + (2.0 / len(s)) * (basis.T @ s_adj)
+        return cost_prior + cost_res, grad, cost_prior, cost_res
+    return cost_prior + cost_res, None, cost_prior, cost_res
+
+
+def _line_search_scales(n=50):
+    return np.linspace(-0.1, 1.1, n)
+
+
+def gauss_newton_dense(vec_guess, target, prior, g, v2s=None, n_scales=50):
+    """One Gauss-Newton pass, core/seis_invert.py:63-107 (prior must have P)."""
+    v2s = v2s or (lambda vec, **kw: fd2d.vel_to_seis(vec, g, **kw))
+    B = prior.basis
+    vec_guess = np.asarray(vec_guess, dtype=np.float64).reshape(-1)
+    x_guess = np.linalg.solve(B.T @ B, B.T @ vec_guess)
+    N = len(target)
+    grad_res = v2s(B @ x_guess, vec_adjoint=target, adjoint_on_residual=True)[2]
+    rhs = -B.T @ grad_res / N
+    rhs = rhs - np.concatenate([prior.lam * prior.P @ x_guess[:-1], [0.0]])
+    n = B.shape[1]
+    P = np.zeros((n, n))
+    P[:-1, :-1] = prior.lam * prior.P
+    J = np.empty((N, n))
+    for i in range(n):
+        J[:, i] = v2s(vec_guess, vec_diff=B[:, i])[1]
+    A = P + J.T @ J / N
+    dx = np.linalg.solve(A, rhs)
+    return _gn_line_search(vec_guess, x_guess, dx, target, prior, g, v2s, n_scales)
+
+
+def _gn_line_search(vec_guess, x_guess, dx, target, prior, g, v2s, n_scales):
+    """Crude line search of core/seis_invert.py:95-102."""
+    B = prior.basis
+    scales = _line_search_scales(n_scales)
+    vals = [cost_and_gradient(x_guess + sc * dx, target, prior, B, g, False, v2s)[0] for sc in scales]
+    best = scales[int(np.argmin(vals))]
+    return vec_guess + best * (B @ dx), best, np.array(vals)
+
+
+def gauss_newton_cg(vec_guess, target, prior, g, n_cg=20, v2s=None, n_scales=50):
+    """GN pass solving (lam P (+) 0 + J^T J / N) dx = rhs by n_cg CG steps from 0.
+
+    Identical to gauss_newton_dense except for the linear solve; with n_cg >=
+    the basis size CG reproduces the dense solve in exact arithmetic.
+    """
+    v2s = v2s or (lambda vec, **kw: fd2d.vel_to_seis(vec, g, **kw))
+    B = prior.basis
+    vec_guess = np.asarray(vec_guess, dtype=np.float64).reshape(-1)
+    x_guess = np.linalg.solve(B.T @ B, B.T @ vec_guess)
+    N = len(target)
+    grad_res = v2s(B @ x_guess, vec_adjoint=target, adjoint_on_residual=True)[2]
+    rhs = -B.T @ grad_res / N
+    rhs = rhs - np.concatenate([prior.lam * prior.P @ x_guess[:-1], [0.0]])
+
+    def A(v):
+        jv = v2s(vec_guess, vec_diff=B @ v)[1]
+        jtjv = v2s(vec_guess, vec_adjoint=jv)[2]
+        out = B.T @ jtjv / N
+        out[:-1] += prior.lam * prior.P @ v[:-1]
+        return out
+
+    dx = conjugate_gradient(A, rhs, n_cg)
+    return _gn_line_search(vec_guess, x_guess, dx, target, prior, g, v2s, n_scales)
+
+
+def conjugate_gradient(A, b, n_iter):
+    """Plain CG from x=0 (Hestenes-Stiefel), float64."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    rr = r @ r
+    for _ in range(n_iter):
+        if rr == 0.0:
+            break
+        Ap = A(p)
+        a = rr / (p @ Ap)
+        x = x + a * p
+        r = r - a * Ap
+        rr_new = r @ r
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    return x
+
+
+# ----------------------------------------------------------------- LBFGS
+def cubic_interpolate(x1, f1, g1, x2, f2, g2, bounds=None):
+    """core/seis_numerics.py:279-304."""
+    if bounds is not None:
+        xmin_bound, xmax_bound = bounds
+    else:
+        xmin_bound, xmax_bound = (x1, x2) if x1 <= x2 else (x2, x1)
+    d1 = g1 + g2 - 3 * (f1 - f2) / (x1 - x2)
+    d2_square = d1 ** 2 - g1 * g2
+    if d2_square >= 0:
+        d2 = np.sqrt(d2_square)
+        if x1 <= x2:
+            min_pos = x2 - (x2 - x1) * ((g2 + d2 - d1) / (g2 - g1 + 2 * d2))
+        else:
+            min_pos = x1 - (x1 - x2) * ((g1 + d2 - d1) / (g1 - g2 + 2 * d2))
+        return min(max(min_pos, xmin_bound), xmax_bound)
+    return (xmin_bound + xmax_bound) / 2.0
+
+
+def strong_wolfe(obj_func, x, t, d, f, g, gtd, c1=1e-4, c2=0.9, tolerance_change=1e-9,
+                 max_ls=25):
+    """core/seis_numerics.py:138-277 (torch.optim.lbfgs._strong_wolfe)."""
+    d_norm = np.abs(d).max()
+    g = g.copy()
+    f_new, g_new = obj_func(x, t, d)
+    ls_func_evals = 1
+    gtd_new = g_new @ d
+    t_prev, f_prev, g_prev, gtd_prev = 0, f, g, gtd
+    done = False
+    ls_iter = 0
+    while ls_iter < max_ls:
+        if f_new > (f + c1 * t * gtd) or (ls_iter > 1 and f_new >= f_prev):
+            bracket = [t_prev, t]
+            bracket_f = [f_prev, f_new]
+            bracket_g = [g_prev, g_new.copy()]
+            bracket_gtd = [gtd_prev, gtd_new]
+            break
+        if abs(gtd_new) <= -c2 * gtd:
+            bracket = [t]
+            bracket_f = [f_new]
+            bracket_g = [g_new]
+            done = True
+            break
+        if gtd_new >= 0:
+            bracket = [t_prev, t]
+            bracket_f = [f_prev, f_new]
+            bracket_g = [g_prev, g_new.copy()]
+            bracket_gtd = [gtd_prev, gtd_new]
+            break
+        min_step = t + 0.01 * (t - t_prev)
+        max_step = t * 10
+        tmp = t
+        t = cubic_interpolate(t_prev, f_prev, gtd_prev, t, f_new, gtd_new, bounds=(min_step, max_step))
+        t_prev = tmp
+        f_prev = f_new
+        g_prev = g_new.copy()
+        gtd_prev = gtd_new
+        f_new, g_new = obj_func(x, t, d)
+        ls_func_evals += 1
+        gtd_new = g_new @ d
+        ls_iter += 1
+    if ls_iter == max_ls:
+        bracket = [0, t]
+        bracket_f = [f, f_new]
+        bracket_g = [g, g_new]
+    insuf_progress = False
+    low_pos, high_pos = (0, 1) if bracket_f[0] <= bracket_f[-1] else (1, 0)
+    while not done and ls_iter < max_ls:
+        if abs(bracket[1] - bracket[0]) * d_norm < tolerance_change:
+            break
+        t = cubic_interpolate(bracket[0], bracket_f[0], bracket_gtd[0],
+                              bracket[1], bracket_f[1], bracket_gtd[1])
+        eps = 0.1 * (max(bracket) - min(bracket))
+        if min(max(bracket) - t, t - min(bracket)) < eps:
+            if insuf_progress or t >= max(bracket) or t <= min(bracket):
+                if abs(t - max(bracket)) < abs(t - min(bracket)):
+                    t = max(bracket) - eps
+                else:
+                    t = min(bracket) + eps
+                insuf_progress = False
+            else:
+                insuf_progress = True
+        else:
+            insuf_progress = False
+        f_new, g_new = obj_func(x, t, d)
+        ls_func_evals += 1
+        gtd_new = g_new @ d
+        ls_iter += 1
+        if f_new > (f + c1 * t * gtd) or f_new >= bracket_f[low_pos]:
+            bracket[high_pos] = t
+            bracket_f[high_pos] = f_new
+            bracket_g[high_pos] = g_new.copy()
+            bracket_gtd[high_pos] = gtd_new
+            low_pos, high_pos = (0, 1) if bracket_f[0] <= bracket_f[1] else (1, 0)
+        else:
+            if abs(gtd_new) <= -c2 * gtd:
+                done = True
+            elif gtd_new * (bracket[high_pos] - bracket[low_pos]) >= 0:
+                bracket[high_pos] = bracket[low_pos]
+                bracket_f[high_pos] = bracket_f[low_pos]
+                bracket_g[high_pos] = bracket_g[low_pos]
+                bracket_gtd[high_pos] = bracket_gtd[low_pos]
+            bracket[low_pos] = t
+            bracket_f[low_pos] = f_new
+            bracket_g[low_pos] = g_new.copy()
+            bracket_gtd[low_pos] = gtd_new
+    t = bracket[low_pos]
+    f_new = bracket_f[low_pos]
+    g_new = bracket_g[low_pos]
+    return f_new, g_new, t, ls_func_evals
+
+
+def bfgs(cost_and_gradient_func, x0, max_iter, tolerance_grad, tolerance_change,
+         history_size=None, torch_compat=False):
+    """core/seis_numerics.py:307-448.  history_size=None keeps every pair
+    (the reference's unbounded history); an int keeps the last pairs only.
+    torch_compat=True adds the two behaviours of the installed torch LBFGS that
+    the reference's port does not have: a break when g.d > -tolerance_change
+    before the line search, and a line search capped at the remaining
+    evaluation budget (the reference uses max_ls=25, :419-421)."""
+    lr = 1
+    max_eval = max_iter * 5 // 4
+    x = np.array(x0, dtype=np.float64).reshape(-1)
+    orig_loss, flat_grad = cost_and_gradient_func(x)
+    loss = float(orig_loss)
+    current_evals = 1
+    if np.abs(flat_grad).max() <= tolerance_grad:
+        return x, current_evals
+    n_iter = 0
+    old_dirs, old_stps, ro = [], [], []
+    prev_flat_grad = None
+    t = None
+    d = None
+    H_diag = 1
+    while n_iter < max_iter:
+        n_iter += 1
+        if n_iter == 1:
+            d = -flat_grad
+            H_diag = 1
+        else:
+            y = flat_grad - prev_flat_grad
+            s = d * t
+            ys = y @ s
+            if ys > 1e-10:
+                if history_size is not None and len(old_dirs) == history_size:
+                    old_dirs.pop(0)
+                    old_stps.pop(0)
+                    ro.pop(0)
+                old_dirs.append(y)
+                old_stps.append(s)
+                ro.append(1.0 / ys)
+                H_diag = ys / (y @ y)
+            num_old = len(old_dirs)
+            q = -flat_grad
+            al = [0.0] * num_old
+            for i in range(num_old - 1, -1, -1):  # lbfgs_backward kernel (:451-501)
+                al[i] = (old_stps[i] @ q) * ro[i]
+                q = q - al[i] * old_dirs[i]
+            r = q * H_diag
+            for i in range(num_old):  # lbfgs_forward kernel (:511-559)
+                be_i = (old_dirs[i] @ r) * ro[i]
+                r = r + (al[i] - be_i) * old_stps[i]
+            d = r
+        prev_flat_grad = flat_grad.copy()
+        prev_loss = loss
+        if n_iter == 1:
+            t = min(1.0, 1.0 / np.abs(flat_grad).sum()) * lr
+        else:
+            t = lr
+        gtd = flat_grad @ d
+        if torch_compat and gtd > -tolerance_change:
+            break
+        x_init = x.copy()
+
+        def obj_func(xx, tt, dd):
+            return cost_and_gradient_func(xx + tt * dd)
+
+        max_ls = (max_eval - current_evals) if torch_compat else 25
+        loss, flat_grad, t, ls_func_evals = strong_wolfe(obj_func, x_init, t, d, loss, flat_grad, gtd,
+                                                         max_ls=max_ls)
+        loss = float(loss)
+        x = x + t * d
+        opt_cond = np.abs(flat_grad).max() <= tolerance_grad
+        current_evals += ls_func_evals
+        if n_iter == max_iter:
+            break
+        if current_evals >= max_eval:
+            break
+        if opt_cond:
+            break
+        if np.abs(d * t).max() <= tolerance_change:
+            break
+        if abs(loss - prev_loss) < tolerance_change:
+            break
+    return x, current_evals
+
+
+def lbfgs_invert(vec_guess, target, prior, g, max_iter, tolerance_grad=1e-7, tolerance_change=0.0,
+                 scaling=1e15, seis_error_tolerance=0.0, history_size=None, v2s=None):
+    """InversionModel.seis_to_vel_lbfgs, core/seis_invert.py:109-152."""
+    B = prior.basis
+    vec_guess = np.asarray(vec_guess, dtype=np.float64).reshape(-1)
+    x_guess = np.linalg.solve(B.T @ B, B.T @ vec_guess)
+    nfev = [0]
+
+    def func(x):
+        nfev[0] += 1
+        cost, grad, cost_prior, cost_res = cost_and_gradient(x, target, prior, B, g, True, v2s)
+        if cost_res < seis_error_tolerance:
+            cost, grad = 0 * cost, 0 * grad
+        return cost * scaling, grad * scaling
+
+    x, _ = bfgs(func, x_guess, max_iter, tolerance_grad, tolerance_change, history_size)
+    if np.any(np.isnan(x)):
+        return vec_guess.copy(), nfev[0]
+    return B @ x, nfev[0]

@@ -74,6 +74,7 @@ struct ModelDev {
   int64_t cell_off;   // into per-model padded coefficient fields
   int64_t seis_off;   // into the concatenated seismograms
   int64_t seis_len;   // n_shots * nt * n_rcv
+  int64_t fold_off;   // into the float64 fold scratch (NZ*nx + NZ per model)
   int rcv_row, rcv_x0, rcv_dx, n_rcv;  // padded row / first padded column
   double dt;          // time step [s]
   double dtdx;        // dt / dx
@@ -449,91 +450,119 @@ __global__ void k_params(const ModelDev* models, const WaveDev* waves, int n_wav
   }
 }
 
-// prep_run_adjoint (:361-377) + unpad_edge_padded_gradient (seis_numerics.py:14-79)
-// + the source-amplitude term (:314): one thread per physical cell sums the
-// padded cells that edge padding maps onto it, over all shots, in float64.
-__global__ __launch_bounds__(NTHR) void k_grad_fold(const ModelDev* models, const WaveDev* waves,
-                                                     const WaveParam* wparam, const float* arena,
-                                                     const float* vec, double* grad,
-                                                     const int64_t* grad_off) {
+constexpr int kRedBlocks = 32;  // partial-sum blocks per model (misfit)
+
+// Block-wide fixed-order tree sum of NTHR doubles (deterministic).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int k = NTHR / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// First half of prep_run_adjoint (:361-377) + unpad_edge_padded_gradient
+// (seis_numerics.py:14-79): for padded row Z, fold the columns that edge
+// padding maps onto each physical column (left strip -> column 0, right strip
+// -> column nx-1), summing the alpha accumulator over shots in float64.  Also
+// the row's share of sum(j^2 * kappa accumulator) for d/d min_vel (:366-370).
+__global__ __launch_bounds__(NTHR) void k_fold_rows(const ModelDev* models, const WaveDev* waves,
+                                                     const float* arena, double* fold) {
+  __shared__ double red[NTHR];
+  const ModelDev& m = models[blockIdx.y];
+  if (!m.active || (int)blockIdx.x >= m.NZ) return;
+  const int Z = blockIdx.x;
+  double* R = fold + m.fold_off + (int64_t)Z * m.nx;
+  double* rowK = fold + m.fold_off + (int64_t)m.NZ * m.nx;
+  double sl = 0.0, sr = 0.0, sk = 0.0;
+  for (int X = threadIdx.x; X < m.NX; X += NTHR) {
+    const int64_t g = (int64_t)Z * m.NX + X;
+    double a = 0.0, k = 0.0;
+    for (int s = 0; s < m.n_shots; ++s) {
+      const WaveDev& w = waves[m.first_wave + s];
+      a += (double)arena[w.off[R_GA] + g];
+      k += (double)arena[w.off[R_GK] + g];
+    }
+    const int j = damp_j(Z, X, m.nz, m.nx, m.nbc);
+    sk += (double)(j * j) * k;
+    const int ix = X - m.nbc;
+    if (ix <= 0) sl += a;
+    else if (ix >= m.nx - 1) sr += a;
+    else R[ix] = a;
+  }
+  sl = block_sum(sl, red);
+  sr = block_sum(sr, red);
+  sk = block_sum(sk, red);
+  if (threadIdx.x == 0) {
+    R[0] = sl;
+    R[m.nx - 1] = sr;
+    rowK[Z] = sk;
+  }
+}
+
+// Second half: fold rows (top strip -> row 0, bottom strip -> row nz-1),
+// apply v_adj = 2 v (dt/dx)^2 alpha_adj (:367-369), add the source-amplitude
+// term 2 dt^2 v_src sum_t s_adj[t] s[t] (:314) and d/d min_vel (:370-374).
+__global__ __launch_bounds__(NTHR) void k_fold_cols(const ModelDev* models, const WaveDev* waves,
+                                                     const WaveParam* wparam, const double* fold,
+                                                     const float* vec, double* grad) {
   const ModelDev& m = models[blockIdx.y];
   if (!m.active) return;
+  const double* R = fold + m.fold_off;
+  const int64_t go = m.vec_off;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double* rowK = R + (int64_t)m.NZ * m.nx;
+    double sk = 0.0;
+    for (int Z = 0; Z < m.NZ; ++Z) sk += rowK[Z];
+    const double nb1 = (double)(m.nbc - 1);
+    grad[go + (int64_t)m.nz * m.nx] = m.dt * m.kappa0 / (nb1 * nb1) * sk;
+  }
   const int n = m.nz * m.nx;
   const int i = blockIdx.x * NTHR + threadIdx.x;
   if (i >= n) return;
   const int iz = i / m.nx, ix = i - (i / m.nx) * m.nx;
   const int z_lo = iz == 0 ? 0 : iz + m.nbc;
   const int z_hi = iz == m.nz - 1 ? m.NZ - 1 : iz + m.nbc;
-  const int x_lo = ix == 0 ? 0 : ix + m.nbc;
-  const int x_hi = ix == m.nx - 1 ? m.NX - 1 : ix + m.nbc;
   double acc = 0.0;
-  for (int s = 0; s < m.n_shots; ++s) {
-    const WaveDev& w = waves[m.first_wave + s];
-    const float* GA = arena + w.off[R_GA];
-    for (int z = z_lo; z <= z_hi; ++z)
-      for (int x = x_lo; x <= x_hi; ++x) acc += (double)GA[(int64_t)z * m.NX + x];
-  }
+  for (int Z = z_lo; Z <= z_hi; ++Z) acc += R[(int64_t)Z * m.nx + ix];
   const double v = vec[m.vec_off + i];
   double gv = 2.0 * v * m.dtdx * m.dtdx * acc;
   for (int s = 0; s < m.n_shots; ++s) {
     const WaveDev& w = waves[m.first_wave + s];
     if (w.src_z == iz && w.src_x == ix) gv += 2.0 * m.dt * m.dt * v * wparam[m.first_wave + s].ssum;
   }
-  grad[grad_off[blockIdx.y] + i] = gv;
+  grad[go + i] = gv;
 }
 
-// d/d min_vel = dt * sum(damp * kappa_adjoint) (:366-370): one block per model,
-// fixed-order tree reduction in float64.
-__global__ __launch_bounds__(1024) void k_grad_minvel(const ModelDev* models, const WaveDev* waves,
-                                                       const float* arena, double* grad,
-                                                       const int64_t* grad_off) {
-  __shared__ double red[1024];
-  const ModelDev& m = models[blockIdx.x];
-  if (!m.active) return;
-  const int64_t P = (int64_t)m.NZ * m.NX;
-  double acc = 0.0;
-  for (int s = 0; s < m.n_shots; ++s) {
-    const float* GK = arena + waves[m.first_wave + s].off[R_GK];
-    for (int64_t i = threadIdx.x; i < P; i += 1024) {
-      const int z = (int)(i / m.NX);
-      const int x = (int)(i - (int64_t)z * m.NX);
-      const int j = damp_j(z, x, m.nz, m.nx, m.nbc);
-      if (j) acc += (double)(j * j) * (double)GK[i];
-    }
-  }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double nb1 = (double)(m.nbc - 1);
-    grad[grad_off[blockIdx.x] + m.nz * m.nx] = m.dt * m.kappa0 / (nb1 * nb1) * red[0];
-  }
-}
-
-// sum((seis - dd)^2) per model (the residual part of cost_and_gradient,
-// core/seis_invert.py:41, before the mean).
-__global__ __launch_bounds__(1024) void k_misfit(const ModelDev* models, const float* seis,
-                                                  const float* dd, double* out) {
-  __shared__ double red[1024];
-  const ModelDev& m = models[blockIdx.x];
+// sum((seis - dd)^2) per model (cost_and_gradient, core/seis_invert.py:41,
+// before the mean): kRedBlocks fixed-order partials, then one final sum.
+__global__ __launch_bounds__(NTHR) void k_misfit_part(const ModelDev* models, const float* seis,
+                                                       const float* dd, double* part) {
+  __shared__ double red[NTHR];
+  const ModelDev& m = models[blockIdx.y];
   if (!m.active) return;
   const float* s = seis + m.seis_off;
   const float* d = dd + m.seis_off;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < m.seis_len; i += 1024) {
+  for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < m.seis_len;
+       i += (int64_t)kRedBlocks * NTHR) {
     const double r = (double)s[i] - (double)d[i];
     acc += r * r;
   }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int k = 512; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * kRedBlocks + blockIdx.x] = acc;
+}
+
+__global__ void k_misfit_final(const ModelDev* models, int n_models, const double* part, double* out) {
+  const int mi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mi >= n_models || !models[mi].active) return;
+  double acc = 0.0;
+  for (int b = 0; b < kRedBlocks; ++b) acc += part[(int64_t)mi * kRedBlocks + b];
+  out[mi] = acc;
 }
 
 }  // namespace
@@ -558,7 +587,7 @@ struct Layout {
   std::vector<ModelDev> models;
   std::vector<WaveDev> waves;
   std::vector<int64_t> vec_off, seis_off;
-  int64_t cells = 0, arena = 0, seis = 0, vec = 0;
+  int64_t cells = 0, arena = 0, seis = 0, vec = 0, fold = 0;
   int K = 0, J = 0;
   bool full = true, fwd_hist = true;
   int n_blk = 0, max_P = 0, max_cells_phys = 0;
@@ -608,6 +637,8 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
     L.vec_off.push_back(L.vec);
     L.seis_off.push_back(L.seis);
     L.vec += (int64_t)d.nz * d.nx + 1;
+    m.fold_off = L.fold;
+    L.fold += align64((int64_t)m.NZ * m.nx + m.NZ);
     L.cells += align64(P);
     L.seis += m.seis_len;
     const int tiles_x = (m.NX + TX - 1) / TX, tiles_z = (m.NZ + TZ - 1) / TZ;
@@ -655,6 +686,7 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
   }
   const int64_t nw = (int64_t)L.waves.size();
   L.bytes = (int64_t)sizeof(float) * (L.arena + L.cells * (tan ? 2 : 1) + L.seis * (1 + (tan ? 1 : 0) + (adj ? 1 : 0)) + nt) +
+            (int64_t)sizeof(double) * (adj ? L.fold : 0) + (int64_t)sizeof(double) * 32 * n_models +
             (int64_t)(sizeof(ModelDev) + 2 * sizeof(int64_t) + sizeof(ParamDev)) * n_models +
             (int64_t)(sizeof(WaveDev) + sizeof(WaveParam)) * nw + (int64_t)sizeof(int2) * L.n_blk;
   return SEIS_OK;
@@ -670,7 +702,8 @@ struct seis_plan {
   ParamDev* d_params = nullptr;
   WaveParam* d_wparam = nullptr;
   int2* d_blk = nullptr;
-  int64_t* d_grad_off = nullptr;
+  double* d_fold = nullptr;
+  double* d_part = nullptr;
   float* d_alpha = nullptr;
   float* d_dalpha = nullptr;
   float* d_wavelet = nullptr;
@@ -691,7 +724,7 @@ void plan_free(seis_plan* p) {
       if (g) (void)hipGraphExecDestroy(g);
   for (auto& g : p->g_adj)
     if (g) (void)hipGraphExecDestroy(g);
-  void* bufs[] = {p->d_models, p->d_waves, p->d_params, p->d_wparam, p->d_blk, p->d_grad_off,
+  void* bufs[] = {p->d_models, p->d_waves, p->d_params, p->d_wparam, p->d_blk, p->d_fold, p->d_part,
                   p->d_alpha, p->d_dalpha, p->d_wavelet, p->d_arena, p->d_seis, p->d_dseis, p->d_dd};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -845,13 +878,21 @@ int run_prep(seis_plan* p, hipStream_t s, const float* vec, const float* dvec) {
 
 int finish_grad(seis_plan* p, hipStream_t s, const float* vec, double* grad) {
   const Layout& L = p->L;
-  hipLaunchKernelGGL(k_grad_fold, dim3((L.max_cells_phys + NTHR - 1) / NTHR, p->n_models),
-                     dim3(NTHR), 0, s, p->d_models, p->d_waves, p->d_wparam, p->d_arena, vec, grad,
-                     p->d_grad_off);
-  hipLaunchKernelGGL(k_grad_minvel, dim3(p->n_models), dim3(1024), 0, s, p->d_models, p->d_waves,
-                     p->d_arena, grad, p->d_grad_off);
+  int max_nz_pad = 0;
+  for (const auto& m : L.models) max_nz_pad = std::max(max_nz_pad, m.NZ);
+  hipLaunchKernelGGL(k_fold_rows, dim3(max_nz_pad, p->n_models), dim3(NTHR), 0, s, p->d_models,
+                     p->d_waves, p->d_arena, p->d_fold);
+  hipLaunchKernelGGL(k_fold_cols, dim3((L.max_cells_phys + NTHR - 1) / NTHR, p->n_models),
+                     dim3(NTHR), 0, s, p->d_models, p->d_waves, p->d_wparam, p->d_fold, vec, grad);
   HIP_TRY(hipGetLastError());
   return SEIS_OK;
+}
+
+void launch_misfit(seis_plan* p, hipStream_t s, double* out) {
+  hipLaunchKernelGGL(k_misfit_part, dim3(kRedBlocks, p->n_models), dim3(NTHR), 0, s, p->d_models,
+                     p->d_seis, p->d_dd, p->d_part);
+  hipLaunchKernelGGL(k_misfit_final, dim3((p->n_models + 63) / 64), dim3(64), 0, s, p->d_models,
+                     p->n_models, p->d_part, out);
 }
 
 int copy_out(seis_plan* p, hipStream_t s, const float* src, float* dst) {
@@ -898,7 +939,8 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
   auto bail = [&](int code) { plan_free(p); return code; };
   if ((rc = dev_alloc(&p->d_models, n_models)) || (rc = dev_alloc(&p->d_waves, p->n_waves)) ||
       (rc = dev_alloc(&p->d_params, n_models)) || (rc = dev_alloc(&p->d_wparam, p->n_waves)) ||
-      (rc = dev_alloc(&p->d_blk, L.n_blk)) || (rc = dev_alloc(&p->d_grad_off, n_models)) ||
+      (rc = dev_alloc(&p->d_blk, L.n_blk)) || (rc = dev_alloc(&p->d_fold, adj ? L.fold : 0)) ||
+      (rc = dev_alloc(&p->d_part, (int64_t)kRedBlocks * n_models)) ||
       (rc = dev_alloc(&p->d_alpha, L.cells)) || (rc = dev_alloc(&p->d_dalpha, tan ? L.cells : 0)) ||
       (rc = dev_alloc(&p->d_wavelet, p->nt)) || (rc = dev_alloc(&p->d_arena, L.arena)) ||
       (rc = dev_alloc(&p->d_seis, L.seis)) || (rc = dev_alloc(&p->d_dseis, tan ? L.seis : 0)) ||
@@ -906,12 +948,10 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
     return bail(rc);
   std::vector<float> wl(p->nt);
   for (int i = 0; i < p->nt; ++i) wl[i] = (float)time->wavelet[i];
-  std::vector<int64_t> goff(L.vec_off.begin(), L.vec_off.end() - 1);
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMemcpy(p->d_models, L.models.data(), sizeof(ModelDev) * n_models, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_waves, L.waves.data(), sizeof(WaveDev) * p->n_waves, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_blk, L.blk.data(), sizeof(int2) * L.n_blk, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->d_grad_off, goff.data(), sizeof(int64_t) * n_models, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_wavelet, wl.data(), sizeof(float) * p->nt, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->d_wparam, 0, sizeof(WaveParam) * p->n_waves);
   if (e == hipSuccess) e = hipMemset(p->d_params, 0, sizeof(ParamDev) * n_models);
@@ -979,8 +1019,7 @@ int seis_gradient(seis_plan_t* p, const float* vec, const float* dd, int32_t on_
   TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
   HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
   if (on_residual && misfit)
-    hipLaunchKernelGGL(k_misfit, dim3(p->n_models), dim3(1024), 0, s, p->d_models, p->d_seis,
-                       p->d_dd, misfit);
+    launch_misfit(p, s, misfit);
   const int kind = on_residual ? 0 : 1;
   TRY(get_graph(p, &p->g_adj[kind], [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
   HIP_TRY(hipGraphLaunch(p->g_adj[kind], s));

@@ -1,0 +1,321 @@
+"""Pins the FP64 oracle (restatement of core/seis_forward2.py, seis_numerics.py,
+seis_invert.py, seis_prior.py) before it is trusted as the parity reference.
+
+The reference's own tests need competition data that is absent here
+(core/seis_test.py:120), so each oracle function is pinned by closed forms,
+exact adjoint identities, brute force on tiny grids, physical symmetries and a
+library special case (torch.optim.LBFGS).  The reference's own recipes are
+restated where they apply (dot test and Taylor test, core/seis_test.py:26-44;
+prior and cost FD tests, :48-89)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd2d, invert, priors
+
+TINY = dict(nz=4, nx=5, nbc=3, nt=150)     # 10 x 11 padded, 2 shots
+SMALL = dict(nz=12, nx=16, nbc=8, nt=200)
+
+
+def _layered(g, seed=0):
+    rng = np.random.default_rng(seed)
+    v = np.repeat(rng.uniform(1500, 4500, (g.nz, 1)), g.nx, 1)
+    v += rng.uniform(-100, 100, (g.nz, g.nx))
+    return fd2d.default_vec(g, v)
+
+
+# ------------------------------------------------------------ closed forms
+def test_ricker_closed_form():
+    """core/seis_forward2.py:20-40 with f=15, dt=1e-3: nw=147, nc=74, peak 1 at k=nc."""
+    w = fd2d.ricker(15, 1e-3)
+    assert len(w) == 147
+    assert w[73] == 1.0                      # k = nc = 74 -> beta = 0
+    np.testing.assert_allclose(w[:73], w[74:][::-1], rtol=0, atol=1e-15)
+    k = 10
+    beta = ((74 - k) * 15 * 1e-3 * np.pi) ** 2
+    assert w[k - 1] == pytest.approx((1 - 2 * beta) * np.exp(-beta), rel=1e-14)
+    s = fd2d.source_wavelet(fd2d.Geometry())
+    assert len(s) == 999 and np.all(s[147:] == 0)
+
+
+def test_damping_profile_and_corners():
+    """AbcCoef2D (:46-59): kappa0 = 3 ln(1e7)/(2 (nbc-1) dx), quadratic, corners use x."""
+    g = fd2d.Geometry()
+    d = fd2d.damping(g)
+    k0 = 3 * np.log(1e7) / (2 * 1190.0)
+    assert k0 == pytest.approx(0.0203169, rel=1e-5)
+    assert d.shape == (310, 310)
+    assert d[0, 150] == pytest.approx(k0)            # outer top row, interior column
+    assert d[150, 0] == pytest.approx(k0)            # outer left column
+    assert d[119, 150] == 0.0 and d[150, 119] == 0.0  # innermost pad cell
+    assert np.all(d[120:190, 120:190] == 0)
+    # corner cell (z=5, x=3): x profile (j = 116), not the z profile (j = 114)
+    assert d[5, 3] == pytest.approx(k0 * (116 / 119) ** 2)
+    assert d[300, 305] == pytest.approx(k0 * (115 / 119) ** 2)
+
+
+def test_reference_geometry_indices():
+    """Shots at x=[0,17,34,52,69] on padded row 121; receivers row 121, cols 120..189."""
+    g = fd2d.Geometry()
+    assert g.src_x == (0, 17, 34, 52, 69)
+    assert g.src_cells() == [(121, 120), (121, 137), (121, 154), (121, 172), (121, 189)]
+    igz, igx = g.rcv_cells()
+    assert np.all(igz == 121) and np.array_equal(igx, np.arange(120, 190))
+    assert g.src_cells()[2][0] * 310 + g.src_cells()[2][1] == 121 * 310 + 154   # src_idx (:94)
+
+
+def test_prep_run_algebra():
+    g = fd2d.Geometry(**SMALL)
+    vec = _layered(g)
+    v, alpha, t1, t2 = fd2d.prep_run(vec, g)
+    damp = fd2d.damping(g)
+    kappa = vec[-1] * damp * g.dt
+    np.testing.assert_allclose(alpha, (v * g.dt / g.dx) ** 2, rtol=1e-15)
+    np.testing.assert_allclose(t1, 2 - 5 * alpha - kappa, rtol=1e-14)
+    np.testing.assert_allclose(t2, 1 - kappa, rtol=1e-15)
+    # edge padding
+    vv = vec[:-1].reshape(g.nz, g.nx)
+    assert v[0, 0] == vv[0, 0] and v[-1, -1] == vv[-1, -1] and v[g.nbc + 3, 0] == vv[3, 0]
+
+
+def test_prep_run_diff_is_derivative():
+    """prep_run_diff (:346-359) equals the central difference of prep_run."""
+    g = fd2d.Geometry(**SMALL)
+    vec = _layered(g, 1)
+    dvec = np.random.default_rng(2).normal(size=vec.shape)
+    v, _, _, _ = fd2d.prep_run(vec, g)
+    _, dt1, dt2, da = fd2d.prep_run_diff(dvec, v, g)
+    h = 1e-3
+    p = fd2d.prep_run(vec + h * dvec, g)
+    m = fd2d.prep_run(vec - h * dvec, g)
+    for ana, i in [(da, 1), (dt1, 2), (dt2, 3)]:
+        np.testing.assert_allclose(ana, (p[i] - m[i]) / (2 * h), rtol=1e-7, atol=1e-12)
+
+
+def test_unpad_is_adjoint_of_edge_pad():
+    """<pad_edge(x), y> == <x, unpad(y)> (core/seis_numerics.py:14-79)."""
+    rng = np.random.default_rng(3)
+    for nz, nx, nbc in [(7, 9, 4), (2, 3, 5), (70, 70, 120)]:
+        x = rng.normal(size=(nz, nx))
+        y = rng.normal(size=(nz + 2 * nbc, nx + 2 * nbc))
+        a = np.sum(np.pad(x, nbc, mode="edge") * y)
+        b = np.sum(x * fd2d.unpad_edge_padded_gradient(y, nbc))
+        assert a == pytest.approx(b, rel=1e-12)
+
+
+def test_first_steps_closed_form():
+    """p[2] = s_mod[0] delta_src; p[3] = temp1 p[2] + alpha lapg(p[2]) + s_mod[1] delta_src.
+    Pins the injection amplitude (v_src dt)^2 (:206), the output index t <-> p[t+2]
+    (:249) and the stencil coefficients c2, c3 (:80-81)."""
+    g = fd2d.Geometry(nz=10, nx=10, nbc=6, nt=150, n_shots=1, src_x=(4,), rcv_x=tuple(range(10)))
+    vec = _layered(g, 4)
+    v, alpha, t1, t2 = fd2d.prep_run(vec, g)
+    s = fd2d.source_wavelet(g)
+    seis = fd2d.vel_to_seis(vec, g)[0].reshape(g.nt, g.n_rcv)
+    zs, xs = g.src_cells()[0]
+    amp = (v[zs, xs] * g.dt) ** 2
+    p2 = amp * s[0]
+    # receivers share the source row: sample t=0 is p[2] on that row
+    assert seis[0, 4] == pytest.approx(p2, rel=1e-15)
+    assert np.all(seis[0, np.arange(10) != 4] == 0)
+    p3_src = t1[zs, xs] * p2 + amp * s[1]
+    assert seis[1, 4] == pytest.approx(p3_src, rel=1e-13)
+    assert seis[1, 5] == pytest.approx(alpha[zs, xs + 1] * (4 / 3) * p2, rel=1e-13)
+    assert seis[1, 6] == pytest.approx(alpha[zs, xs + 2] * (-1 / 12) * p2, rel=1e-13)
+    assert seis[1, 7] == 0.0
+
+
+def test_homogeneous_arrival_time():
+    """Direct-wave moveout between two far-field receivers is their offset
+    difference over v (pins the (dt/dx)^2 scaling of alpha; the 2-D phase lag
+    cancels in the difference)."""
+    g = fd2d.Geometry(nz=30, nx=120, nbc=30, nt=650, n_shots=1, src_x=(10,))
+    for v in (2000.0, 3000.0):
+        vec = fd2d.default_vec(g, np.full((g.nz, g.nx), v))
+        seis = fd2d.vel_to_seis(vec, g)[0].reshape(g.nt, g.n_rcv)
+        t60, t100 = np.argmax(seis[:, 60]), np.argmax(seis[:, 100])
+        expected = 40 * g.dx / v / g.dt
+        assert abs((t100 - t60) - expected) <= 2, (v, t100 - t60, expected)
+
+
+# ------------------------------------------------------------ adjoint identities
+def test_dot_product_reference_recipe():
+    """core/seis_test.py:26-38: dm = 1e-4 N(0,1) seed 0, y = data + N(0,1) seed 0;
+    |<J dm, s-y> - <dm, J^T (s-y)>| / |<J dm, s-y>| < 1e-10 in FP64."""
+    g = fd2d.Geometry(**SMALL)
+    vec = _layered(g, 5)
+    data = fd2d.vel_to_seis(fd2d.default_vec(g, vec[:-1].reshape(g.nz, g.nx) * 1.03), g)[0]
+    dm = 1e-4 * np.random.default_rng(seed=0).normal(0, 1, vec.shape)
+    y = data + np.random.default_rng(seed=0).normal(0, 1, data.shape)
+    s, ds, gr = fd2d.vel_to_seis(vec, g, vec_diff=dm, vec_adjoint=y, adjoint_on_residual=True)
+    a = np.sum((s - y) * ds)
+    b = np.sum(gr * dm)
+    assert abs(a - b) / abs(a) < 1e-10
+
+
+def test_taylor_reference_recipe():
+    """core/seis_test.py:41-44: rms(s(v+dm)-s(v)-J dm)/rms(s(v+dm)-s(v)) < 1e-5."""
+    g = fd2d.Geometry(**SMALL)
+    vec = _layered(g, 6)
+    dm = 1e-4 * np.random.default_rng(seed=0).normal(0, 1, vec.shape)
+    s, ds, _ = fd2d.vel_to_seis(vec, g, vec_diff=dm)
+    s2 = fd2d.vel_to_seis(vec + dm, g)[0]
+    rms = lambda a: np.sqrt(np.mean(a ** 2))
+    assert rms(s2 - s - ds) / rms(s2 - s) < 1e-5
+
+
+def test_brute_force_jacobian_tiny():
+    """Column-by-column central-difference Jacobian (every velocity and min_vel)
+    against the tangent pass, and J^T e_k against the adjoint pass."""
+    g = fd2d.Geometry(**TINY, n_shots=2)
+    vec = _layered(g, 7)
+    n = len(vec)
+    h = 1e-2
+    J_fd = np.stack([(fd2d.vel_to_seis(vec + h * e, g)[0] - fd2d.vel_to_seis(vec - h * e, g)[0]) / (2 * h)
+                     for e in np.eye(n)], axis=1)
+    J_tan = np.stack([fd2d.vel_to_seis(vec, g, vec_diff=e)[1] for e in np.eye(n)], axis=1)
+    np.testing.assert_allclose(J_tan, J_fd, rtol=0, atol=1e-6 * np.abs(J_fd).max())
+    rng = np.random.default_rng(8)
+    w = rng.normal(size=J_fd.shape[0])
+    jt = fd2d.vel_to_seis(vec, g, vec_adjoint=w)[2]
+    np.testing.assert_allclose(jt, J_tan.T @ w, rtol=1e-10, atol=1e-12 * np.abs(jt).max())
+
+
+# ------------------------------------------------------------ symmetries
+def test_mirror_symmetry():
+    """Laterally symmetric v: shot 0 == flip_x(shot 4) (used by core/seis_model.py:155)."""
+    g = fd2d.Geometry(**SMALL)
+    rng = np.random.default_rng(9)
+    half = rng.uniform(1500, 4500, (g.nz, g.nx // 2))
+    v = np.concatenate([half, half[:, ::-1]], axis=1)
+    assert g.src_x == (0, 4, 8, 11, 15)
+    seis = fd2d.vel_to_seis(fd2d.default_vec(g, v), g)[0].reshape(g.n_shots, g.nt, g.n_rcv)
+    np.testing.assert_allclose(seis[0], seis[4][:, ::-1], rtol=0, atol=1e-12 * np.abs(seis).max())
+    np.testing.assert_allclose(seis[1], seis[3][:, ::-1], rtol=0, atol=1e-12 * np.abs(seis).max())
+
+
+def test_reciprocity():
+    """Source and receivers on one row: seis_i[t, x_j] == seis_j[t, x_i]
+    (symmetric spatial operator after scaling by 1/alpha, source scaled by alpha dx^2)."""
+    g = fd2d.Geometry(**SMALL)
+    vec = _layered(g, 10)
+    seis = fd2d.vel_to_seis(vec, g)[0].reshape(g.n_shots, g.nt, g.n_rcv)
+    xs = g.src_x
+    scale = np.abs(seis).max()
+    for i in range(g.n_shots):
+        for j in range(g.n_shots):
+            np.testing.assert_allclose(seis[i, :, xs[j]], seis[j, :, xs[i]], rtol=0, atol=1e-10 * scale)
+
+
+# ------------------------------------------------------------ priors
+@pytest.mark.parametrize("cls", [priors.RowTotalVariation, priors.TotalVariation, priors.L2Smoothness])
+def test_prior_gradient_fd(cls):
+    """core/seis_test.py:48-69: <g, d> vs cost(x+d)-cost(x), d = 1e-6 N(0,1), rel < 1e-6."""
+    p = cls(12, 16, lam=15.0)
+    x = np.random.default_rng(seed=0).normal(0, 1, p.N)
+    d = 1e-6 * np.random.default_rng(seed=0).normal(0, 1, p.N)
+    c, gr = p.cost_grad(x)
+    c2, _ = p.cost_grad(x + d)
+    assert abs(gr @ d - (c2 - c)) / abs(c2 - c) < 1e-6
+
+
+def test_row_tv_basis():
+    p = priors.RowTotalVariation(12, 16)
+    assert p.basis.shape == (12 * 16 + 1, 13)
+    x = np.arange(13.0)
+    v = p.basis @ x
+    assert np.all(v[:-1].reshape(12, 16) == np.arange(12.0)[:, None]) and v[-1] == 12.0
+
+
+def test_l2_smoothness_closed_form():
+    p = priors.L2Smoothness(5, 6, lam=2.0)
+    xm = np.random.default_rng(1).normal(size=(5, 6))
+    x = np.concatenate([xm.ravel(), [3.0]])
+    ref = 2.0 * (np.sum(np.diff(xm, axis=0) ** 2) + np.sum(np.diff(xm, axis=1) ** 2))
+    assert p.cost_grad(x)[0] == pytest.approx(ref, rel=1e-13)
+
+
+def test_cost_and_gradient_fd():
+    """core/seis_test.py:71-89 with the RowTV prior, lam 1e-9, dx = 1e-4 N(0,1): rel < 1e-4."""
+    g = fd2d.Geometry(**SMALL)
+    p = priors.RowTotalVariation(g.nz, g.nx, lam=1e-9)
+    target = fd2d.vel_to_seis(_layered(g, 11), g)[0]
+    base = 3000 + 100 * np.random.default_rng(seed=0).normal(0, 1, p.N)
+    base[-1] = 1800
+    off = 1e-4 * np.random.default_rng(seed=0).normal(0, 1, p.N)
+    c2 = invert.cost_and_gradient(base + off, target, p, p.basis, g)[0]
+    c, gr, _, _ = invert.cost_and_gradient(base, target, p, p.basis, g, compute_gradient=True)
+    assert abs(gr @ off - (c2 - c)) / abs(c2 - c) < 1e-4
+
+
+# ------------------------------------------------------------ optimisers
+def _rosen(x):
+    f = np.sum(100 * (x[1:] - x[:-1] ** 2) ** 2 + (1 - x[:-1]) ** 2)
+    g = np.zeros_like(x)
+    g[:-1] = -400 * x[:-1] * (x[1:] - x[:-1] ** 2) - 2 * (1 - x[:-1])
+    g[1:] += 200 * (x[1:] - x[:-1] ** 2)
+    return f, g
+
+
+@pytest.mark.parametrize("max_iter,hist", [(30, None), (60, None), (40, 10)])
+def test_bfgs_matches_torch_lbfgs(max_iter, hist):
+    """bfgs is a port of torch 2.7 LBFGS (core/seis_numerics.py:126-129): same
+    iterates as torch.optim.LBFGS(lr=1, line_search_fn='strong_wolfe') in FP64.
+    The installed torch adds a g.d stopping test and caps each line search at
+    the remaining evaluation budget; bfgs reproduces both with torch_compat=True
+    and without it follows the reference wherever they do not bind."""
+    x0 = np.array([-1.2, 1.0, -0.5, 0.8, 1.3, -0.7])
+    x_ref = torch.tensor(x0, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([x_ref], lr=1, max_iter=max_iter, tolerance_grad=1e-7,
+                            tolerance_change=1e-9, history_size=hist or 10 ** 6,
+                            line_search_fn="strong_wolfe")
+
+    def closure():
+        opt.zero_grad()
+        f, g = _rosen(x_ref.detach().numpy())
+        x_ref.grad = torch.tensor(g)
+        return torch.tensor(f)
+
+    opt.step(closure)
+    x, _ = invert.bfgs(_rosen, x0, max_iter, 1e-7, 1e-9, history_size=hist, torch_compat=True)
+    np.testing.assert_allclose(x, x_ref.detach().numpy(), rtol=1e-8, atol=1e-9)
+    if max_iter <= 30:  # neither torch-only rule binds here: reference semantics agree
+        x25, _ = invert.bfgs(_rosen, x0, max_iter, 1e-7, 1e-9, history_size=hist)
+        np.testing.assert_array_equal(x25, x)
+
+
+def test_cg_reproduces_dense_solve():
+    rng = np.random.default_rng(12)
+    M = rng.normal(size=(15, 15))
+    A = M @ M.T + 15 * np.eye(15)
+    b = rng.normal(size=15)
+    x = invert.conjugate_gradient(lambda v: A @ v, b, 15)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9)
+
+
+def test_gn_cg_equals_dense_gn_small_basis():
+    """With CG steps >= basis size, GN-CG reproduces the reference's dense GN pass
+    (core/seis_invert.py:63-107) on a RowTV-style basis (n = nz+1)."""
+    g = fd2d.Geometry(nz=6, nx=8, nbc=5, nt=160)
+    p = priors.L2Smoothness(g.nz, g.nx, lam=1e-9)
+    # restrict to the RowTV basis so the dense J is cheap; P in that basis
+    rt = priors.RowTotalVariation(g.nz, g.nx)
+    B = rt.basis
+
+    class RowL2:
+        basis = B
+        lam = 1e-9
+        P = (B[:-1, :-1].T @ p.P @ B[:-1, :-1])
+
+        def cost_grad(self, x):
+            xv = x[:-1]
+            return self.lam * xv @ self.P @ xv, np.concatenate([2 * self.lam * self.P @ xv, [0.0]])
+
+    prior = RowL2()
+    true = fd2d.default_vec(g, np.repeat(np.linspace(1800, 3500, g.nz)[:, None], g.nx, 1))
+    target = fd2d.vel_to_seis(true, g)[0]
+    start = B @ np.concatenate([np.linspace(2000, 3000, g.nz), [1800.0]])
+    v_dense, s_dense, _ = invert.gauss_newton_dense(start, target, prior, g, n_scales=11)
+    v_cg, s_cg, _ = invert.gauss_newton_cg(start, target, prior, g, n_cg=B.shape[1] + 4, n_scales=11)
+    assert s_dense == s_cg
+    np.testing.assert_allclose(v_cg, v_dense, rtol=1e-6)
