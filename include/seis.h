@@ -110,6 +110,12 @@ int seis_set_active(seis_plan_t* plan, const int32_t* active /* host */);
 int seis_forward(seis_plan_t* plan, const float* vec, float* seis,
                  seis_stream_t stream);
 
+/* Forward + data misfit: seis = F(vec), misfit[m] = sum((seis - dd)^2) per
+ * model (the forward-only cost of core/seis_invert.py:38-41, used by the
+ * Gauss-Newton line search :95-101).  seis may be NULL. */
+int seis_forward_misfit(seis_plan_t* plan, const float* vec, const float* dd,
+                        float* seis, double* misfit, seis_stream_t stream);
+
 /* Forward + tangent: seis = F(vec), dseis = J(vec) dvec.
  * (vel_to_seis with vec_diff, :252-283) */
 int seis_tangent(seis_plan_t* plan, const float* vec, const float* dvec,
@@ -129,8 +135,28 @@ int seis_gradient(seis_plan_t* plan, const float* vec, const float* dd,
 int seis_gn_product(seis_plan_t* plan, const float* vec, const float* dvec,
                     float* dseis, double* grad, seis_stream_t stream);
 
-/* Number of kernel launches recorded in each captured time-loop graph so far
- * (diagnostics) and the plan's device bytes. */
+/* Kernel kinds with a tunable launch configuration. */
+enum {
+  SEIS_KIND_FORWARD = 0, /* forward step kernel (also segment recompute)     */
+  SEIS_KIND_TANGENT = 1, /* fused forward + tangent step kernel               */
+  SEIS_KIND_ADJOINT = 2, /* adjoint step kernel with fused imaging            */
+  SEIS_KIND_COUNT = 3
+};
+
+/* Launch configuration of one kernel kind: an rx x rz cell window (halo
+ * included) is staged in LDS per workgroup and advanced k time steps per
+ * launch; the window's interior, (rz-4k) x (rx-4k) cells, is written back.
+ * Supported windows: 64x32, 64x64, 128x32; 1 <= k <= 8 with an interior of
+ * at least 4 rows and 8 columns.  Results do not depend on the config
+ * (bit-identical wavefields).  No reference counterpart: the reference runs
+ * one 32x32-thread launch per step (core/seis_forward2.py:191-193).
+ * Setting a config synchronizes the device and drops captured graphs. */
+int seis_plan_set_config(seis_plan_t* plan, int32_t kind, int32_t rx, int32_t rz,
+                         int32_t k);
+int seis_plan_get_config(const seis_plan_t* plan, int32_t kind, int32_t* rx,
+                         int32_t* rz, int32_t* k /* host */);
+
+/* Device bytes held by the plan. */
 int64_t seis_plan_device_bytes(const seis_plan_t* plan);
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -10,6 +10,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libseis.so"
 CAP_FORWARD = 1
 CAP_TANGENT = 2
 CAP_ADJOINT = 4
+KIND_FORWARD, KIND_TANGENT, KIND_ADJOINT = 0, 1, 2
 MAX_SHOTS = 16
 
 ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "HIP error",
@@ -34,11 +35,14 @@ EXPORTS = {
     "seis_plan_layout": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "seis_set_active": (c_int32, [c_void_p, POINTER(c_int32)]),
     "seis_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "seis_forward_misfit": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_tangent": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_gradient": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                                 c_void_p]),
     "seis_gn_product": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_plan_device_bytes": (c_int64, [c_void_p]),
+    "seis_plan_set_config": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32]),
+    "seis_plan_get_config": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "seis_last_error": (c_char_p, []),
 }
 

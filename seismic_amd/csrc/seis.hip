@@ -42,14 +42,8 @@ namespace {
 constexpr float kC2 = 4.0f / 3.0f;    // core/seis_forward2.py:80
 constexpr float kC3 = -1.0f / 12.0f;  // core/seis_forward2.py:81
 
-constexpr int TX = 64;           // tile width (contiguous x), one wavefront row
-constexpr int TZ = 16;           // tile height (z)
-constexpr int NTHR = 256;        // 4 wavefronts
-constexpr int RZ = TZ / (NTHR / TX);  // rows per thread (4)
-constexpr int HALO = 2;
-constexpr int LW = TX + 2 * HALO;  // 68
-constexpr int LH = TZ + 2 * HALO;  // 20
-constexpr int NXCD = 8;            // MI355X: 8 XCDs, blocks dealt round-robin
+constexpr int NTHR = 256;        // 4 wavefronts per workgroup
+constexpr int NXCD = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
 
 // Per-wavefield state regions in the float arena.
 enum Region {
@@ -132,9 +126,12 @@ __device__ __forceinline__ int damp_j(int z, int x, int nz, int nx, int nbc) {
   return 0;
 }
 
-// Lf(p) at tile position (r, c) (centre at T[r][c]): lapg(p) + 2*c1*p written
-// as neighbour-minus-centre differences, periodic wrap handled by the loader.
-__device__ __forceinline__ float lap_full(const float (*T)[LW], int r, int c) {
+// Lf(p) at (r, c) of an LDS window with row length W: lapg(p) + 2*c1*p written
+// as neighbour-minus-centre differences (exact cancellation of the centre
+// weight, 4*c2 + 4*c3 = -2*c1).  Fixed evaluation order: every kernel that
+// advances a wavefield shares it, so results do not depend on tiling.
+template <int W>
+__device__ __forceinline__ float lap_full(const float (*T)[W], int r, int c) {
   const float pc = T[r][c];
   float t1 = (T[r][c + 1] - pc) + (T[r][c - 1] - pc);
   t1 = t1 + (T[r + 1][c] - pc);
@@ -145,51 +142,18 @@ __device__ __forceinline__ float lap_full(const float (*T)[LW], int r, int c) {
   return fmaf(kC2, t1, kC3 * t2);
 }
 
-// Stage the (TZ+4) x (TX+4) window around tile (z0, x0) of field F into LDS,
-// wrapping periodically (core/seis_forward2.py:395-403).
-__device__ __forceinline__ void load_tile(float (*T)[LW], const float* __restrict__ F,
-                                          int NZ, int NX, int z0, int x0) {
-  for (int i = threadIdx.x; i < LH * LW; i += NTHR) {
-    const int r = i / LW, c = i - (i / LW) * LW;
-    const int gz = wrapi(z0 + r - HALO, NZ);
-    const int gx = wrapi(x0 + c - HALO, NX);
-    T[r][c] = F[(int64_t)gz * NX + gx];
-  }
-}
-
-// Same, for the product alpha*lam (adjoint stencil, :567-575).
-__device__ __forceinline__ void load_tile_prod(float (*T)[LW], const float* __restrict__ A,
-                                               const float* __restrict__ L, int NZ, int NX,
-                                               int z0, int x0) {
-  for (int i = threadIdx.x; i < LH * LW; i += NTHR) {
-    const int r = i / LW, c = i - (i / LW) * LW;
-    const int gz = wrapi(z0 + r - HALO, NZ);
-    const int gx = wrapi(x0 + c - HALO, NX);
-    const int64_t g = (int64_t)gz * NX + gx;
-    T[r][c] = A[g] * L[g];
-  }
-}
-
-struct BlockInfo {
-  int w, z0, x0;
+// Tile of the compacted work list: one (wavefield, interior origin) pair.
+// Interior origins are unwrapped padded-grid coordinates; the window staged
+// in LDS extends 2K cells beyond the interior on every side and wraps
+// periodically (core/seis_forward2.py:395-403).
+struct TileEnt {
+  int w, z0, x0, pad;
 };
 
-__device__ __forceinline__ bool block_info(const int2* __restrict__ blk, const WaveDev* waves,
-                                           const ModelDev* models, BlockInfo& bi) {
-  const int2 e = blk[blockIdx.x];
-  if (e.x < 0) return false;
-  const WaveDev& w = waves[e.x];
-  if (!models[w.model].active) return false;
-  const int tz = e.y / w.tiles_x;
-  bi.w = e.x;
-  bi.z0 = tz * TZ;
-  bi.x0 = (e.y - tz * w.tiles_x) * TX;
-  return true;
-}
-
-// ---------------------------------------------------------------- kernels
+// ---------------------------------------------------------------- forward (+ tangent)
 struct FwdArgs {
-  const int2* blk;
+  const TileEnt* tiles;
+  const int* n_tiles;
   const WaveDev* waves;
   const ModelDev* models;
   const ParamDev* params;
@@ -200,102 +164,183 @@ struct FwdArgs {
   float* arena;
   float* seis;
   float* dseis;
-  int it;                // time step: produces p[it+2]
-  int in_reg, in_slot, out_reg, out_slot;  // where p[n] is read / p[n+1] written
-  int dp_in_slot, dp_out_slot;             // tangent ping-pong slots
-  int ck_p_slot, ck_q_slot;                // checkpoint saves (-1 = none)
+  int nt;                // time samples
+  int n0, K;             // advance state n0 -> n0+K; step n produces p[n+1] with s[n-1]
+  int Kt;                // steps the tile list was built for (halo 2*Kt); K <= Kt
+  int p_in_reg, p_in_slot;
+  int p_out_reg, p_out_slot0, hist_all;  // hist_all: p[n0+s+1] -> slot p_out_slot0+s
+  int q_in_slot, q_out_slot;
+  int dp_in_slot, dp_out_slot, dq_in_slot, dq_out_slot;
+  int ck_slot;           // >= 0: also store the final (p, q) as checkpoint ck_slot
   int write_seis;
 };
 
-// One time step of the forward (and, with TAN, the fused tangent) recursion
-// for every (model, shot) tile of the batch.  update_p :428-462 and
-// update_p_diff :470-509 with lapg :381-421 fused in.
-template <bool TAN>
-__global__ __launch_bounds__(NTHR) void k_fwd_step(FwdArgs a) {
-  __shared__ float Tp[LH][LW];
-  __shared__ float Td[TAN ? LH : 1][TAN ? LW : 1];
-  BlockInfo bi;
-  if (!block_info(a.blk, a.waves, a.models, bi)) return;
-  const WaveDev& w = a.waves[bi.w];
-  const ModelDev& m = a.models[w.model];
-  const int NZ = w.NZ, NX = w.NX;
-  const int64_t P = (int64_t)NZ * NX;
-  const float* pin = a.arena + w.off[a.in_reg] + a.in_slot * P;
-  float* pout = a.arena + w.off[a.out_reg] + a.out_slot * P;
-  float* q = a.arena + w.off[R_Q];
-  load_tile(Tp, pin, NZ, NX, bi.z0, bi.x0);
-  const float* dpin = nullptr;
-  float* dpout = nullptr;
-  float* dq = nullptr;
-  if constexpr (TAN) {
-    dpin = a.arena + w.off[R_DP] + a.dp_in_slot * P;
-    dpout = a.arena + w.off[R_DP] + a.dp_out_slot * P;
-    dq = a.arena + w.off[R_DQ];
-    load_tile(Td, dpin, NZ, NX, bi.z0, bi.x0);
-  }
-  __syncthreads();
-
-  const float kscale = a.params[w.model].kscale;
-  const float dkscale = TAN ? a.params[w.model].dkscale : 0.f;
-  const float s_t = a.wavelet[a.it];
-  const float src_add = a.wparam[bi.w].amp * s_t;
-  const float src_add_d = TAN ? a.wparam[bi.w].amp_d * s_t : 0.f;
-  const float* alpha = a.alpha + m.cell_off;
-  const float* dalpha = TAN ? a.dalpha + m.cell_off : nullptr;
-  float* ckp = a.ck_p_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_p_slot * P : nullptr;
-  float* ckq = a.ck_q_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_q_slot * P : nullptr;
-
-  const int tx = threadIdx.x % TX;
-  const int ty = threadIdx.x / TX;
-  const int x = bi.x0 + tx;
-  if (x >= NX) return;
-#pragma unroll
-  for (int k = 0; k < RZ; ++k) {
-    const int r = ty * RZ + k;
-    const int z = bi.z0 + r;
-    if (z >= NZ) break;
-    const int64_t g = (int64_t)z * NX + x;
-    const int j = damp_j(z, x, m.nz, m.nx, m.nbc);
-    const float jj = (float)(j * j);
-    const float k1 = 1.0f - kscale * jj;
-    const float L = lap_full(Tp, r + HALO, tx + HALO);
-    const float pc = Tp[r + HALO][tx + HALO];
-    const float qo = q[g];
-    float qn = fmaf(k1, qo, alpha[g] * L);
-    if (g == w.src_cell) qn = qn + src_add;
-    const float pn = pc + qn;
-    q[g] = qn;
-    pout[g] = pn;
-    if (ckp) ckp[g] = pn;
-    if (ckq) ckq[g] = qn;
+// K forward time steps (update_p :428-462 with lapg :381-421 and the source
+// injection fused), and with TAN the Born recursion (update_p_diff :470-509),
+// for every tile of the work list.  Thread-owned cells keep q, alpha and
+// (1-kappa) in registers across the K steps; p (and dp) live in LDS.
+template <int RX, int RZ, bool TAN>
+__global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
+  constexpr int CPR = RX / 64, RPW = RZ / 4, NC = CPR * RPW;
+  static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
+  __shared__ float P[RZ][RX];
+  __shared__ float D[TAN ? RZ : 1][TAN ? RX : 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ntiles = *a.n_tiles;
+  const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
+  for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    const TileEnt te = a.tiles[e];
+    if (te.w < 0) continue;
+    const WaveDev& w = a.waves[te.w];
+    const ModelDev& m = a.models[w.model];
+    const int NZ = w.NZ, NX = w.NX;
+    const int64_t Pn = (int64_t)NZ * NX;
+    const float* pin = a.arena + w.off[a.p_in_reg] + a.p_in_slot * Pn;
+    const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
+    float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
+    float* pbase = a.arena + w.off[a.p_out_reg];
+    const float* alpha = a.alpha + m.cell_off;
+    const float kscale = a.params[w.model].kscale;
+    const float amp = a.wparam[te.w].amp;
+    const int oz = te.z0 - H, ox = te.x0 - H;
+    float q[NC], al[NC], k1[NC], pn[NC];
+    float dq[TAN ? NC : 1], dal[TAN ? NC : 1], dk[TAN ? NC : 1], dpn[TAN ? NC : 1];
+    const float* dpin = nullptr;
+    const float* dqin = nullptr;
+    float dkscale = 0.f, amp_d = 0.f;
     if constexpr (TAN) {
-      const float Ld = lap_full(Td, r + HALO, tx + HALO);
-      const float dpc = Td[r + HALO][tx + HALO];
-      const float dqo = dq[g];
-      // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
-      float dqn = fmaf(k1, dqo, alpha[g] * Ld);
-      dqn = fmaf(dalpha[g], L, dqn);
-      dqn = fmaf(-(dkscale * jj), qo, dqn);
-      if (g == w.src_cell) dqn = dqn + src_add_d;
-      const float dpn = dpc + dqn;
-      dq[g] = dqn;
-      dpout[g] = dpn;
-      if (a.write_seis && z == m.rcv_row) {
-        const int d = x - m.rcv_x0;
-        if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv)
-          a.dseis[w.seis_off + (int64_t)a.it * m.n_rcv + d / m.rcv_dx] = dpn;
+      dpin = a.arena + w.off[R_DP] + a.dp_in_slot * Pn;
+      dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
+      dkscale = a.params[w.model].dkscale;
+      amp_d = a.wparam[te.w].amp_d;
+    }
+    unsigned srcmask = 0, rcvmask = 0;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + 4 * k;
+      const int gz = wrapi(oz + r, NZ);
+      if (gz == m.rcv_row) rcvmask |= 1u << k;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int i = k * CPR + c;
+        const int cc = lane + 64 * c;
+        const int gx = wrapi(ox + cc, NX);
+        const int64_t g = (int64_t)gz * NX + gx;
+        P[r][cc] = pin[g];
+        q[i] = qin[g];
+        al[i] = alpha[g];
+        const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
+        const float jj = (float)(j * j);
+        k1[i] = 1.0f - kscale * jj;
+        if (g == w.src_cell) srcmask |= 1u << i;
+        if constexpr (TAN) {
+          D[r][cc] = dpin[g];
+          dq[i] = dqin[g];
+          dal[i] = a.dalpha[m.cell_off + g];
+          dk[i] = dkscale * jj;
+        }
       }
     }
-    if (a.write_seis && z == m.rcv_row) {
-      const int d = x - m.rcv_x0;
-      if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv)
-        a.seis[w.seis_off + (int64_t)a.it * m.n_rcv + d / m.rcv_dx] = pn;
+    __syncthreads();
+    for (int s = 0; s < a.K; ++s) {
+      const int n = a.n0 + s;
+      const int it = n - 1;
+      const bool live = it >= 0 && it < a.nt;
+      const float sv = live ? a.wavelet[it] : 0.f;
+      const float src = amp * sv;
+      const float srcd = amp_d * sv;
+      const int lo = 2 * (s + 1);
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + 4 * k;
+        if (r < lo || r >= RZ - lo) continue;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int i = k * CPR + c;
+          const int cc = lane + 64 * c;
+          if (cc < lo || cc >= RX - lo) continue;
+          const float L = lap_full<RX>(P, r, cc);
+          const float pc = P[r][cc];
+          float qn = fmaf(k1[i], q[i], al[i] * L);
+          if ((srcmask >> i) & 1u) qn = qn + src;
+          if constexpr (TAN) {
+            // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
+            const float Ld = lap_full<RX>(D, r, cc);
+            float dqn = fmaf(k1[i], dq[i], al[i] * Ld);
+            dqn = fmaf(dal[i], L, dqn);
+            dqn = fmaf(-dk[i], q[i], dqn);
+            if ((srcmask >> i) & 1u) dqn = dqn + srcd;
+            dpn[i] = D[r][cc] + dqn;
+            dq[i] = dqn;
+          }
+          pn[i] = pc + qn;
+          q[i] = qn;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + 4 * k;
+        if (r < lo || r >= RZ - lo) continue;
+        const int zz = oz + r;
+        const bool zin = r >= H && r < RZ - H && zz < NZ;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int i = k * CPR + c;
+          const int cc = lane + 64 * c;
+          if (cc < lo || cc >= RX - lo) continue;
+          P[r][cc] = pn[i];
+          if constexpr (TAN) D[r][cc] = dpn[i];
+          const int xx = ox + cc;
+          if (!(zin && cc >= H && cc < RX - H && xx < NX)) continue;
+          const int64_t g = (int64_t)zz * NX + xx;
+          if (a.hist_all) pbase[(a.p_out_slot0 + s) * Pn + g] = pn[i];
+          if (a.write_seis && live && ((rcvmask >> k) & 1u)) {
+            const int d = xx - m.rcv_x0;
+            if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv) {
+              const int64_t si = w.seis_off + (int64_t)it * m.n_rcv + d / m.rcv_dx;
+              a.seis[si] = pn[i];
+              if constexpr (TAN) a.dseis[si] = dpn[i];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // state (p, q[, dp, dq]) at n0+K on the interior
+    float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
+    float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + 4 * k;
+      const int zz = oz + r;
+      if (r < H || r >= RZ - H || zz >= NZ) continue;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int i = k * CPR + c;
+        const int cc = lane + 64 * c;
+        const int xx = ox + cc;
+        if (cc < H || cc >= RX - H || xx >= NX) continue;
+        const int64_t g = (int64_t)zz * NX + xx;
+        qout[g] = q[i];
+        if (!a.hist_all) pbase[a.p_out_slot0 * Pn + g] = pn[i];
+        if (ckp) {
+          ckp[g] = pn[i];
+          ckq[g] = q[i];
+        }
+        if constexpr (TAN) {
+          a.arena[w.off[R_DP] + a.dp_out_slot * Pn + g] = dpn[i];
+          a.arena[w.off[R_DQ] + a.dq_out_slot * Pn + g] = dq[i];
+        }
+      }
     }
   }
 }
 
+// ---------------------------------------------------------------- adjoint
 struct AdjArgs {
-  const int2* blk;
+  const TileEnt* tiles;
+  const int* n_tiles;
   const WaveDev* waves;
   const ModelDev* models;
   const ParamDev* params;
@@ -305,101 +350,198 @@ struct AdjArgs {
   float* arena;
   const float* inj_a;   // adjoint source = inj_a - inj_b (inj_b may be null)
   const float* inj_b;
-  int m;                // produces lam[m] (m = nt+1 .. 2)
-  int lam_in_slot, lam_out_slot;
-  int p_reg, p1_slot, p0_slot;  // p[m-1], p[m-2]
+  int nt;
+  int M0, K;            // produces lam[M0-1] .. lam[M0-K]
+  int Kt;               // steps the tile list was built for (halo 2*Kt); K <= Kt
+  int lam_in_slot, lam_out_slot, mu_in_slot, mu_out_slot;
+  int p_reg, p_base;    // forward p[n] sits at slot n - p_base of region p_reg
 };
 
-// One reverse step: update_p_adjoint (:517-579) in gather form, with the
-// imaging accumulators and the source-wavelet correlation (:314) fused.
-__global__ __launch_bounds__(NTHR) void k_adj_step(AdjArgs a) {
-  __shared__ float Ta[LH][LW];  // alpha*lam[m+1]
-  __shared__ float Tp[LH][LW];  // p[m-1]
-  BlockInfo bi;
-  if (!block_info(a.blk, a.waves, a.models, bi)) return;
-  const WaveDev& w = a.waves[bi.w];
-  const ModelDev& m = a.models[w.model];
-  const int NZ = w.NZ, NX = w.NX;
-  const int64_t P = (int64_t)NZ * NX;
-  const float* alpha = a.alpha + m.cell_off;
-  const float* lin = a.arena + w.off[R_LAM] + a.lam_in_slot * P;
-  float* lout = a.arena + w.off[R_LAM] + a.lam_out_slot * P;
-  float* mu = a.arena + w.off[R_MU];
-  float* GA = a.arena + w.off[R_GA];
-  float* GK = a.arena + w.off[R_GK];
-  const float* p1 = a.arena + w.off[a.p_reg] + a.p1_slot * P;
-  const float* p0 = a.arena + w.off[a.p_reg] + a.p0_slot * P;
-  load_tile_prod(Ta, alpha, lin, NZ, NX, bi.z0, bi.x0);
-  load_tile(Tp, p1, NZ, NX, bi.z0, bi.x0);
-  __syncthreads();
-
-  const float kscale = a.params[w.model].kscale;
-  const int tsrc = a.m - 2;
-  const int tx = threadIdx.x % TX;
-  const int ty = threadIdx.x / TX;
-  const int x = bi.x0 + tx;
-  if (x >= NX) return;
+// Stage the (rows x cols) window at unwrapped origin (z0, x0) of F into T.
+template <int RX, int RZ>
+__device__ __forceinline__ void load_window(float (*T)[RX], const float* __restrict__ F, int NZ,
+                                            int NX, int z0, int x0, int rows, int cols) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < RZ; ++k) {
-    const int r = ty * RZ + k;
-    const int z = bi.z0 + r;
-    if (z >= NZ) break;
-    const int64_t g = (int64_t)z * NX + x;
-    const int j = damp_j(z, x, m.nz, m.nx, m.nbc);
-    const float k1 = 1.0f - kscale * (float)(j * j);
-    const float La = lap_full(Ta, r + HALO, tx + HALO);
-    float mun = fmaf(k1, mu[g], La);
-    if (z == m.rcv_row) {
-      const int d = x - m.rcv_x0;
-      if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv) {
-        const int64_t si = w.seis_off + (int64_t)tsrc * m.n_rcv + d / m.rcv_dx;
-        const float inj = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
-        mun = mun + inj;
-      }
+  for (int k = 0; k < RZ / 4; ++k) {
+    const int r = wv + 4 * k;
+    if (r >= rows) continue;
+    const int gz = wrapi(z0 + r, NZ);
+#pragma unroll
+    for (int c = 0; c < RX / 64; ++c) {
+      const int cc = lane + 64 * c;
+      if (cc >= cols) continue;
+      T[r][cc] = F[(int64_t)gz * NX + wrapi(x0 + cc, NX)];
     }
-    const float ln = lin[g] + mun;
-    mu[g] = mun;
-    lout[g] = ln;
-    const float Lp = lap_full(Tp, r + HALO, tx + HALO);
-    const float pc = Tp[r + HALO][tx + HALO];
-    GA[g] = fmaf(ln, Lp, GA[g]);
-    GK[g] = fmaf(ln, p0[g] - pc, GK[g]);
-    if (g == w.src_cell)
-      a.wparam[bi.w].ssum += (double)ln * (double)a.wavelet[tsrc];
   }
 }
 
-// Zero `nslots` slots of region `reg` of every wavefield (graph-safe fill;
-// no hipMemset nodes are recorded in captured graphs).
-__global__ __launch_bounds__(NTHR) void k_fill(const WaveDev* waves, const ModelDev* models,
-                                                float* arena, int reg, int slot0, int nslots) {
-  const WaveDev& w = waves[blockIdx.y];
-  if (!models[w.model].active) return;
-  const int64_t P = (int64_t)w.NZ * w.NX;
-  float* base = arena + w.off[reg] + slot0 * P;
-  const int64_t n = P * nslots;
-  for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTHR)
-    base[i] = 0.f;
+// K reverse steps of update_p_adjoint (:517-579) in gather form for every
+// tile, with the imaging accumulators (alpha / kappa adjoints) and the
+// source-wavelet correlation (:314) fused on the tile interior.  lam, mu,
+// alpha and (1-kappa) of owned cells stay in registers; alpha*lam is staged
+// in LDS for the stencil, and the forward wavefield p[m-1], p[m-2] streams
+// through two LDS windows (interior + 2 halo).
+template <int RX, int RZ>
+__global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
+  constexpr int CPR = RX / 64, RPW = RZ / 4, NC = CPR * RPW;
+  static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
+  __shared__ float A[RZ][RX];
+  __shared__ float PT[2][RZ][RX];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ntiles = *a.n_tiles;
+  const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
+  for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    const TileEnt te = a.tiles[e];
+    if (te.w < 0) continue;
+    const WaveDev& w = a.waves[te.w];
+    const ModelDev& m = a.models[w.model];
+    const int NZ = w.NZ, NX = w.NX;
+    const int64_t Pn = (int64_t)NZ * NX;
+    const float* lin = a.arena + w.off[R_LAM] + a.lam_in_slot * Pn;
+    const float* min_ = a.arena + w.off[R_MU] + a.mu_in_slot * Pn;
+    float* GA = a.arena + w.off[R_GA];
+    float* GK = a.arena + w.off[R_GK];
+    const float* pbase = a.arena + w.off[a.p_reg];
+    const float* alpha = a.alpha + m.cell_off;
+    const float kscale = a.params[w.model].kscale;
+    const int oz = te.z0 - H, ox = te.x0 - H;
+    const int TH = RZ - 2 * H + 4, TW = RX - 2 * H + 4;  // p windows: interior + 2
+    float lam[NC], mu[NC], al[NC], k1[NC], ln[NC];
+    unsigned srcmask = 0, rcvmask = 0;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + 4 * k;
+      const int gz = wrapi(oz + r, NZ);
+      if (gz == m.rcv_row) rcvmask |= 1u << k;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int i = k * CPR + c;
+        const int cc = lane + 64 * c;
+        const int gx = wrapi(ox + cc, NX);
+        const int64_t g = (int64_t)gz * NX + gx;
+        lam[i] = lin[g];
+        mu[i] = min_[g];
+        al[i] = alpha[g];
+        A[r][cc] = al[i] * lam[i];
+        const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
+        k1[i] = 1.0f - kscale * (float)(j * j);
+        if (g == w.src_cell) srcmask |= 1u << i;
+      }
+    }
+    load_window<RX, RZ>(PT[0], pbase + (a.M0 - 2 - a.p_base) * Pn, NZ, NX, te.z0 - 2, te.x0 - 2, TH, TW);
+    __syncthreads();
+    for (int s = 0; s < a.K; ++s) {
+      const int mm = a.M0 - 1 - s;
+      const int tsrc = mm - 2;
+      const int lo = 2 * (s + 1);
+      const int cur = s & 1, nxt = cur ^ 1;
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + 4 * k;
+        if (r < lo || r >= RZ - lo) continue;
+        const bool rrow = (rcvmask >> k) & 1u;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int i = k * CPR + c;
+          const int cc = lane + 64 * c;
+          if (cc < lo || cc >= RX - lo) continue;
+          const float La = lap_full<RX>(A, r, cc);
+          float mun = fmaf(k1[i], mu[i], La);
+          if (rrow) {
+            const int d = wrapi(ox + cc, NX) - m.rcv_x0;
+            if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv) {
+              const int64_t si = w.seis_off + (int64_t)tsrc * m.n_rcv + d / m.rcv_dx;
+              const float inj = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
+              mun = mun + inj;
+            }
+          }
+          ln[i] = lam[i] + mun;
+          mu[i] = mun;
+        }
+      }
+      load_window<RX, RZ>(PT[nxt], pbase + (mm - 2 - a.p_base) * Pn, NZ, NX, te.z0 - 2, te.x0 - 2, TH, TW);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + 4 * k;
+        if (r < lo || r >= RZ - lo) continue;
+        const int zz = oz + r;
+        const bool zin = r >= H && r < RZ - H && zz < NZ;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int i = k * CPR + c;
+          const int cc = lane + 64 * c;
+          if (cc < lo || cc >= RX - lo) continue;
+          A[r][cc] = al[i] * ln[i];
+          lam[i] = ln[i];
+          const int xx = ox + cc;
+          if (!(zin && cc >= H && cc < RX - H && xx < NX)) continue;
+          const int pr = r - H + 2, pc = cc - H + 2;
+          const float Lp = lap_full<RX>(PT[cur], pr, pc);
+          const float p1 = PT[cur][pr][pc];
+          const float p0 = PT[nxt][pr][pc];
+          const int64_t g = (int64_t)zz * NX + xx;
+          GA[g] = fmaf(ln[i], Lp, GA[g]);
+          GK[g] = fmaf(ln[i], p0 - p1, GK[g]);
+          if ((srcmask >> i) & 1u) a.wparam[te.w].ssum += (double)ln[i] * (double)a.wavelet[tsrc];
+        }
+      }
+      __syncthreads();
+    }
+    float* lout = a.arena + w.off[R_LAM] + a.lam_out_slot * Pn;
+    float* mout = a.arena + w.off[R_MU] + a.mu_out_slot * Pn;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + 4 * k;
+      const int zz = oz + r;
+      if (r < H || r >= RZ - H || zz >= NZ) continue;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int i = k * CPR + c;
+        const int cc = lane + 64 * c;
+        const int xx = ox + cc;
+        if (cc < H || cc >= RX - H || xx >= NX) continue;
+        const int64_t g = (int64_t)zz * NX + xx;
+        lout[g] = lam[i];
+        mout[g] = mu[i];
+      }
+    }
+  }
 }
 
-// Copy slot `src_slot` of region `src_reg` into `dst_reg`/`dst_slot`, and
-// optionally dst2 = dst + add (restores p[jK+1] = p[jK] + q[jK+1]).
+// Zero `nslots` slots of region `reg` of every active wavefield (graph-safe
+// fill: no hipMemset nodes are recorded in captured graphs).
+__global__ __launch_bounds__(NTHR) void k_fill(const WaveDev* waves, const ModelDev* models,
+                                                int n_waves, float* arena, int reg, int slot0,
+                                                int nslots) {
+  for (int wi = blockIdx.y; wi < n_waves; wi += gridDim.y) {
+    const WaveDev& w = waves[wi];
+    if (!models[w.model].active) continue;
+    const int64_t P = (int64_t)w.NZ * w.NX;
+    float* base = arena + w.off[reg] + slot0 * P;
+    const int64_t n = P * nslots;
+    for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTHR)
+      base[i] = 0.f;
+  }
+}
+
+// Restore checkpoint j (state p[jK], q[jK]) into segment slot 0 and q slot 0
+// before the segment is recomputed.
 __global__ __launch_bounds__(NTHR) void k_restore(const WaveDev* waves, const ModelDev* models,
-                                                   float* arena, int j) {
-  const WaveDev& w = waves[blockIdx.y];
-  if (!models[w.model].active) return;
-  const int64_t P = (int64_t)w.NZ * w.NX;
-  const float* ckp = arena + w.off[R_CKP] + j * P;
-  const float* ckq = arena + w.off[R_CKQ] + j * P;
-  float* s0 = arena + w.off[R_HIST];
-  float* s1 = s0 + P;
-  float* q = arena + w.off[R_Q];
-  for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < P; i += (int64_t)gridDim.x * NTHR) {
-    const float pv = ckp[i];
-    const float qv = ckq[i];
-    s0[i] = pv;
-    q[i] = qv;
-    s1[i] = pv + qv;
+                                                   int n_waves, float* arena, int j) {
+  for (int wi = blockIdx.y; wi < n_waves; wi += gridDim.y) {
+    const WaveDev& w = waves[wi];
+    if (!models[w.model].active) continue;
+    const int64_t P = (int64_t)w.NZ * w.NX;
+    const float* ckp = arena + w.off[R_CKP] + j * P;
+    const float* ckq = arena + w.off[R_CKQ] + j * P;
+    float* s0 = arena + w.off[R_HIST];
+    float* q = arena + w.off[R_Q];
+    for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < P; i += (int64_t)gridDim.x * NTHR) {
+      s0[i] = ckp[i];
+      q[i] = ckq[i];
+    }
   }
 }
 
@@ -570,8 +712,6 @@ __global__ void k_misfit_final(const ModelDev* models, int n_models, const doubl
 // ======================================================================== host
 namespace {
 
-constexpr int kFillBlocks = 128;
-
 __global__ __launch_bounds__(NTHR) void k_copy_active(const ModelDev* models, const float* src,
                                                        float* dst) {
   const ModelDev& m = models[blockIdx.y];
@@ -583,15 +723,30 @@ __global__ __launch_bounds__(NTHR) void k_copy_active(const ModelDev* models, co
 
 int64_t align64(int64_t n) { return (n + 63) & ~int64_t(63); }
 
+// Kernel launch configuration: LDS window (rx x rz cells, halo included) and
+// time steps per launch k.  The interior of a window is (rz-4k) x (rx-4k).
+struct KCfg {
+  int rx, rz, k;
+};
+
+bool shape_ok(int rx, int rz) {
+  return (rx == 64 && rz == 32) || (rx == 64 && rz == 64) || (rx == 128 && rz == 32);
+}
+
+bool cfg_ok(const KCfg& c) {
+  return shape_ok(c.rx, c.rz) && c.k >= 1 && c.k <= 8 && c.rz - 4 * c.k >= 4 && c.rx - 4 * c.k >= 8;
+}
+
+const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 64, 2}, {64, 32, 1}, {64, 64, 2}};
+
 struct Layout {
   std::vector<ModelDev> models;
   std::vector<WaveDev> waves;
   std::vector<int64_t> vec_off, seis_off;
   int64_t cells = 0, arena = 0, seis = 0, vec = 0, fold = 0;
-  int K = 0, J = 0;
+  int Kc = 0, J = 0;          // checkpoint interval and count (checkpoint mode)
   bool full = true, fwd_hist = true;
-  int n_blk = 0, max_P = 0, max_cells_phys = 0;
-  std::vector<int2> blk;
+  int max_P = 0, max_cells_phys = 0, max_NZ = 0;
   int64_t bytes = 0;
 };
 
@@ -606,20 +761,21 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
   const bool tan = caps & SEIS_CAP_TANGENT, adj = caps & SEIS_CAP_ADJOINT;
   const int nt = td->nt, nbc = td->nbc;
   L.full = ckpt_every == 0 || ckpt_every >= nt;
-  L.K = L.full ? 0 : ckpt_every;
-  L.J = L.full ? 0 : (nt + L.K - 1) / L.K;
+  L.Kc = L.full ? 0 : ckpt_every;
+  L.J = L.full ? 0 : (nt + L.Kc - 1) / L.Kc;
   L.fwd_hist = L.full && adj;
-  std::vector<int2> logical;
   for (int mi = 0; mi < n_models; ++mi) {
     const seis_model_desc& d = md[mi];
-    if (d.nz < 2 || d.nx < 2) return fail(SEIS_EINVAL, "model " + std::to_string(mi) + ": nz, nx must be >= 2");
-    if (!(d.dx > 0.f)) return fail(SEIS_EINVAL, "model " + std::to_string(mi) + ": dx must be positive");
-    if (d.n_shots < 1 || d.n_shots > SEIS_MAX_SHOTS)
-      return fail(SEIS_EINVAL, "model " + std::to_string(mi) + ": n_shots out of range");
+    const std::string who = "model " + std::to_string(mi) + ": ";
+    if (d.nz < 2 || d.nx < 2) return fail(SEIS_EINVAL, who + "nz, nx must be >= 2");
+    if (!(d.dx > 0.f)) return fail(SEIS_EINVAL, who + "dx must be positive");
+    if (d.n_shots < 1 || d.n_shots > SEIS_MAX_SHOTS) return fail(SEIS_EINVAL, who + "n_shots out of range");
     if (d.src_z < 0 || d.src_z >= d.nz || d.rcv_z < 0 || d.rcv_z >= d.nz)
-      return fail(SEIS_EINVAL, "model " + std::to_string(mi) + ": source/receiver row outside grid");
+      return fail(SEIS_EINVAL, who + "source/receiver row outside grid");
     if (d.n_rcv < 1 || d.rcv_dx < 1 || d.rcv_x0 < 0 || d.rcv_x0 + (int64_t)(d.n_rcv - 1) * d.rcv_dx >= d.nx)
-      return fail(SEIS_EINVAL, "model " + std::to_string(mi) + ": receivers outside grid");
+      return fail(SEIS_EINVAL, who + "receivers outside grid");
+    for (int s = 0; s < d.n_shots; ++s)
+      if (d.src_x[s] < 0 || d.src_x[s] >= d.nx) return fail(SEIS_EINVAL, who + "source column outside grid");
     ModelDev m{};
     m.nz = d.nz; m.nx = d.nx; m.NZ = d.nz + 2 * nbc; m.NX = d.nx + 2 * nbc;
     m.nbc = nbc; m.n_shots = d.n_shots; m.first_wave = (int)L.waves.size(); m.active = 1;
@@ -631,8 +787,9 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
     m.kappa0 = 3.0 * std::log(1e7) / (2.0 * (nbc - 1) * (double)d.dx);
     m.kunit = (double)td->dt * m.kappa0 / ((double)(nbc - 1) * (nbc - 1));
     const int64_t P = (int64_t)m.NZ * m.NX;
-    if (P > INT32_MAX / 2) return fail(SEIS_EINVAL, "model too large");
+    if (P > INT32_MAX / 4) return fail(SEIS_EINVAL, who + "model too large");
     L.max_P = std::max<int>(L.max_P, (int)P);
+    L.max_NZ = std::max(L.max_NZ, m.NZ);
     L.max_cells_phys = std::max(L.max_cells_phys, d.nz * d.nx);
     L.vec_off.push_back(L.vec);
     L.seis_off.push_back(L.seis);
@@ -641,12 +798,9 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
     L.fold += align64((int64_t)m.NZ * m.nx + m.NZ);
     L.cells += align64(P);
     L.seis += m.seis_len;
-    const int tiles_x = (m.NX + TX - 1) / TX, tiles_z = (m.NZ + TZ - 1) / TZ;
     for (int s = 0; s < d.n_shots; ++s) {
-      if (d.src_x[s] < 0 || d.src_x[s] >= d.nx)
-        return fail(SEIS_EINVAL, "model " + std::to_string(mi) + ": source column outside grid");
       WaveDev w{};
-      w.model = mi; w.shot = s; w.NZ = m.NZ; w.NX = m.NX; w.tiles_x = tiles_x;
+      w.model = mi; w.shot = s; w.NZ = m.NZ; w.NX = m.NX; w.tiles_x = 0;
       w.src_z = d.src_z; w.src_x = d.src_x[s];
       w.src_cell = (d.src_z + nbc) * m.NX + (d.src_x[s] + nbc);
       w.seis_off = m.seis_off + (int64_t)s * nt * d.n_rcv;
@@ -654,56 +808,48 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
         w.off[r] = L.arena;
         L.arena += align64(P) * slots;
       };
-      region(R_HIST, L.fwd_hist ? nt + 2 : (adj ? L.K + 1 : 0));
+      region(R_HIST, L.fwd_hist ? nt + 2 : (adj ? L.Kc + 1 : 0));
       region(R_PP, L.fwd_hist ? 0 : 2);
-      region(R_Q, 1);
+      region(R_Q, 2);
       region(R_DP, tan ? 2 : 0);
-      region(R_DQ, tan ? 1 : 0);
+      region(R_DQ, tan ? 2 : 0);
       region(R_LAM, adj ? 2 : 0);
-      region(R_MU, adj ? 1 : 0);
+      region(R_MU, adj ? 2 : 0);
       region(R_GA, adj ? 1 : 0);
       region(R_GK, adj ? 1 : 0);
       region(R_CKP, adj && !L.full ? L.J : 0);
       region(R_CKQ, adj && !L.full ? L.J : 0);
-      const int wi = (int)L.waves.size();
-      for (int t = 0; t < tiles_x * tiles_z; ++t) logical.push_back(make_int2(wi, t));
       L.waves.push_back(w);
     }
     L.models.push_back(m);
   }
   L.vec_off.push_back(L.vec);
   L.seis_off.push_back(L.seis);
-  // XCD-aware block order: hardware deals block b to XCD b % 8, so physical
-  // block b takes logical tile (b % 8) * (n/8) + b / 8 and consecutive tiles of
-  // one wavefield (which share halos) stay in one XCD's L2.
-  const int nlog = (int)logical.size();
-  L.n_blk = (nlog + NXCD - 1) / NXCD * NXCD;
-  const int per = L.n_blk / NXCD;
-  L.blk.assign(L.n_blk, make_int2(-1, 0));
-  for (int b = 0; b < L.n_blk; ++b) {
-    const int lg = (b % NXCD) * per + b / NXCD;
-    if (lg < nlog) L.blk[b] = logical[lg];
-  }
   const int64_t nw = (int64_t)L.waves.size();
-  L.bytes = (int64_t)sizeof(float) * (L.arena + L.cells * (tan ? 2 : 1) + L.seis * (1 + (tan ? 1 : 0) + (adj ? 1 : 0)) + nt) +
+  L.bytes = (int64_t)sizeof(float) * (L.arena + L.cells * (tan ? 2 : 1) + L.seis * (2 + (tan ? 1 : 0)) + nt) +
             (int64_t)sizeof(double) * (adj ? L.fold : 0) + (int64_t)sizeof(double) * 32 * n_models +
-            (int64_t)(sizeof(ModelDev) + 2 * sizeof(int64_t) + sizeof(ParamDev)) * n_models +
-            (int64_t)(sizeof(WaveDev) + sizeof(WaveParam)) * nw + (int64_t)sizeof(int2) * L.n_blk;
+            (int64_t)(sizeof(ModelDev) + sizeof(ParamDev)) * n_models +
+            (int64_t)(sizeof(WaveDev) + sizeof(WaveParam)) * nw;
   return SEIS_OK;
 }
+
+// Work list of one kernel kind on the current active set.
+struct TileList {
+  TileEnt* d = nullptr;
+  int* d_count = nullptr;
+  int cap = 0;     // entries allocated (all models active)
+  int grid = 0;    // persistent grid (multiple of 8), fixed while the config is
+};
 
 }  // namespace
 
 struct seis_plan {
   Layout L;
-  int n_models = 0, n_waves = 0, nt = 0, caps = 0;
+  int n_models = 0, n_waves = 0, nt = 0, caps = 0, num_cu = 0;
   ModelDev* d_models = nullptr;
   WaveDev* d_waves = nullptr;
   ParamDev* d_params = nullptr;
   WaveParam* d_wparam = nullptr;
-  int2* d_blk = nullptr;
-  double* d_fold = nullptr;
-  double* d_part = nullptr;
   float* d_alpha = nullptr;
   float* d_dalpha = nullptr;
   float* d_wavelet = nullptr;
@@ -711,6 +857,10 @@ struct seis_plan {
   float* d_seis = nullptr;
   float* d_dseis = nullptr;
   float* d_dd = nullptr;
+  double* d_fold = nullptr;
+  double* d_part = nullptr;
+  KCfg cfg[SEIS_KIND_COUNT];
+  TileList tl[SEIS_KIND_COUNT];
   hipStream_t cap = nullptr;
   hipGraphExec_t g_fwd[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraphExec_t g_adj[3] = {nullptr, nullptr, nullptr};
@@ -718,16 +868,24 @@ struct seis_plan {
 
 namespace {
 
-void plan_free(seis_plan* p) {
+void drop_graphs(seis_plan* p) {
   for (auto& a : p->g_fwd)
     for (auto& g : a)
-      if (g) (void)hipGraphExecDestroy(g);
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   for (auto& g : p->g_adj)
-    if (g) (void)hipGraphExecDestroy(g);
-  void* bufs[] = {p->d_models, p->d_waves, p->d_params, p->d_wparam, p->d_blk, p->d_fold, p->d_part,
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+void plan_free(seis_plan* p) {
+  drop_graphs(p);
+  void* bufs[] = {p->d_models, p->d_waves, p->d_params, p->d_wparam, p->d_fold, p->d_part,
                   p->d_alpha, p->d_dalpha, p->d_wavelet, p->d_arena, p->d_seis, p->d_dseis, p->d_dd};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (auto& t : p->tl) {
+    if (t.d) (void)hipFree(t.d);
+    if (t.d_count) (void)hipFree(t.d_count);
+  }
   if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
 }
@@ -745,107 +903,221 @@ int dev_alloc(T** ptr, int64_t count) {
   return SEIS_OK;
 }
 
-FwdArgs fwd_args(seis_plan* p) {
-  FwdArgs a{};
-  a.blk = p->d_blk; a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params;
-  a.wparam = p->d_wparam; a.alpha = p->d_alpha; a.dalpha = p->d_dalpha; a.wavelet = p->d_wavelet;
-  a.arena = p->d_arena; a.seis = p->d_seis; a.dseis = p->d_dseis;
-  a.ck_p_slot = a.ck_q_slot = -1;
-  return a;
+// ---- kernel dispatch over the instantiated window shapes
+template <int RX, int RZ>
+const void* kernel_ptr_t(int kind) {
+  if (kind == SEIS_KIND_FORWARD) return (const void*)k_fwd<RX, RZ, false>;
+  if (kind == SEIS_KIND_TANGENT) return (const void*)k_fwd<RX, RZ, true>;
+  return (const void*)k_adj<RX, RZ>;
+}
+
+const void* kernel_ptr(int kind, const KCfg& c) {
+  if (c.rx == 64 && c.rz == 32) return kernel_ptr_t<64, 32>(kind);
+  if (c.rx == 64 && c.rz == 64) return kernel_ptr_t<64, 64>(kind);
+  return kernel_ptr_t<128, 32>(kind);
+}
+
+template <int RX, int RZ>
+void launch_fwd_t(bool tan, int grid, hipStream_t s, const FwdArgs& a) {
+  if (tan)
+    hipLaunchKernelGGL((k_fwd<RX, RZ, true>), dim3(grid), dim3(NTHR), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fwd<RX, RZ, false>), dim3(grid), dim3(NTHR), 0, s, a);
+}
+
+void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
+  if (c.rx == 64 && c.rz == 32) launch_fwd_t<64, 32>(tan, grid, s, a);
+  else if (c.rx == 64 && c.rz == 64) launch_fwd_t<64, 64>(tan, grid, s, a);
+  else launch_fwd_t<128, 32>(tan, grid, s, a);
+}
+
+void launch_adj(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
+  if (c.rx == 64 && c.rz == 32) hipLaunchKernelGGL((k_adj<64, 32>), dim3(grid), dim3(NTHR), 0, s, a);
+  else if (c.rx == 64 && c.rz == 64) hipLaunchKernelGGL((k_adj<64, 64>), dim3(grid), dim3(NTHR), 0, s, a);
+  else hipLaunchKernelGGL((k_adj<128, 32>), dim3(grid), dim3(NTHR), 0, s, a);
+}
+
+// Tiles of the active wavefields for window config c, laid out so that the
+// persistent block handling list entry e (on XCD e % 8) receives a contiguous
+// eighth of the tile sequence: neighbouring tiles of a wavefield, which share
+// halo rows and columns, stay in one XCD's L2.
+std::vector<TileEnt> make_tiles(const Layout& L, const KCfg& c, bool all_active) {
+  std::vector<TileEnt> seq;
+  const int IZ = c.rz - 4 * c.k, IX = c.rx - 4 * c.k;
+  for (int wi = 0; wi < (int)L.waves.size(); ++wi) {
+    const WaveDev& w = L.waves[wi];
+    if (!all_active && !L.models[w.model].active) continue;
+    for (int z0 = 0; z0 < w.NZ; z0 += IZ)
+      for (int x0 = 0; x0 < w.NX; x0 += IX) seq.push_back(TileEnt{wi, z0, x0, 0});
+  }
+  const int n = (int)seq.size();
+  const int per = (n + NXCD - 1) / NXCD;
+  std::vector<TileEnt> out((size_t)per * NXCD, TileEnt{-1, 0, 0, 0});
+  for (int e = 0; e < per * NXCD; ++e) {
+    const int lg = (e % NXCD) * per + e / NXCD;
+    if (lg < n) out[e] = seq[lg];
+  }
+  return out;
+}
+
+int upload_tiles(seis_plan* p, int kind) {
+  TileList& t = p->tl[kind];
+  const std::vector<TileEnt> v = make_tiles(p->L, p->cfg[kind], false);
+  const int n = (int)v.size();
+  if (n > t.cap) return fail(SEIS_EINVAL, "tile list overflow");
+  if (n) HIP_TRY(hipMemcpy(t.d, v.data(), sizeof(TileEnt) * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.d_count, &n, sizeof(int), hipMemcpyHostToDevice));
+  return SEIS_OK;
+}
+
+// (Re)build the work list of `kind` for its current config: capacity for the
+// all-active case, persistent grid from the kernel's occupancy.
+int setup_kind(seis_plan* p, int kind) {
+  TileList& t = p->tl[kind];
+  const int cap = (int)make_tiles(p->L, p->cfg[kind], true).size();
+  if (cap > t.cap) {
+    if (t.d) (void)hipFree(t.d);
+    t.d = nullptr;
+    t.cap = 0;
+    int rc = dev_alloc(&t.d, cap);
+    if (rc) return rc;
+    t.cap = cap;
+  }
+  if (!t.d_count) {
+    int rc = dev_alloc(&t.d_count, 1);
+    if (rc) return rc;
+  }
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kind, p->cfg[kind]), NTHR, 0));
+  int gmax = std::max(1, per_cu) * p->num_cu;
+  gmax = std::max(NXCD, gmax / NXCD * NXCD);
+  t.grid = std::max(NXCD, std::min(cap, gmax));
+  return upload_tiles(p, kind);
 }
 
 void fill(seis_plan* p, hipStream_t s, int reg, int slot0, int nslots) {
-  hipLaunchKernelGGL(k_fill, dim3(kFillBlocks, p->n_waves), dim3(NTHR), 0, s, p->d_waves,
-                     p->d_models, p->d_arena, reg, slot0, nslots);
+  hipLaunchKernelGGL(k_fill, dim3(64, std::min(p->n_waves, 256)), dim3(NTHR), 0, s, p->d_waves,
+                     p->d_models, p->n_waves, p->d_arena, reg, slot0, nslots);
 }
 
-// Forward time loop (with the fused tangent when `tan`), saving checkpoints in
-// checkpoint mode when `save`.  core/seis_forward2.py:211-247 / :255-283.
+FwdArgs fwd_args(seis_plan* p, int kind) {
+  FwdArgs a{};
+  a.tiles = p->tl[kind].d; a.n_tiles = p->tl[kind].d_count;
+  a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params; a.wparam = p->d_wparam;
+  a.alpha = p->d_alpha; a.dalpha = p->d_dalpha; a.wavelet = p->d_wavelet;
+  a.arena = p->d_arena; a.seis = p->d_seis; a.dseis = p->d_dseis;
+  a.nt = p->nt;
+  a.Kt = p->cfg[kind].k;
+  a.ck_slot = -1;
+  return a;
+}
+
+// Forward time loop from the zero state at n = 0 through n = nt (p[1] .. p[nt+1],
+// core/seis_forward2.py:211-247), fused with the tangent when `tan`
+// (:255-283).  In checkpoint mode with `save`, launches stop at every
+// checkpoint boundary n = j*Kc and store the state there.
 void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   const Layout& L = p->L;
+  const int kind = tan ? SEIS_KIND_TANGENT : SEIS_KIND_FORWARD;
+  const KCfg& c = p->cfg[kind];
+  const bool ck = save && !L.full && (p->caps & SEIS_CAP_ADJOINT);
   fill(p, s, R_Q, 0, 1);
-  if (L.fwd_hist) {
-    fill(p, s, R_HIST, 0, 2);
-  } else {
-    fill(p, s, R_PP, 0, 2);
-    if (save && (p->caps & SEIS_CAP_ADJOINT)) {
-      fill(p, s, R_CKP, 0, 1);
-      fill(p, s, R_CKQ, 0, 1);
-    }
+  fill(p, s, L.fwd_hist ? R_HIST : R_PP, 0, 1);
+  if (ck) {
+    fill(p, s, R_CKP, 0, 1);
+    fill(p, s, R_CKQ, 0, 1);
   }
   if (tan) {
-    fill(p, s, R_DP, 0, 2);
+    fill(p, s, R_DP, 0, 1);
     fill(p, s, R_DQ, 0, 1);
   }
-  FwdArgs a = fwd_args(p);
+  FwdArgs a = fwd_args(p, kind);
   a.write_seis = 1;
-  const bool ck = save && !L.full && (p->caps & SEIS_CAP_ADJOINT);
-  for (int it = 0; it < p->nt; ++it) {
-    const int n = it + 1;
-    a.it = it;
+  int n = 0, li = 0;
+  while (n < p->nt + 1) {
+    int K = std::min(c.k, p->nt + 1 - n);
+    if (ck) K = std::min(K, (n / L.Kc + 1) * L.Kc - n);
+    a.n0 = n;
+    a.K = K;
     if (L.fwd_hist) {
-      a.in_reg = R_HIST; a.in_slot = n; a.out_reg = R_HIST; a.out_slot = n + 1;
+      a.p_in_reg = R_HIST; a.p_in_slot = n;
+      a.p_out_reg = R_HIST; a.p_out_slot0 = n + 1; a.hist_all = 1;
     } else {
-      a.in_reg = R_PP; a.in_slot = n & 1; a.out_reg = R_PP; a.out_slot = (n + 1) & 1;
+      a.p_in_reg = R_PP; a.p_in_slot = li & 1;
+      a.p_out_reg = R_PP; a.p_out_slot0 = (li + 1) & 1; a.hist_all = 0;
     }
-    a.dp_in_slot = n & 1;
-    a.dp_out_slot = (n + 1) & 1;
-    a.ck_p_slot = a.ck_q_slot = -1;
-    if (ck) {
-      if ((n + 1) % L.K == 0 && (n + 1) / L.K < L.J) a.ck_p_slot = (n + 1) / L.K;
-      if (n % L.K == 0 && n / L.K >= 1 && n / L.K < L.J) a.ck_q_slot = n / L.K;
-    }
-    if (tan)
-      hipLaunchKernelGGL(k_fwd_step<true>, dim3(L.n_blk), dim3(NTHR), 0, s, a);
-    else
-      hipLaunchKernelGGL(k_fwd_step<false>, dim3(L.n_blk), dim3(NTHR), 0, s, a);
+    a.q_in_slot = a.dp_in_slot = a.dq_in_slot = li & 1;
+    a.q_out_slot = a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
+    a.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
+    launch_fwd(c, tan, p->tl[kind].grid, s, a);
+    n += K;
+    ++li;
   }
 }
 
-// Reverse time loop: update_p_adjoint (:292-313) for m = nt+1 .. 2, with
-// segment recompute from checkpoints in checkpoint mode.
+// Reverse time loop: lam[m] for m = nt+1 .. 2 (update_p_adjoint graph,
+// core/seis_forward2.py:292-313).  In checkpoint mode each segment of Kc
+// steps is first recomputed from its checkpoint into the segment buffer.
 void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
   const Layout& L = p->L;
-  fill(p, s, R_LAM, 0, 2);
+  fill(p, s, R_LAM, 0, 1);
   fill(p, s, R_MU, 0, 1);
   fill(p, s, R_GA, 0, 1);
   fill(p, s, R_GK, 0, 1);
+  const KCfg& ca = p->cfg[SEIS_KIND_ADJOINT];
   AdjArgs a{};
-  a.blk = p->d_blk; a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params;
-  a.wparam = p->d_wparam; a.alpha = p->d_alpha; a.wavelet = p->d_wavelet; a.arena = p->d_arena;
+  a.tiles = p->tl[SEIS_KIND_ADJOINT].d; a.n_tiles = p->tl[SEIS_KIND_ADJOINT].d_count;
+  a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params; a.wparam = p->d_wparam;
+  a.alpha = p->d_alpha; a.wavelet = p->d_wavelet; a.arena = p->d_arena; a.nt = p->nt;
+  a.Kt = ca.k;
   if (kind == 0) { a.inj_a = p->d_seis; a.inj_b = p->d_dd; }
   else if (kind == 1) { a.inj_a = p->d_dd; a.inj_b = nullptr; }
   else { a.inj_a = p->d_dseis; a.inj_b = nullptr; }
-  const int nt = p->nt;
-  auto adj_step = [&](int m, int reg, int base) {
-    a.m = m;
-    a.lam_in_slot = (m + 1) & 1;
-    a.lam_out_slot = m & 1;
-    a.p_reg = reg;
-    a.p1_slot = m - 1 - base;
-    a.p0_slot = m - 2 - base;
-    hipLaunchKernelGGL(k_adj_step, dim3(L.n_blk), dim3(NTHR), 0, s, a);
+  const int grid = p->tl[SEIS_KIND_ADJOINT].grid;
+  int li = 0;
+  auto run_range = [&](int m_hi, int m_lo, int reg, int base) {
+    int M0 = m_hi + 1;
+    while (M0 - 1 >= m_lo) {
+      const int K = std::min(ca.k, M0 - m_lo);
+      a.M0 = M0;
+      a.K = K;
+      a.lam_in_slot = a.mu_in_slot = li & 1;
+      a.lam_out_slot = a.mu_out_slot = (li + 1) & 1;
+      a.p_reg = reg;
+      a.p_base = base;
+      launch_adj(ca, grid, s, a);
+      M0 -= K;
+      ++li;
+    }
   };
   if (L.full) {
-    for (int m = nt + 1; m >= 2; --m) adj_step(m, R_HIST, 0);
+    run_range(p->nt + 1, 2, R_HIST, 0);
     return;
   }
-  FwdArgs f = fwd_args(p);
+  const KCfg& cf = p->cfg[SEIS_KIND_FORWARD];
+  FwdArgs f = fwd_args(p, SEIS_KIND_FORWARD);
   f.write_seis = 0;
-  f.in_reg = f.out_reg = R_HIST;
+  f.p_in_reg = f.p_out_reg = R_HIST;
+  f.hist_all = 1;
   for (int j = L.J - 1; j >= 0; --j) {
-    const int lo = j * L.K;
-    const int hi = std::min(lo + L.K, nt);  // largest p index this segment needs
-    hipLaunchKernelGGL(k_restore, dim3(kFillBlocks, p->n_waves), dim3(NTHR), 0, s, p->d_waves,
-                       p->d_models, p->d_arena, j);
-    for (int n = lo + 1; n <= hi - 1; ++n) {
-      f.it = n - 1;
-      f.in_slot = n - lo;
-      f.out_slot = n + 1 - lo;
-      hipLaunchKernelGGL(k_fwd_step<false>, dim3(L.n_blk), dim3(NTHR), 0, s, f);
+    const int lo = j * L.Kc;
+    const int hi = std::min(lo + L.Kc, p->nt);  // largest p index the segment needs
+    hipLaunchKernelGGL(k_restore, dim3(64, std::min(p->n_waves, 256)), dim3(NTHR), 0, s, p->d_waves,
+                       p->d_models, p->n_waves, p->d_arena, j);
+    int n = lo, fi = 0;
+    while (n < hi) {
+      const int K = std::min(cf.k, hi - n);
+      f.n0 = n;
+      f.K = K;
+      f.p_in_slot = n - lo;
+      f.p_out_slot0 = n + 1 - lo;
+      f.q_in_slot = fi & 1;
+      f.q_out_slot = (fi + 1) & 1;
+      launch_fwd(cf, false, p->tl[SEIS_KIND_FORWARD].grid, s, f);
+      n += K;
+      ++fi;
     }
-    const int m_hi = std::min(lo + L.K + 1, nt + 1);
-    for (int m = m_hi; m >= lo + 2; --m) adj_step(m, R_HIST, lo);
+    run_range(std::min(lo + L.Kc + 1, p->nt + 1), lo + 2, R_HIST, lo);
   }
 }
 
@@ -878,9 +1150,7 @@ int run_prep(seis_plan* p, hipStream_t s, const float* vec, const float* dvec) {
 
 int finish_grad(seis_plan* p, hipStream_t s, const float* vec, double* grad) {
   const Layout& L = p->L;
-  int max_nz_pad = 0;
-  for (const auto& m : L.models) max_nz_pad = std::max(max_nz_pad, m.NZ);
-  hipLaunchKernelGGL(k_fold_rows, dim3(max_nz_pad, p->n_models), dim3(NTHR), 0, s, p->d_models,
+  hipLaunchKernelGGL(k_fold_rows, dim3(L.max_NZ, p->n_models), dim3(NTHR), 0, s, p->d_models,
                      p->d_waves, p->d_arena, p->d_fold);
   hipLaunchKernelGGL(k_fold_cols, dim3((L.max_cells_phys + NTHR - 1) / NTHR, p->n_models),
                      dim3(NTHR), 0, s, p->d_models, p->d_waves, p->d_wparam, p->d_fold, vec, grad);
@@ -902,9 +1172,9 @@ int copy_out(seis_plan* p, hipStream_t s, const float* src, float* dst) {
   return SEIS_OK;
 }
 
-#define TRY(expr)                 \
-  do {                            \
-    int rc_ = (expr);             \
+#define TRY(expr)                   \
+  do {                              \
+    int rc_ = (expr);               \
     if (rc_ != SEIS_OK) return rc_; \
   } while (0)
 
@@ -914,8 +1184,8 @@ extern "C" {
 
 const char* seis_last_error(void) { return g_last_error.c_str(); }
 
-int64_t seis_plan_bytes(const seis_model_desc* models, int32_t n_models,
-                        const seis_time_desc* time, int32_t ckpt_every, int32_t caps) {
+int64_t seis_plan_bytes(const seis_model_desc* models, int32_t n_models, const seis_time_desc* time,
+                        int32_t ckpt_every, int32_t caps) {
   Layout L;
   int rc = validate_and_layout(models, n_models, time, ckpt_every, caps, L);
   return rc == SEIS_OK ? L.bytes : (int64_t)rc;
@@ -935,23 +1205,29 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
   p->n_waves = (int)L.waves.size();
   p->nt = time->nt;
   p->caps = caps;
+  for (int k = 0; k < SEIS_KIND_COUNT; ++k) p->cfg[k] = kDefaultCfg[k];
   const bool tan = caps & SEIS_CAP_TANGENT, adj = caps & SEIS_CAP_ADJOINT;
   auto bail = [&](int code) { plan_free(p); return code; };
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&p->num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    fail(SEIS_EHIP, "cannot query the device");
+    return bail(SEIS_EHIP);
+  }
   if ((rc = dev_alloc(&p->d_models, n_models)) || (rc = dev_alloc(&p->d_waves, p->n_waves)) ||
       (rc = dev_alloc(&p->d_params, n_models)) || (rc = dev_alloc(&p->d_wparam, p->n_waves)) ||
-      (rc = dev_alloc(&p->d_blk, L.n_blk)) || (rc = dev_alloc(&p->d_fold, adj ? L.fold : 0)) ||
+      (rc = dev_alloc(&p->d_fold, adj ? L.fold : 0)) ||
       (rc = dev_alloc(&p->d_part, (int64_t)kRedBlocks * n_models)) ||
       (rc = dev_alloc(&p->d_alpha, L.cells)) || (rc = dev_alloc(&p->d_dalpha, tan ? L.cells : 0)) ||
       (rc = dev_alloc(&p->d_wavelet, p->nt)) || (rc = dev_alloc(&p->d_arena, L.arena)) ||
       (rc = dev_alloc(&p->d_seis, L.seis)) || (rc = dev_alloc(&p->d_dseis, tan ? L.seis : 0)) ||
-      (rc = dev_alloc(&p->d_dd, adj ? L.seis : 0)))
+      (rc = dev_alloc(&p->d_dd, L.seis)))
     return bail(rc);
   std::vector<float> wl(p->nt);
   for (int i = 0; i < p->nt; ++i) wl[i] = (float)time->wavelet[i];
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMemcpy(p->d_models, L.models.data(), sizeof(ModelDev) * n_models, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_waves, L.waves.data(), sizeof(WaveDev) * p->n_waves, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->d_blk, L.blk.data(), sizeof(int2) * L.n_blk, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_wavelet, wl.data(), sizeof(float) * p->nt, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->d_wparam, 0, sizeof(WaveParam) * p->n_waves);
   if (e == hipSuccess) e = hipMemset(p->d_params, 0, sizeof(ParamDev) * n_models);
@@ -960,6 +1236,8 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
     fail(SEIS_EHIP, std::string("plan setup: ") + hipGetErrorString(e));
     return bail(SEIS_EHIP);
   }
+  for (int k = 0; k < SEIS_KIND_COUNT; ++k)
+    if ((rc = setup_kind(p, k))) return bail(rc);
   *plan_out = p;
   return SEIS_OK;
 }
@@ -980,11 +1258,33 @@ int seis_plan_layout(const seis_plan_t* plan, int64_t* vec_off, int64_t* seis_of
   return SEIS_OK;
 }
 
+int seis_plan_set_config(seis_plan_t* p, int32_t kind, int32_t rx, int32_t rz, int32_t k) {
+  if (!p || kind < 0 || kind >= SEIS_KIND_COUNT) return fail(SEIS_EINVAL, "bad plan or kind");
+  const KCfg c{rx, rz, k};
+  if (!cfg_ok(c))
+    return fail(SEIS_EINVAL, "unsupported config: window " + std::to_string(rx) + "x" + std::to_string(rz) +
+                                 ", k=" + std::to_string(k));
+  HIP_TRY(hipDeviceSynchronize());
+  drop_graphs(p);
+  p->cfg[kind] = c;
+  return setup_kind(p, kind);
+}
+
+int seis_plan_get_config(const seis_plan_t* p, int32_t kind, int32_t* rx, int32_t* rz, int32_t* k) {
+  if (!p || kind < 0 || kind >= SEIS_KIND_COUNT || !rx || !rz || !k) return fail(SEIS_EINVAL, "bad argument");
+  *rx = p->cfg[kind].rx;
+  *rz = p->cfg[kind].rz;
+  *k = p->cfg[kind].k;
+  return SEIS_OK;
+}
+
 int seis_set_active(seis_plan_t* p, const int32_t* active) {
   if (!p || !active) return fail(SEIS_EINVAL, "NULL argument");
+  HIP_TRY(hipDeviceSynchronize());
   for (int i = 0; i < p->n_models; ++i) p->L.models[i].active = active[i] ? 1 : 0;
   HIP_TRY(hipMemcpy(p->d_models, p->L.models.data(), sizeof(ModelDev) * p->n_models,
                     hipMemcpyHostToDevice));
+  for (int k = 0; k < SEIS_KIND_COUNT; ++k) TRY(upload_tiles(p, k));
   return SEIS_OK;
 }
 
@@ -994,6 +1294,19 @@ int seis_forward(seis_plan_t* p, const float* vec, float* seis, seis_stream_t st
   TRY(run_prep(p, s, vec, nullptr));
   TRY(get_graph(p, &p->g_fwd[0][0], [&](hipStream_t c) { enqueue_forward(p, c, false, false); }));
   HIP_TRY(hipGraphLaunch(p->g_fwd[0][0], s));
+  return copy_out(p, s, p->d_seis, seis);
+}
+
+int seis_forward_misfit(seis_plan_t* p, const float* vec, const float* dd, float* seis, double* misfit,
+                        seis_stream_t stream) {
+  if (!p || !vec || !dd || !misfit) return fail(SEIS_EINVAL, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  TRY(run_prep(p, s, vec, nullptr));
+  HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
+  TRY(get_graph(p, &p->g_fwd[0][0], [&](hipStream_t c) { enqueue_forward(p, c, false, false); }));
+  HIP_TRY(hipGraphLaunch(p->g_fwd[0][0], s));
+  launch_misfit(p, s, misfit);
+  HIP_TRY(hipGetLastError());
   return copy_out(p, s, p->d_seis, seis);
 }
 
@@ -1018,8 +1331,7 @@ int seis_gradient(seis_plan_t* p, const float* vec, const float* dd, int32_t on_
   HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
   TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
   HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
-  if (on_residual && misfit)
-    launch_misfit(p, s, misfit);
+  if (on_residual && misfit) launch_misfit(p, s, misfit);
   const int kind = on_residual ? 0 : 1;
   TRY(get_graph(p, &p->g_adj[kind], [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
   HIP_TRY(hipGraphLaunch(p->g_adj[kind], s));

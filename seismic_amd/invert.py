@@ -1,0 +1,531 @@
+"""Inversion loops driving the propagator (core/seis_invert.py, core/seis_numerics.py).
+
+Every loop is written as a *machine*: a generator that yields propagator
+requests (``Op``) and is resumed with their results.  One machine per
+inversion request; an ``Executor`` runs the pending ops of many machines as
+batched launches (continuous batching), or one at a time (the plain path).
+Control flow and float64 vector algebra stay on the host side in torch; every
+wavefield operation goes through libseis.
+
+  lbfgs_machine   bfgs + _strong_wolfe + _cubic_interpolate, core/seis_numerics.py:138-448,
+                  with the cost closure of seis_to_vel_lbfgs (core/seis_invert.py:125-138)
+  gn_machine      Gauss-Newton passes of seis_to_vel_gn (core/seis_invert.py:63-107) with
+                  the dense J^T J solve replaced by conjugate gradients on J^T J products
+  bfgs            reference signature, driving lbfgs_machine on one model
+  cost_and_gradient  reference signature (core/seis_invert.py:19-49)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Any, Callable, Sequence
+
+import numpy as np
+import torch
+
+from ._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
+from .geometry import Geometry
+from .propagator import Propagator, vel_to_seis
+
+
+# --------------------------------------------------------------------- ops
+@dataclass
+class Op:
+    """A propagator request from a machine.
+
+    kind 'fev': vec -> (misfit, g) with misfit = sum((F(vec) - target)^2) and
+                g = J(vec)^T (F(vec) - target)                     (float64)
+    kind 'gnp': (vec, dvec) -> J(vec)^T J(vec) dvec                 (float64)
+    kind 'fwd': vecs (list) -> [sum((F(v) - target)^2) for v in vecs]
+    """
+
+    kind: str
+    vec: Any = None
+    dvec: Any = None
+    vecs: Any = None
+
+
+@dataclass
+class Request:
+    """One inversion: geometry, observed data (float32, device) and prior."""
+
+    geom: Geometry
+    target: torch.Tensor
+    prior: Any
+    start: torch.Tensor                  # float64 (n_params,)
+    rid: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_data(self) -> int:
+        return self.geom.n_data
+
+
+# --------------------------------------------------------------------- LBFGS
+def _cubic_interpolate(x1, f1, g1, x2, f2, g2, bounds=None):
+    """core/seis_numerics.py:279-304 (scalars as Python floats)."""
+    if bounds is not None:
+        xmin_bound, xmax_bound = bounds
+    else:
+        xmin_bound, xmax_bound = (x1, x2) if x1 <= x2 else (x2, x1)
+    d1 = g1 + g2 - 3 * (f1 - f2) / (x1 - x2)
+    d2_square = d1 ** 2 - g1 * g2
+    if d2_square >= 0:
+        d2 = math.sqrt(d2_square)
+        if x1 <= x2:
+            min_pos = x2 - (x2 - x1) * ((g2 + d2 - d1) / (g2 - g1 + 2 * d2))
+        else:
+            min_pos = x1 - (x1 - x2) * ((g1 + d2 - d1) / (g1 - g2 + 2 * d2))
+        return min(max(min_pos, xmin_bound), xmax_bound)
+    return (xmin_bound + xmax_bound) / 2.0
+
+
+def _strong_wolfe(obj, x, t, d, f, g, gtd, c1=1e-4, c2=0.9, tolerance_change=1e-9, max_ls=25):
+    """core/seis_numerics.py:138-277 as a sub-machine; ``obj(x, t, d)`` is a
+    machine returning (f: float, g: tensor)."""
+    d_norm = float(d.abs().max())
+    g = g.clone()
+    f_new, g_new = yield from obj(x, t, d)
+    ls_func_evals = 1
+    gtd_new = float(g_new.dot(d))
+    t_prev, f_prev, g_prev, gtd_prev = 0, f, g, gtd
+    done = False
+    ls_iter = 0
+    while ls_iter < max_ls:
+        if f_new > (f + c1 * t * gtd) or (ls_iter > 1 and f_new >= f_prev):
+            bracket, bracket_f = [t_prev, t], [f_prev, f_new]
+            bracket_g, bracket_gtd = [g_prev, g_new.clone()], [gtd_prev, gtd_new]
+            break
+        if abs(gtd_new) <= -c2 * gtd:
+            bracket, bracket_f, bracket_g = [t], [f_new], [g_new]
+            done = True
+            break
+        if gtd_new >= 0:
+            bracket, bracket_f = [t_prev, t], [f_prev, f_new]
+            bracket_g, bracket_gtd = [g_prev, g_new.clone()], [gtd_prev, gtd_new]
+            break
+        min_step = t + 0.01 * (t - t_prev)
+        max_step = t * 10
+        tmp = t
+        t = _cubic_interpolate(t_prev, f_prev, gtd_prev, t, f_new, gtd_new, bounds=(min_step, max_step))
+        t_prev, f_prev, g_prev, gtd_prev = tmp, f_new, g_new.clone(), gtd_new
+        f_new, g_new = yield from obj(x, t, d)
+        ls_func_evals += 1
+        gtd_new = float(g_new.dot(d))
+        ls_iter += 1
+    if ls_iter == max_ls:
+        bracket, bracket_f, bracket_g = [0, t], [f, f_new], [g, g_new]
+    insuf_progress = False
+    low_pos, high_pos = (0, 1) if bracket_f[0] <= bracket_f[-1] else (1, 0)
+    while not done and ls_iter < max_ls:
+        if abs(bracket[1] - bracket[0]) * d_norm < tolerance_change:
+            break
+        t = _cubic_interpolate(bracket[0], bracket_f[0], bracket_gtd[0],
+                               bracket[1], bracket_f[1], bracket_gtd[1])
+        eps = 0.1 * (max(bracket) - min(bracket))
+        if min(max(bracket) - t, t - min(bracket)) < eps:
+            if insuf_progress or t >= max(bracket) or t <= min(bracket):
+                if abs(t - max(bracket)) < abs(t - min(bracket)):
+                    t = max(bracket) - eps
+                else:
+                    t = min(bracket) + eps
+                insuf_progress = False
+            else:
+                insuf_progress = True
+        else:
+            insuf_progress = False
+        f_new, g_new = yield from obj(x, t, d)
+        ls_func_evals += 1
+        gtd_new = float(g_new.dot(d))
+        ls_iter += 1
+        if f_new > (f + c1 * t * gtd) or f_new >= bracket_f[low_pos]:
+            bracket[high_pos] = t
+            bracket_f[high_pos] = f_new
+            bracket_g[high_pos] = g_new.clone()
+            bracket_gtd[high_pos] = gtd_new
+            low_pos, high_pos = (0, 1) if bracket_f[0] <= bracket_f[1] else (1, 0)
+        else:
+            if abs(gtd_new) <= -c2 * gtd:
+                done = True
+            elif gtd_new * (bracket[high_pos] - bracket[low_pos]) >= 0:
+                bracket[high_pos] = bracket[low_pos]
+                bracket_f[high_pos] = bracket_f[low_pos]
+                bracket_g[high_pos] = bracket_g[low_pos]
+                bracket_gtd[high_pos] = bracket_gtd[low_pos]
+            bracket[low_pos] = t
+            bracket_f[low_pos] = f_new
+            bracket_g[low_pos] = g_new.clone()
+            bracket_gtd[low_pos] = gtd_new
+    t = bracket[low_pos]
+    return bracket_f[low_pos], bracket_g[low_pos], t, ls_func_evals
+
+
+def bfgs_machine(func, x0: torch.Tensor, max_iter: int, tolerance_grad: float,
+                 tolerance_change: float, history_size: int | None = None):
+    """core/seis_numerics.py:307-448 as a machine.  ``func(x)`` is a machine
+    returning (loss: float, grad: tensor).  history_size=None keeps every
+    (s, y) pair like the reference; an int keeps the most recent ones.
+    Returns (x, n_evals, n_iter)."""
+    lr = 1
+    max_eval = max_iter * 5 // 4
+    loss, flat_grad = yield from func(x0)
+    x = x0.clone()
+    current_evals = 1
+    if float(flat_grad.abs().max()) <= tolerance_grad:
+        return x, current_evals, 0
+    n_iter = 0
+    old_dirs: list = []
+    old_stps: list = []
+    ro: list = []
+    prev_flat_grad = None
+    H_diag = 1.0
+    d = t = None
+    while n_iter < max_iter:
+        n_iter += 1
+        if n_iter == 1:
+            d = flat_grad.neg()
+            H_diag = 1.0
+        else:
+            y = flat_grad.sub(prev_flat_grad)
+            s = d.mul(t)
+            ys = float(y.dot(s))
+            if ys > 1e-10:
+                if history_size is not None and len(old_dirs) == history_size:
+                    old_dirs.pop(0)
+                    old_stps.pop(0)
+                    ro.pop(0)
+                old_dirs.append(y)
+                old_stps.append(s)
+                ro.append(1.0 / ys)
+                H_diag = ys / float(y.dot(y))
+            num_old = len(old_dirs)
+            q = flat_grad.neg()
+            al = [0.0] * num_old
+            for i in range(num_old - 1, -1, -1):        # lbfgs_backward (:451-501)
+                al[i] = float(old_stps[i].dot(q)) * ro[i]
+                q.add_(old_dirs[i], alpha=-al[i])
+            r = torch.mul(q, H_diag)
+            for i in range(num_old):                     # lbfgs_forward (:511-559)
+                be_i = float(old_dirs[i].dot(r)) * ro[i]
+                r.add_(old_stps[i], alpha=al[i] - be_i)
+            d = r
+        prev_flat_grad = flat_grad.clone()
+        prev_loss = loss
+        if n_iter == 1:
+            t = min(1.0, 1.0 / float(flat_grad.abs().sum())) * lr
+        else:
+            t = lr
+        gtd = float(flat_grad.dot(d))
+        x_init = x.clone()
+
+        def obj(xx, tt, dd):
+            return (yield from func(xx + tt * dd))
+
+        loss, flat_grad, t, ls_func_evals = yield from _strong_wolfe(obj, x_init, t, d, loss, flat_grad, gtd)
+        x.add_(d, alpha=t)
+        opt_cond = float(flat_grad.abs().max()) <= tolerance_grad
+        current_evals += ls_func_evals
+        if n_iter == max_iter:
+            break
+        if current_evals >= max_eval:
+            break
+        if opt_cond:
+            break
+        if float(d.mul(t).abs().max()) <= tolerance_change:
+            break
+        if abs(loss - prev_loss) < tolerance_change:
+            break
+    return x, current_evals, n_iter
+
+
+def _basis_of(prior):
+    return getattr(prior, "basis", None)
+
+
+def lbfgs_machine(req: Request, max_iter: int = 30, tolerance_grad: float = 1e-7,
+                  tolerance_change: float = 0.0, scaling: float = 1e15,
+                  seis_error_tolerance: float = 0.0, history_size: int | None = None):
+    """InversionModel.seis_to_vel_lbfgs (core/seis_invert.py:109-152) as a machine.
+    Returns the inverted parameter vector (float64, n_params)."""
+    prior = req.prior
+    N = req.n_data
+    x_guess = prior.to_basis(req.start)
+    stats = req.meta.setdefault("stats", {"fev": 0})
+
+    def func(x):
+        vec = prior.from_basis(x)
+        misfit, gres = yield Op("fev", vec=vec)
+        stats["fev"] += 1
+        cost_prior, grad_prior = prior.cost_grad(x)
+        cost_res = misfit / N
+        cost = float(cost_prior) + cost_res
+        grad = grad_prior + (2.0 / N) * prior.basis_T(gres)
+        stats["misfit"] = cost_res
+        if cost_res < seis_error_tolerance:
+            cost, grad = 0.0 * cost, 0.0 * grad
+        return cost * scaling, grad * scaling
+
+    x, n_evals, n_iter = yield from bfgs_machine(func, x_guess, max_iter, tolerance_grad,
+                                                 tolerance_change, history_size)
+    stats["n_iter"] = n_iter
+    if torch.isnan(x).any():
+        return req.start.clone()       # core/seis_invert.py:146-149
+    return prior.from_basis(x)
+
+
+# --------------------------------------------------------------------- Gauss-Newton
+def conjugate_gradient_machine(apply_A, b: torch.Tensor, n_iter: int):
+    """CG from 0 on an SPD operator given as a machine ``apply_A(p)``."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    p = r.clone()
+    rr = float(r.dot(r))
+    for _ in range(n_iter):
+        if rr == 0.0:
+            break
+        Ap = yield from apply_A(p)
+        a = rr / float(p.dot(Ap))
+        x.add_(p, alpha=a)
+        r.add_(Ap, alpha=-a)
+        rr_new = float(r.dot(r))
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    return x
+
+
+def gn_machine(req: Request, n_outer: int = 10, n_cg: int = 20, n_scales: int = 50):
+    """n_outer passes of seis_to_vel_gn (core/seis_invert.py:63-107).
+
+    Each pass: rhs = -B^T J^T r / N - lam P x  (:77-78); solve
+    (lam P (+) 0 + J^T J / N) dx = rhs by n_cg CG steps, with J^T J products
+    from one tangent + one adjoint pass each (replacing the dense J of
+    :84-92); line search over linspace(-0.1, 1.1, n_scales) of the forward
+    cost (:95-102).  The prior must provide apply_P.
+    """
+    prior = req.prior
+    N = req.n_data
+    vec = req.start.clone()
+    scales = np.linspace(-0.1, 1.1, n_scales)
+    stats = req.meta.setdefault("stats", {"fev": 0, "gnp": 0, "fwd": 0})
+    for _ in range(n_outer):
+        x_guess = prior.to_basis(vec)
+        misfit, gres = yield Op("fev", vec=prior.from_basis(x_guess))
+        stats["fev"] += 1
+        rhs = -prior.basis_T(gres) / N
+        rhs[:-1] -= prior.lam * prior.apply_P(x_guess[:-1])
+
+        def apply_A(p):
+            jtj = yield Op("gnp", vec=vec, dvec=prior.from_basis(p))
+            stats["gnp"] += 1
+            out = prior.basis_T(jtj) / N
+            out[:-1] += prior.lam * prior.apply_P(p[:-1])
+            return out
+
+        dx = yield from conjugate_gradient_machine(apply_A, rhs, n_cg)
+        trial = [x_guess + float(sc) * dx for sc in scales]
+        misfits = yield Op("fwd", vecs=[prior.from_basis(x) for x in trial])
+        stats["fwd"] += len(trial)
+        costs = [float(prior.cost_grad(x)[0]) + m / N for x, m in zip(trial, misfits)]
+        best = float(scales[int(np.argmin(costs))])
+        vec = vec + best * prior.from_basis(dx)
+        stats["misfit"] = min(costs)
+    return vec
+
+
+# --------------------------------------------------------------------- executor
+class Executor:
+    """Runs the pending ops of a set of machines on fixed slots.
+
+    ``slots`` lists one geometry per slot; request i of a batch occupies slot
+    i.  ``max_batch`` > 1 runs each op kind as one launch over every slot that
+    has one pending (inactive slots are skipped by the kernels); max_batch = 1
+    is the plain path: every op runs alone on a single-model plan.
+    """
+
+    def __init__(self, slots: Sequence[Geometry], ckpt_every: int = 0, max_batch: int | None = None,
+                 fwd_fanout: int = 50, device="cuda"):
+        self.slots = list(slots)
+        self.n = len(self.slots)
+        self.max_batch = self.n if max_batch is None else max_batch
+        self.ckpt_every = ckpt_every
+        self.fwd_fanout = fwd_fanout
+        self.device = torch.device(device)
+        self.targets: list = [None] * self.n
+        self._plans: dict = {}
+
+    # plan cache --------------------------------------------------------
+    def _plan(self, key, geoms, caps):
+        if key not in self._plans:
+            self._plans[key] = Propagator(geoms, ckpt_every=self.ckpt_every, caps=caps, device=self.device)
+        return self._plans[key]
+
+    def _grad_plan(self, slot_ids):
+        if self.max_batch == 1:
+            g = self.slots[slot_ids[0]]
+            return self._plan(("g1", g), [g], CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT)
+        return self._plan(("gB",), self.slots, CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT)
+
+    def set_target(self, slot: int, target: torch.Tensor):
+        self.targets[slot] = target.to(self.device, torch.float32).reshape(-1)
+        self._dd = None
+
+    def _targets_cat(self):
+        if getattr(self, "_dd", None) is None:
+            self._dd = torch.cat([t if t is not None else torch.zeros(g.n_data, device=self.device)
+                                  for t, g in zip(self.targets, self.slots)])
+        return self._dd
+
+    # execution ---------------------------------------------------------
+    def execute(self, pending: dict) -> dict:
+        """pending: slot -> Op.  Returns slot -> result."""
+        out = {}
+        by_kind: dict = {}
+        for s, op in pending.items():
+            by_kind.setdefault(op.kind, []).append(s)
+        for kind, ids in by_kind.items():
+            ids = sorted(ids)
+            chunks = [ids[i:i + self.max_batch] for i in range(0, len(ids), self.max_batch)]
+            for ch in chunks:
+                if kind == "fev":
+                    out.update(self._fev(ch, pending))
+                elif kind == "gnp":
+                    out.update(self._gnp(ch, pending))
+                elif kind == "fwd":
+                    out.update(self._fwd(ch, pending))
+                else:
+                    raise ValueError(kind)
+        return out
+
+    def _pack(self, P: Propagator, ids, vecs):
+        buf = torch.zeros(P.n_vec, dtype=torch.float32, device=self.device)
+        for i, v in zip(ids, vecs):
+            a, b = int(P.vec_off[i]), int(P.vec_off[i + 1])
+            buf[a:b] = v
+        return buf
+
+    def _local(self, ids):
+        """(plan, local slot ids) for a chunk."""
+        P = self._grad_plan(ids)
+        if self.max_batch == 1:
+            return P, [0]
+        P.set_active([i in set(ids) for i in range(self.n)])
+        return P, list(ids)
+
+    def _fev(self, ids, pending):
+        P, loc = self._local(ids)
+        v = self._pack(P, loc, [pending[i].vec for i in ids])
+        dd = self.targets[ids[0]] if self.max_batch == 1 else self._targets_cat()
+        _, misfit, grad = P.gradient(v, dd, on_residual=True)
+        mis = misfit.cpu().tolist()
+        return {i: (mis[l], grad[int(P.vec_off[l]):int(P.vec_off[l + 1])].clone()) for i, l in zip(ids, loc)}
+
+    def _gnp(self, ids, pending):
+        P, loc = self._local(ids)
+        v = self._pack(P, loc, [pending[i].vec for i in ids])
+        dv = self._pack(P, loc, [pending[i].dvec for i in ids])
+        _, grad = P.gn_product(v, dv)
+        return {i: grad[int(P.vec_off[l]):int(P.vec_off[l + 1])].clone() for i, l in zip(ids, loc)}
+
+    def _fwd(self, ids, pending):
+        """Line-search forwards: every trial model of every op in one launch
+        (batched) or one model per launch (plain)."""
+        out = {}
+        if self.max_batch == 1:
+            for i in ids:
+                g = self.slots[i]
+                P = self._plan(("f1", g), [g], CAP_FORWARD)
+                ms = []
+                for v in pending[i].vecs:
+                    ms.append(P.forward_misfit(P._vec(v), self.targets[i]))
+                out[i] = [float(m[0]) for m in torch.stack(ms).cpu()]
+            return out
+        groups = [(i, len(pending[i].vecs)) for i in ids]
+        geoms = [self.slots[i] for i, k in groups for _ in range(k)]
+        key = ("fB", tuple(geoms))
+        P = self._plan(key, geoms, CAP_FORWARD)
+        vecs = [v for i, _ in groups for v in pending[i].vecs]
+        v = self._pack(P, list(range(len(vecs))), vecs)
+        dd = torch.cat([self.targets[i] for i, k in groups for _ in range(k)])
+        mis = P.forward_misfit(v, dd).cpu().tolist()
+        pos = 0
+        for i, k in groups:
+            out[i] = mis[pos:pos + k]
+            pos += k
+        return out
+
+
+def run_machines(machines: dict, executor: Executor, on_finish: Callable | None = None) -> dict:
+    """Drive machines (slot -> generator) to completion through ``executor``."""
+    pending = {}
+    results = {}
+    for s, m in machines.items():
+        pending[s] = next(m)
+    while pending:
+        res = executor.execute(pending)
+        for s, r in res.items():
+            try:
+                pending[s] = machines[s].send(r)
+            except StopIteration as e:
+                results[s] = e.value
+                del pending[s]
+                if on_finish:
+                    on_finish(s, e.value)
+    return results
+
+
+# --------------------------------------------------------------------- reference signatures
+def cost_and_gradient(x, target, prior, basis_functions, compute_gradient=False, geometry=None):
+    """core/seis_invert.py:19-49 on the GPU propagator (single model).
+
+    x (N,1) float64, target (n_data,1).  Returns (cost, grad, cost_prior,
+    cost_residual) or (cost, cost_prior, cost_residual)."""
+    x = torch.as_tensor(x, dtype=torch.float64).reshape(-1)
+    cost_prior, grad_prior = prior.cost_grad(x.to(prior.device))
+    B = basis_functions
+    vec = (B @ x.to(prior.device)) if B is not None else x.to(prior.device)
+    target = torch.as_tensor(target)
+    if compute_gradient:
+        s, _, s_adj = vel_to_seis(vec.reshape(-1, 1), vec_adjoint=target.reshape(-1, 1),
+                                  adjoint_on_residual=True, geometry=geometry)
+    else:
+        s = vel_to_seis(vec.reshape(-1, 1), geometry=geometry)[0]
+    t = target.to(s.device, torch.float64).reshape(-1, 1)
+    cost_residual = torch.mean((s - t) ** 2)
+    if compute_gradient:
+        g = s_adj.reshape(-1)
+        gres = (2.0 / len(s)) * ((B.T @ g) if B is not None else g)
+        return cost_prior + cost_residual, (grad_prior + gres).reshape(-1, 1), cost_prior, cost_residual
+    return cost_prior + cost_residual, cost_prior, cost_residual
+
+
+def bfgs(cost_and_gradient_func, x0, max_iter, tolerance_grad, tolerance_change, history_size=None):
+    """core/seis_numerics.py:307 signature: cost_and_gradient_func(x) -> (loss, grad)."""
+    def func(x):
+        f, g = cost_and_gradient_func(x)
+        return float(f), g
+        yield  # pragma: no cover  (makes this a generator)
+
+    m = bfgs_machine(func, torch.as_tensor(x0), max_iter, tolerance_grad, tolerance_change, history_size)
+    try:
+        next(m)
+    except StopIteration as e:
+        return e.value[0]
+    raise RuntimeError("bfgs machine yielded an op without a propagator")
+
+
+def invert(requests: Sequence[Request], method: str = "lbfgs", batched: bool = True,
+           ckpt_every: int = 0, **kw) -> list:
+    """Invert a list of requests; batched=False runs the plain one-at-a-time path."""
+    ex = Executor([r.geom for r in requests], ckpt_every=ckpt_every,
+                  max_batch=None if batched else 1)
+    machines = {}
+    for i, r in enumerate(requests):
+        ex.set_target(i, r.target)
+        machines[i] = (lbfgs_machine(r, **kw) if method == "lbfgs" else gn_machine(r, **kw))
+    if batched:
+        res = run_machines(machines, ex)
+        return [res[i] for i in range(len(requests))]
+    out = []
+    for i in range(len(requests)):
+        out.append(run_machines({i: machines[i]}, ex)[i])
+    return out

@@ -1,0 +1,151 @@
+"""Priors for the inversion loops (core/seis_prior.py), on device in float64.
+
+Each prior carries ``basis`` (n_params x N, or None for the identity), the
+weight ``lam`` and ``cost_grad(x) -> (cost, grad)`` with lam applied
+(Prior.compute_cost_and_gradient, core/seis_prior.py:47-65).  Priors usable by
+Gauss-Newton also provide ``apply_P(xv)`` (the precision matrix acting on the
+velocity part, core/seis_invert.py:78,82).
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+
+class Prior:
+    basis: torch.Tensor | None = None
+    lam: float = 1.0
+
+    def __init__(self, nz: int, nx: int, device="cuda"):
+        self.nz, self.nx = nz, nx
+        self.device = torch.device(device)
+
+    @property
+    def N(self) -> int:
+        return self.nz * self.nx + 1 if self.basis is None else self.basis.shape[1]
+
+    def to_basis(self, vec: torch.Tensor) -> torch.Tensor:
+        """x = solve(B^T B, B^T vec) (core/seis_invert.py:72,121)."""
+        if self.basis is None:
+            return vec.clone()
+        B = self.basis
+        return torch.linalg.solve(B.T @ B, B.T @ vec)
+
+    def from_basis(self, x: torch.Tensor) -> torch.Tensor:
+        return x if self.basis is None else self.basis @ x
+
+    def basis_T(self, g: torch.Tensor) -> torch.Tensor:
+        return g if self.basis is None else self.basis.T @ g
+
+
+class RowTotalVariation(Prior):
+    """Rows constant, smoothed |row differences| (core/seis_prior.py:67-109)."""
+
+    def __init__(self, nz, nx, lam=1e-8, epsilon=0.1, device="cuda"):
+        super().__init__(nz, nx, device)
+        self.lam, self.epsilon = lam, epsilon
+        B = torch.zeros(nz * nx + 1, nz + 1, dtype=torch.float64)
+        for i in range(nz):
+            B[i * nx:(i + 1) * nx, i] = 1.0
+        B[-1, nz] = 1.0
+        self.basis = B.to(self.device)
+
+    def cost_grad(self, x):
+        nz = self.nz
+        diff = torch.diff(x[:nz])
+        cpi = torch.sqrt(diff ** 2 + self.epsilon ** 2)
+        cost = cpi.mean()
+        sign = diff / cpi
+        g = torch.zeros_like(x)
+        g[0] = -sign[0]
+        g[1:nz - 1] = sign[:-1] - sign[1:]
+        g[nz - 1] = sign[-1]
+        return self.lam * cost, self.lam * g / (nz - 1)
+
+
+class TotalVariation(Prior):
+    """Smoothed 2-D total variation, identity basis (core/seis_prior.py:112-177)."""
+
+    def __init__(self, nz, nx, lam=1e-8, epsilon=0.1, device="cuda"):
+        super().__init__(nz, nx, device)
+        self.lam, self.epsilon = lam, epsilon
+
+    def cost_grad(self, x):
+        xm = x[:-1].reshape(self.nz, self.nx)
+        d1 = torch.diff(xm, dim=0)
+        d2 = torch.diff(xm, dim=1)
+        e2 = self.epsilon ** 2
+        n = d1.numel() + d2.numel()
+        cost = (torch.sqrt(d1 ** 2 + e2).sum() + torch.sqrt(d2 ** 2 + e2).sum()) / n
+        g1 = d1 / torch.sqrt(d1 ** 2 + e2) / n
+        g2 = d2 / torch.sqrt(d2 ** 2 + e2) / n
+        gm = torch.zeros_like(xm)
+        gm[1:, :] += g1
+        gm[:-1, :] -= g1
+        gm[:, 1:] += g2
+        gm[:, :-1] -= g2
+        g = torch.cat([gm.reshape(-1), torch.zeros(1, dtype=x.dtype, device=x.device)])
+        return self.lam * cost, self.lam * g
+
+
+class L2Smoothness(Prior):
+    """lam * x^T D^T D x over the velocity part, D = first differences in z and x.
+
+    The Gauss-Newton workload's smoothness prior; like SquaredExponential
+    (core/seis_prior.py:232-242) its cost is lam x^T P x and its gradient
+    2 lam P x, and P = D^T D is applied matrix-free.
+    """
+
+    def __init__(self, nz, nx, lam=1e-8, device="cuda"):
+        super().__init__(nz, nx, device)
+        self.lam = lam
+
+    def apply_P(self, xv):
+        xm = xv.reshape(self.nz, self.nx)
+        d1 = torch.diff(xm, dim=0)
+        d2 = torch.diff(xm, dim=1)
+        out = torch.zeros_like(xm)
+        out[1:, :] += d1
+        out[:-1, :] -= d1
+        out[:, 1:] += d2
+        out[:, :-1] -= d2
+        return out.reshape(-1)
+
+    def cost_grad(self, x):
+        xv = x[:-1]
+        Px = self.apply_P(xv)
+        cost = torch.dot(xv, Px)
+        g = torch.cat([2 * Px, torch.zeros(1, dtype=x.dtype, device=x.device)])
+        return self.lam * cost, self.lam * g
+
+
+class SquaredExponential(Prior):
+    """Gaussian-process prior (core/seis_prior.py:181-242), identity basis.
+
+    K = sigma^2 exp(-d^2 / 2 l^2) + sigma_mean^2 + sigma_slope^2 y y^T + noise^2 I
+    over the (nz, nx) grid centred as in the reference; P = K^-1 (dense).
+    """
+
+    def __init__(self, nz, nx, lam=1e-8, length_scale=math.log(32.4), noise=0.1, sigma=183.4,
+                 sigma_mean=520.0, sigma_slope=31.4, device="cuda"):
+        super().__init__(nz, nx, device)
+        self.lam = lam
+        y = (torch.arange(nz, dtype=torch.float64) - nz // 2)[:, None] + torch.zeros(1, nx, dtype=torch.float64)
+        x = (torch.arange(nx, dtype=torch.float64) - nx // 2)[None, :] + torch.zeros(nz, 1, dtype=torch.float64)
+        x = x.reshape(-1, 1)
+        y = y.reshape(-1, 1)
+        d2 = (x - x.T) ** 2 + (y - y.T) ** 2
+        K = sigma ** 2 * torch.exp(-d2 / (2 * length_scale ** 2)) + sigma_mean ** 2
+        K = K + sigma_slope ** 2 * (y @ y.T) + noise ** 2 * torch.eye(nz * nx, dtype=torch.float64)
+        self.P = torch.linalg.inv(K.to(self.device))
+
+    def apply_P(self, xv):
+        return self.P @ xv
+
+    def cost_grad(self, x):
+        xv = x[:-1]
+        Px = self.P @ xv
+        cost = torch.dot(xv, Px)
+        g = torch.cat([2 * Px, torch.zeros(1, dtype=x.dtype, device=x.device)])
+        return self.lam * cost, self.lam * g

@@ -124,6 +124,18 @@ class Propagator:
             raise ValueError(f"data vector has {d.numel()} entries, plan expects {self.n_seis}")
         return d.to(device=self.device, dtype=torch.float32).contiguous()
 
+    KINDS = {"forward": _lib.KIND_FORWARD, "tangent": _lib.KIND_TANGENT, "adjoint": _lib.KIND_ADJOINT}
+
+    def set_config(self, kind: str, rx: int, rz: int, k: int):
+        """Launch config of one kernel kind: rx x rz LDS window, k steps per launch."""
+        _lib.check(_lib.lib().seis_plan_set_config(self._h, self.KINDS[kind], rx, rz, k), "seis_plan_set_config")
+
+    def get_config(self, kind: str):
+        rx, rz, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(_lib.lib().seis_plan_get_config(self._h, self.KINDS[kind], ctypes.byref(rx), ctypes.byref(rz),
+                                                   ctypes.byref(k)), "seis_plan_get_config")
+        return rx.value, rz.value, k.value
+
     def set_active(self, mask: Sequence[bool]):
         m = (ctypes.c_int32 * self.n_models)(*[1 if a else 0 for a in mask])
         _lib.check(_lib.lib().seis_set_active(self._h, m), "seis_set_active")
@@ -140,6 +152,14 @@ class Propagator:
         seis = out if out is not None else torch.empty(self.n_seis, device=self.device)
         _lib.check(_lib.lib().seis_forward(self._h, _ptr(v), _ptr(seis), _stream()), "seis_forward")
         return seis
+
+    def forward_misfit(self, vec, dd, seis_out: torch.Tensor | None = None) -> torch.Tensor:
+        """misfit[m] = sum((F(vec) - dd)^2) per model, float64 (n_models,)."""
+        v, d = self._vec(vec), self._data(dd)
+        misfit = torch.empty(self.n_models, device=self.device, dtype=torch.float64)
+        _lib.check(_lib.lib().seis_forward_misfit(self._h, _ptr(v), _ptr(d), _ptr(seis_out), _ptr(misfit),
+                                                  _stream()), "seis_forward_misfit")
+        return misfit
 
     def tangent(self, vec, dvec):
         v, dv = self._vec(vec), self._vec(dvec)
