@@ -175,16 +175,29 @@ struct FwdArgs {
   int write_seis;
 };
 
+constexpr int KMAX = 8;  // most time steps per launch
+
+// Receiver column index of window column cc (-1 if none).
+__device__ __forceinline__ int rcv_index(int gx, const ModelDev& m) {
+  const int d = gx - m.rcv_x0;
+  if (d < 0) return -1;
+  const int r = d / m.rcv_dx;
+  return (r * m.rcv_dx == d && r < m.n_rcv) ? r : -1;
+}
+
 // K forward time steps (update_p :428-462 with lapg :381-421 and the source
 // injection fused), and with TAN the Born recursion (update_p_diff :470-509),
 // for every tile of the work list.  Thread-owned cells keep q, alpha and
-// (1-kappa) in registers across the K steps; p (and dp) live in LDS.
+// (1-kappa) in registers across the K steps; p (and dp) live in LDS.  No
+// global load is issued inside the step loop: the wavelet samples of the
+// launch are staged in LDS with the window.
 template <int RX, int RZ, bool TAN>
 __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
   constexpr int CPR = RX / 64, RPW = RZ / 4, NC = CPR * RPW;
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
   __shared__ float P[RZ][RX];
   __shared__ float D[TAN ? RZ : 1][TAN ? RX : 1];
+  __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
@@ -197,24 +210,29 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
     const int64_t Pn = (int64_t)NZ * NX;
     const float* pin = a.arena + w.off[a.p_in_reg] + a.p_in_slot * Pn;
     const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
-    float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
-    float* pbase = a.arena + w.off[a.p_out_reg];
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float amp = a.wparam[te.w].amp;
     const int oz = te.z0 - H, ox = te.x0 - H;
     float q[NC], al[NC], k1[NC], pn[NC];
     float dq[TAN ? NC : 1], dal[TAN ? NC : 1], dk[TAN ? NC : 1], dpn[TAN ? NC : 1];
+    float dkscale = 0.f, amp_d = 0.f;
     const float* dpin = nullptr;
     const float* dqin = nullptr;
-    float dkscale = 0.f, amp_d = 0.f;
     if constexpr (TAN) {
       dpin = a.arena + w.off[R_DP] + a.dp_in_slot * Pn;
       dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
       dkscale = a.params[w.model].dkscale;
       amp_d = a.wparam[te.w].amp_d;
     }
+    if (threadIdx.x < KMAX) {
+      const int it = a.n0 + (int)threadIdx.x - 1;
+      SW[threadIdx.x] = ((int)threadIdx.x < a.K && it >= 0 && it < a.nt) ? a.wavelet[it] : 0.f;
+    }
     unsigned srcmask = 0, rcvmask = 0;
+    int rcol[CPR];
+#pragma unroll
+    for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int r = wv + 4 * k;
@@ -225,7 +243,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
         const int i = k * CPR + c;
         const int cc = lane + 64 * c;
         const int gx = wrapi(ox + cc, NX);
-        const int64_t g = (int64_t)gz * NX + gx;
+        const int g = gz * NX + gx;
         P[r][cc] = pin[g];
         q[i] = qin[g];
         al[i] = alpha[g];
@@ -239,28 +257,30 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
           dal[i] = a.dalpha[m.cell_off + g];
           dk[i] = dkscale * jj;
         }
+        pn[i] = 0.f;
+        if constexpr (TAN) dpn[i] = 0.f;
       }
     }
     __syncthreads();
     for (int s = 0; s < a.K; ++s) {
-      const int n = a.n0 + s;
-      const int it = n - 1;
+      const int it = a.n0 + s - 1;
       const bool live = it >= 0 && it < a.nt;
-      const float sv = live ? a.wavelet[it] : 0.f;
+      const float sv = SW[s];
       const float src = amp * sv;
       const float srcd = amp_d * sv;
       const int lo = 2 * (s + 1);
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
         const int r = wv + 4 * k;
-        if (r < lo || r >= RZ - lo) continue;
+        if (r < lo || r >= RZ - lo) continue;  // wave-uniform
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          if (cc < lo || cc >= RX - lo) continue;
+          // every column is computed (its stencil stays inside P); only the
+          // valid band [lo, RX-lo) is committed
+          const bool ok = cc >= lo && cc < RX - lo;
           const float L = lap_full<RX>(P, r, cc);
-          const float pc = P[r][cc];
           float qn = fmaf(k1[i], q[i], al[i] * L);
           if ((srcmask >> i) & 1u) qn = qn + src;
           if constexpr (TAN) {
@@ -270,11 +290,11 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
             dqn = fmaf(dal[i], L, dqn);
             dqn = fmaf(-dk[i], q[i], dqn);
             if ((srcmask >> i) & 1u) dqn = dqn + srcd;
-            dpn[i] = D[r][cc] + dqn;
-            dq[i] = dqn;
+            dpn[i] = ok ? D[r][cc] + dqn : dpn[i];
+            dq[i] = ok ? dqn : dq[i];
           }
-          pn[i] = pc + qn;
-          q[i] = qn;
+          pn[i] = ok ? P[r][cc] + qn : pn[i];
+          q[i] = ok ? qn : q[i];
         }
       }
       __syncthreads();
@@ -293,21 +313,20 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
           if constexpr (TAN) D[r][cc] = dpn[i];
           const int xx = ox + cc;
           if (!(zin && cc >= H && cc < RX - H && xx < NX)) continue;
-          const int64_t g = (int64_t)zz * NX + xx;
-          if (a.hist_all) pbase[(a.p_out_slot0 + s) * Pn + g] = pn[i];
-          if (a.write_seis && live && ((rcvmask >> k) & 1u)) {
-            const int d = xx - m.rcv_x0;
-            if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv) {
-              const int64_t si = w.seis_off + (int64_t)it * m.n_rcv + d / m.rcv_dx;
-              a.seis[si] = pn[i];
-              if constexpr (TAN) a.dseis[si] = dpn[i];
-            }
+          const int g = zz * NX + xx;
+          if (a.hist_all) a.arena[w.off[a.p_out_reg] + (a.p_out_slot0 + s) * Pn + g] = pn[i];
+          if (a.write_seis && live && ((rcvmask >> k) & 1u) && rcol[c] >= 0) {
+            const int64_t si = w.seis_off + (int64_t)it * m.n_rcv + rcol[c];
+            a.seis[si] = pn[i];
+            if constexpr (TAN) a.dseis[si] = dpn[i];
           }
         }
       }
       __syncthreads();
     }
     // state (p, q[, dp, dq]) at n0+K on the interior
+    float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
+    float* pout = a.arena + w.off[a.p_out_reg] + a.p_out_slot0 * Pn;
     float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
     float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
 #pragma unroll
@@ -321,9 +340,9 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
         const int cc = lane + 64 * c;
         const int xx = ox + cc;
         if (cc < H || cc >= RX - H || xx >= NX) continue;
-        const int64_t g = (int64_t)zz * NX + xx;
+        const int g = zz * NX + xx;
         qout[g] = q[i];
-        if (!a.hist_all) pbase[a.p_out_slot0 * Pn + g] = pn[i];
+        if (!a.hist_all) pout[g] = pn[i];
         if (ckp) {
           ckp[g] = pn[i];
           ckq[g] = q[i];
@@ -357,37 +376,23 @@ struct AdjArgs {
   int p_reg, p_base;    // forward p[n] sits at slot n - p_base of region p_reg
 };
 
-// Stage the (rows x cols) window at unwrapped origin (z0, x0) of F into T.
-template <int RX, int RZ>
-__device__ __forceinline__ void load_window(float (*T)[RX], const float* __restrict__ F, int NZ,
-                                            int NX, int z0, int x0, int rows, int cols) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < RZ / 4; ++k) {
-    const int r = wv + 4 * k;
-    if (r >= rows) continue;
-    const int gz = wrapi(z0 + r, NZ);
-#pragma unroll
-    for (int c = 0; c < RX / 64; ++c) {
-      const int cc = lane + 64 * c;
-      if (cc >= cols) continue;
-      T[r][cc] = F[(int64_t)gz * NX + wrapi(x0 + cc, NX)];
-    }
-  }
-}
-
 // K reverse steps of update_p_adjoint (:517-579) in gather form for every
 // tile, with the imaging accumulators (alpha / kappa adjoints) and the
 // source-wavelet correlation (:314) fused on the tile interior.  lam, mu,
-// alpha and (1-kappa) of owned cells stay in registers; alpha*lam is staged
-// in LDS for the stencil, and the forward wavefield p[m-1], p[m-2] streams
-// through two LDS windows (interior + 2 halo).
+// alpha, (1-kappa) and the imaging accumulators of owned cells stay in
+// registers; alpha*lam is staged in LDS for the stencil, and the forward
+// wavefield p[m-1], p[m-2] streams through two LDS windows (interior + 2
+// halo) whose global loads are issued before the lam update of each step.
+// The accumulators see the same fused multiply-add sequence as a per-step
+// read-modify-write, so results do not depend on K.
 template <int RX, int RZ>
 __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
   constexpr int CPR = RX / 64, RPW = RZ / 4, NC = CPR * RPW;
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
   __shared__ float A[RZ][RX];
   __shared__ float PT[2][RZ][RX];
+  __shared__ float INJ[KMAX][RX];
+  __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
@@ -406,20 +411,40 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const int oz = te.z0 - H, ox = te.x0 - H;
-    const int TH = RZ - 2 * H + 4, TW = RX - 2 * H + 4;  // p windows: interior + 2
-    float lam[NC], mu[NC], al[NC], k1[NC], ln[NC];
-    unsigned srcmask = 0, rcvmask = 0;
+    const int pz = te.z0 - 2, px = te.x0 - 2;        // p windows: interior + 2 halo
+    const int TH = RZ - 2 * H + 4, TW = RX - 2 * H + 4;
+    float lam[NC], mu[NC], al[NC], k1[NC], ln[NC], ga[NC], gk[NC], pv[NC];
+    unsigned srcmask = 0, rcvmask = 0, intmask = 0;
+    int rcol[CPR];
+#pragma unroll
+    for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
+    if (threadIdx.x < KMAX) {
+      const int t = a.M0 - 3 - (int)threadIdx.x;
+      SW[threadIdx.x] = ((int)threadIdx.x < a.K && t >= 0 && t < a.nt) ? a.wavelet[t] : 0.f;
+    }
+    // adjoint sources of this launch's steps along the window's columns
+    for (int idx = threadIdx.x; idx < a.K * RX; idx += NTHR) {
+      const int s = idx / RX, cc = idx - (idx / RX) * RX;
+      const int rr = rcv_index(wrapi(ox + cc, NX), m);
+      float v = 0.f;
+      if (rr >= 0) {
+        const int64_t si = w.seis_off + (int64_t)(a.M0 - 3 - s) * m.n_rcv + rr;
+        v = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
+      }
+      INJ[s][cc] = v;
+    }
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int r = wv + 4 * k;
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << k;
+      const int zz = oz + r;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = k * CPR + c;
         const int cc = lane + 64 * c;
         const int gx = wrapi(ox + cc, NX);
-        const int64_t g = (int64_t)gz * NX + gx;
+        const int g = gz * NX + gx;
         lam[i] = lin[g];
         mu[i] = min_[g];
         al[i] = alpha[g];
@@ -427,47 +452,82 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
         const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
         k1[i] = 1.0f - kscale * (float)(j * j);
         if (g == w.src_cell) srcmask |= 1u << i;
+        const int xx = ox + cc;
+        const bool inner = r >= H && r < RZ - H && zz < NZ && cc >= H && cc < RX - H && xx < NX;
+        ga[i] = 0.f;
+        gk[i] = 0.f;
+        if (inner) {
+          intmask |= 1u << i;
+          ga[i] = GA[zz * NX + xx];
+          gk[i] = GK[zz * NX + xx];
+        }
+        ln[i] = 0.f;
+        // p[M0-2] window (row r, column cc of the p window)
+        pv[i] = (r < TH && cc < TW) ? pbase[(int64_t)(a.M0 - 2 - a.p_base) * Pn +
+                                            wrapi(pz + r, NZ) * NX + wrapi(px + cc, NX)]
+                                      : 0.f;
       }
     }
-    load_window<RX, RZ>(PT[0], pbase + (a.M0 - 2 - a.p_base) * Pn, NZ, NX, te.z0 - 2, te.x0 - 2, TH, TW);
+    double ssum = 0.0;
+    const bool src_here = (srcmask & intmask) != 0;
+    if (src_here) ssum = a.wparam[te.w].ssum;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + 4 * k;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int cc = lane + 64 * c;
+        if (r < TH && cc < TW) PT[0][r][cc] = pv[k * CPR + c];
+      }
+    }
     __syncthreads();
     for (int s = 0; s < a.K; ++s) {
       const int mm = a.M0 - 1 - s;
-      const int tsrc = mm - 2;
       const int lo = 2 * (s + 1);
       const int cur = s & 1, nxt = cur ^ 1;
+      // issue the loads of p[mm-2] before the lam update
+      const float* pnext = pbase + (int64_t)(mm - 2 - a.p_base) * Pn;
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
         const int r = wv + 4 * k;
-        if (r < lo || r >= RZ - lo) continue;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int cc = lane + 64 * c;
+          if (r < TH && cc < TW) pv[k * CPR + c] = pnext[wrapi(pz + r, NZ) * NX + wrapi(px + cc, NX)];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + 4 * k;
+        if (r < lo || r >= RZ - lo) continue;  // wave-uniform
         const bool rrow = (rcvmask >> k) & 1u;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          if (cc < lo || cc >= RX - lo) continue;
+          const bool ok = cc >= lo && cc < RX - lo;
           const float La = lap_full<RX>(A, r, cc);
           float mun = fmaf(k1[i], mu[i], La);
-          if (rrow) {
-            const int d = wrapi(ox + cc, NX) - m.rcv_x0;
-            if (d >= 0 && d % m.rcv_dx == 0 && d / m.rcv_dx < m.n_rcv) {
-              const int64_t si = w.seis_off + (int64_t)tsrc * m.n_rcv + d / m.rcv_dx;
-              const float inj = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
-              mun = mun + inj;
-            }
-          }
-          ln[i] = lam[i] + mun;
-          mu[i] = mun;
+          if (rrow && rcol[c] >= 0) mun = mun + INJ[s][cc];
+          ln[i] = ok ? lam[i] + mun : ln[i];
+          mu[i] = ok ? mun : mu[i];
         }
       }
-      load_window<RX, RZ>(PT[nxt], pbase + (mm - 2 - a.p_base) * Pn, NZ, NX, te.z0 - 2, te.x0 - 2, TH, TW);
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + 4 * k;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int cc = lane + 64 * c;
+          if (r < TH && cc < TW) PT[nxt][r][cc] = pv[k * CPR + c];
+        }
+      }
       __syncthreads();
+      const float sw = SW[s];
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
         const int r = wv + 4 * k;
         if (r < lo || r >= RZ - lo) continue;
-        const int zz = oz + r;
-        const bool zin = r >= H && r < RZ - H && zz < NZ;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
@@ -475,16 +535,14 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
           if (cc < lo || cc >= RX - lo) continue;
           A[r][cc] = al[i] * ln[i];
           lam[i] = ln[i];
-          const int xx = ox + cc;
-          if (!(zin && cc >= H && cc < RX - H && xx < NX)) continue;
+          if (!((intmask >> i) & 1u)) continue;
           const int pr = r - H + 2, pc = cc - H + 2;
           const float Lp = lap_full<RX>(PT[cur], pr, pc);
           const float p1 = PT[cur][pr][pc];
           const float p0 = PT[nxt][pr][pc];
-          const int64_t g = (int64_t)zz * NX + xx;
-          GA[g] = fmaf(ln[i], Lp, GA[g]);
-          GK[g] = fmaf(ln[i], p0 - p1, GK[g]);
-          if ((srcmask >> i) & 1u) a.wparam[te.w].ssum += (double)ln[i] * (double)a.wavelet[tsrc];
+          ga[i] = fmaf(ln[i], Lp, ga[i]);
+          gk[i] = fmaf(ln[i], p0 - p1, gk[i]);
+          if ((srcmask >> i) & 1u) ssum += (double)ln[i] * (double)sw;
         }
       }
       __syncthreads();
@@ -495,18 +553,19 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
     for (int k = 0; k < RPW; ++k) {
       const int r = wv + 4 * k;
       const int zz = oz + r;
-      if (r < H || r >= RZ - H || zz >= NZ) continue;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = k * CPR + c;
-        const int cc = lane + 64 * c;
-        const int xx = ox + cc;
-        if (cc < H || cc >= RX - H || xx >= NX) continue;
-        const int64_t g = (int64_t)zz * NX + xx;
+        if (!((intmask >> i) & 1u)) continue;
+        const int g = zz * NX + ox + lane + 64 * c;
         lout[g] = lam[i];
         mout[g] = mu[i];
+        GA[g] = ga[i];
+        GK[g] = gk[i];
       }
     }
+    if (src_here && ((srcmask & intmask) != 0)) a.wparam[te.w].ssum = ssum;
+    __syncthreads();
   }
 }
 
@@ -737,7 +796,7 @@ bool cfg_ok(const KCfg& c) {
   return shape_ok(c.rx, c.rz) && c.k >= 1 && c.k <= 8 && c.rz - 4 * c.k >= 4 && c.rx - 4 * c.k >= 8;
 }
 
-const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 64, 2}, {64, 32, 1}, {64, 64, 2}};
+const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3}, {64, 32, 3}, {64, 32, 3}};
 
 struct Layout {
   std::vector<ModelDev> models;
