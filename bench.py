@@ -1,0 +1,333 @@
+"""Benchmark: batched Gauss-Newton inversions per second vs the plain path.
+
+Workload (one MI355X): synthetic 70x70 models, curved layers with a dipping
+fault (seeded), depth-linear start 1500 -> 4000 m/s, L2 smoothness prior,
+5 shots, 70 receivers, nt = 1000 steps of 1 ms.  One inversion = 10
+Gauss-Newton passes, each with 20 CG steps on J^T J (one tangent + one
+adjoint pass per step) and the reference's 50-point line search.
+
+A *step* is one Gauss-Newton pass over the whole batch.  ``value`` is the
+batch's inversions per second (steps/10 inversions per model); the same
+inversion run one model at a time (the plain path: same kernels, batch 1,
+hipGraph time loops) is timed beside it, and a served LBFGS run with Poisson
+arrivals reports p50/p99 latency.  ``roofline`` is the dominant kernel's
+algorithmic HBM bytes per launch over its average launch time (HIP events on
+the launch stream); ``cpu_baseline`` is the NumPy float64 oracle timed on a
+bounded sample on the host.
+
+Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12                     # MI355X HBM3E bytes/s
+ALG_BYTES = {"forward": 20, "tangent": 40, "adjoint": 28}   # per cell-update (float32)
+N_OUTER, N_CG, N_SCALES = 10, 20, 50
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_requests(B, g, seed0, device, lam):
+    import torch
+    from seismic_amd import invert, priors, synth
+    from seismic_amd.propagator import Propagator
+    trues = [synth.curve_fault_model(g.nz, g.nx, seed0 + i) for i in range(B)]
+    P = Propagator([g] * B, caps=1, device=device)
+    data = P.forward(torch.tensor(np.concatenate([synth.to_vec(t) for t in trues])))
+    torch.cuda.synchronize()
+    start = synth.to_vec(synth.depth_linear_start(g.nz, g.nx, 1500.0, 4000.0), min_vel=1500.0)
+    reqs = []
+    for i in range(B):
+        tgt = data[int(P.seis_off[i]):int(P.seis_off[i + 1])].clone()
+        reqs.append(invert.Request(geom=g, target=tgt, prior=priors.L2Smoothness(g.nz, g.nx, lam=lam, device=device),
+                                   start=torch.tensor(start, dtype=torch.float64, device=device), rid=i))
+    P.close()
+    return reqs, trues
+
+
+def run_gn(executor, reqs, n_outer):
+    """Run n_outer GN passes of every request through `executor`; returns (seconds, models)."""
+    import torch
+    from seismic_amd import invert
+    for i, r in enumerate(reqs):
+        executor.set_target(i, r.target)
+        r.meta.clear()
+    machines = {i: invert.gn_machine(r, n_outer=n_outer, n_cg=N_CG, n_scales=N_SCALES) for i, r in enumerate(reqs)}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if executor.max_batch == 1:
+        res = {}
+        for i in machines:
+            res.update(invert.run_machines({i: machines[i]}, executor))
+    else:
+        res = invert.run_machines(machines, executor)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, [res[i] for i in range(len(reqs))]
+
+
+def kernel_roofline(P, B, g, reps=3):
+    """Average launch time of each step kernel on a batch plan, from HIP events
+    around its captured time loop (launch stream = torch's current stream)."""
+    import torch
+    from seismic_amd import synth
+    vec = torch.tensor(np.concatenate([synth.to_vec(synth.curve_fault_model(g.nz, g.nx, i)) for i in range(B)]),
+                       dtype=torch.float32, device="cuda")
+    dd = P.forward(vec).clone()
+    s = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    t_fwd = timed(lambda: P.forward(vec))
+    t_tan = timed(lambda: P.tangent(vec, vec))
+    t_grad = timed(lambda: P.gradient(vec, dd))
+    cells = B * g.n_shots * g.padded_cells
+    out = {}
+    for kind, t_pass, steps in [("forward", t_fwd, g.nt + 1), ("tangent", t_tan, g.nt + 1),
+                                ("adjoint", t_grad - t_fwd, g.nt)]:
+        k = P.get_config(kind)[2]
+        launches = math.ceil(steps / k)
+        out[kind] = {"pass_s": t_pass, "launches": launches, "launch_s": t_pass / launches,
+                     "cell_updates_per_launch": cells * k, "k": k, "window": P.get_config(kind)[:2]}
+    return out
+
+
+def cpu_baseline(g, seconds_hint=25.0):
+    """Oracle (NumPy float64 restatement of the reference propagator) on the
+    host: one forward pass and one forward+adjoint gradient pass over all
+    shots, one process per shot.  Scaled to GN inversions/s with the pass
+    counts of the GPU workload (tangent pass priced at forward+adjoint)."""
+    from concurrent.futures import ProcessPoolExecutor
+    from oracle import fd2d
+    from seismic_amd import synth
+    og = fd2d.Geometry(nz=g.nz, nx=g.nx, nbc=g.nbc, nt=g.nt, dt=g.dt, dx=g.dx, freq=g.freq)
+    vec = synth.to_vec(synth.curve_fault_model(g.nz, g.nx, 0))
+    target = fd2d.forward_only(synth.to_vec(synth.depth_linear_start(g.nz, g.nx)), og).reshape(-1)
+    n = og.n_shots
+    workers = min(n, len(os.sched_getaffinity(0)))
+    with ProcessPoolExecutor(workers) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(_cpu_fwd, [(vec, og, s) for s in range(n)]))
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        list(ex.map(_cpu_fev, [(vec, og, s, target) for s in range(n)]))
+        t_fev = time.perf_counter() - t0
+    t_gnp = t_fev + t_fev - t_fwd           # tangent ~ forward + adjoint cost, then adjoint
+    t_inv = N_OUTER * (t_fev + N_CG * t_gnp + N_SCALES * t_fwd)
+    return {"value": 1.0 / t_inv, "unit": "inversions/s", "cores": workers, "kind": "port",
+            "sample": f"NumPy float64 oracle, {g.nz}x{g.nx} nt={g.nt}, {n} shots in {workers} processes: "
+                      f"forward {t_fwd:.2f} s, forward+adjoint gradient {t_fev:.2f} s; "
+                      f"scaled to {N_OUTER} GN passes x ({N_CG} CG + {N_SCALES} line-search forwards)"}
+
+
+def _cpu_fwd(a):
+    from oracle import fd2d
+    vec, og, s = a
+    return fd2d.forward_only(vec, og, shots=[s])
+
+
+def _cpu_fev(a):
+    from oracle import fd2d
+    vec, og, s, target = a
+    tgt = target.reshape(og.n_shots, og.nt, og.n_rcv)[s].reshape(-1)
+    return fd2d.vel_to_seis(vec, og, vec_adjoint=tgt, adjoint_on_residual=True, shots=[s])[2]
+
+
+def serving_run(n_req, rate, device, seed=100):
+    """Served LBFGS inversions (30 iterations, memory 10, smoothness prior) of
+    mixed grid sizes arriving as a Poisson process."""
+    import torch
+    from seismic_amd import invert, priors, serve, synth
+    from seismic_amd.geometry import Geometry
+    from seismic_amd.propagator import Propagator
+    sizes = [(70, 70), (100, 100), (140, 140)]
+    mix = [0] * (n_req // 2) + [1] * (n_req // 3) + [2] * (n_req - n_req // 2 - n_req // 3)
+    geoms = {s: Geometry(nz=s[0], nx=s[1], nt=1000) for s in sizes}
+    trues = [synth.curve_fault_model(*sizes[si], seed + i) for i, si in enumerate(mix)]
+    targets = [None] * n_req
+    for si, sz in enumerate(sizes):
+        ids = [i for i, k in enumerate(mix) if k == si]
+        if not ids:
+            continue
+        Pf = Propagator([geoms[sz]] * len(ids), caps=1)
+        d = Pf.forward(torch.tensor(np.concatenate([synth.to_vec(trues[i]) for i in ids])))
+        for j, i in enumerate(ids):
+            targets[i] = d[int(Pf.seis_off[j]):int(Pf.seis_off[j + 1])].clone()
+        Pf.close()
+
+    def make_reqs():
+        out = []
+        for i, si in enumerate(mix):
+            g = geoms[sizes[si]]
+            start = synth.to_vec(synth.depth_linear_start(g.nz, g.nx), min_vel=1500.0)
+            out.append(invert.Request(geom=g, target=targets[i], prior=priors.L2Smoothness(g.nz, g.nx, lam=1e-8),
+                                      start=torch.tensor(start, device=device), rid=i))
+        return out
+
+    reqs = make_reqs()
+    slots = [geoms[sizes[si]] for si in mix]
+    srv = serve.InversionServer(slots, hbm_budget=200 * 2 ** 30, ckpt_every=50, max_iter=30, history_size=10)
+    srv.kw["max_iter"] = 1                      # capture the server's graphs before timing
+    srv.run(make_reqs(), [0.0] * n_req)
+    srv.kw["max_iter"] = 30
+    srv.log.clear()
+    arrivals = serve.poisson_arrivals(n_req, rate, seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = srv.run(reqs, arrivals)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = serve.percentiles([r.latency for r in recs])
+    # plain path: the same LBFGS inversion, one 70x70 request alone
+    g = geoms[(70, 70)]
+    ex = invert.Executor([g], ckpt_every=0, max_batch=1)
+    r0 = reqs[mix.index(0)]
+    ex.set_target(0, r0.target)
+    r0.meta.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    invert.run_machines({0: invert.lbfgs_machine(r0, max_iter=30, history_size=10)}, ex)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    return {"requests": n_req, "rate_req_per_s": rate, "sizes": [f"{a}x{b}" for a, b in sizes],
+            "inversions_per_s": n_req / wall, "wall_s": wall, "latency_s": lat,
+            "max_in_flight": max(n for _, n in srv.log) if srv.log else 0,
+            "plain_70x70_inversions_per_s": 1.0 / t_plain,
+            "failed": int(sum(r.failed for r in recs))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="GN passes timed (10 = one full inversion)")
+    ap.add_argument("--warmup", type=int, default=1, help="GN passes run untimed first")
+    ap.add_argument("--batch", type=int, default=16, help="models per batched launch")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--lam", type=float, default=1e-8)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); inversions are independent, "
+              "so N GPUs would run N copies of this workload", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from seismic_amd import invert
+    from seismic_amd.geometry import Geometry
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    dev = "cuda"
+    g = Geometry(nt=1000)
+    B = args.batch
+    log(f"[bench] building {B} requests")
+    reqs, trues = make_requests(B, g, 0, dev, args.lam)
+
+    # ---- batched path
+    ex_b = invert.Executor([g] * B, ckpt_every=0)
+    log(f"[bench] warmup {args.warmup} GN pass(es), batch {B}")
+    run_gn(ex_b, reqs, args.warmup)
+    log(f"[bench] timing {args.steps} GN pass(es), batch {B}")
+    torch.cuda.synchronize()
+    t_b, v_b = run_gn(ex_b, reqs, args.steps)
+    value = B * (args.steps / N_OUTER) / t_b
+    mis_b = [r.meta["stats"]["misfit"] for r in reqs]
+    hist_b0 = list(reqs[0].meta["stats"]["misfit_hist"])
+    log(f"[bench] batched: {t_b:.2f} s, {value:.4f} inversions/s")
+
+    # ---- roofline of the step kernels on the batched plan
+    P = ex_b._grad_plan(list(range(B)))
+    P.set_active([True] * B)
+    P._mask = tuple([True] * B)
+    kr = kernel_roofline(P, B, g)
+    per_outer = {"forward": 1 + N_SCALES, "tangent": N_CG, "adjoint": 1 + N_CG}
+    share = {k: per_outer[k] * kr[k]["pass_s"] for k in kr}
+    dom = max(share, key=share.get)
+    kd = kr[dom]
+    achieved = ALG_BYTES[dom] * kd["cell_updates_per_launch"] / kd["launch_s"]
+    log(f"[bench] kernels: " + ", ".join(f"{k} {v['launch_s']*1e6:.1f} us/launch" for k, v in kr.items()))
+
+    # ---- plain path: same kernels, one model at a time
+    ex_p = invert.Executor([g], ckpt_every=0, max_batch=1)
+    preq = [reqs[0]]
+    run_gn(ex_p, preq, 1)
+    log(f"[bench] timing plain path, {args.plain_steps} GN pass(es)")
+    t_p, v_p = run_gn(ex_p, preq, args.plain_steps)
+    plain = 1 * (args.plain_steps / N_OUTER) / t_p
+    log(f"[bench] plain: {t_p:.2f} s, {plain:.4f} inversions/s")
+
+    serving = None
+    if not args.no_serve:
+        log(f"[bench] serving {args.serve_requests} LBFGS requests at {args.serve_rate}/s")
+        serving = serving_run(args.serve_requests, args.serve_rate, dev)
+        log(f"[bench] serving: {serving['inversions_per_s']:.3f} inv/s p50 {serving['latency_s']['p50']:.2f} s")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(g)
+
+    out = {
+        "metric": "inversions/sec batched+served vs plain at fixed misfit; p50/p99 latency",
+        "value": value,
+        "unit": "inversions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_b / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded curved-fault velocity models; data from the GPU forward of the true model)",
+        "config": {"workload": "Gauss-Newton inversion, 70x70 dx=10 m, nbc=120, 5 shots, 70 receivers, "
+                               "nt=1000 dt=1 ms, 10 GN x 20 CG + 50-point line search, L2 smoothness prior",
+                   "batch_models": B, "step": "one GN pass over the batch", "ckpt_every": 0},
+        "plain": {"inversions_per_s": plain, "seconds_per_gn_pass": t_p / args.plain_steps},
+        "speedup_vs_plain": value / plain,
+        "misfit": {"batched_mean_final": float(np.mean(mis_b)), "after_gn_passes": args.steps,
+                   "model0_batched": hist_b0[:args.plain_steps],
+                   "model0_plain": preq[0].meta["stats"]["misfit_hist"],
+                   "equal": hist_b0[:args.plain_steps] == preq[0].meta["stats"]["misfit_hist"]},
+        "serving": serving,
+        "roofline": {"bound": "hbm", "kernel": f"k_{'fwd' if dom != 'adjoint' else 'adj'}"
+                     f"{'(tangent)' if dom == 'tangent' else ''}",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "launch_us": kd["launch_s"] * 1e6, "alg_bytes_per_cell_update": ALG_BYTES[dom],
+                     "cell_updates_per_launch": kd["cell_updates_per_launch"], "k": kd["k"],
+                     "window": kd["window"]},
+        "kernels": {k: {"launch_us": v["launch_s"] * 1e6, "k": v["k"],
+                        "gcell_updates_per_s": v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
+                        "alg_GBps": ALG_BYTES[k] * v["cell_updates_per_launch"] / v["launch_s"] / 1e9}
+                    for k, v in kr.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

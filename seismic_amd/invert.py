@@ -329,6 +329,7 @@ def gn_machine(req: Request, n_outer: int = 10, n_cg: int = 20, n_scales: int = 
         best = float(scales[int(np.argmin(costs))])
         vec = vec + best * prior.from_basis(dx)
         stats["misfit"] = min(costs)
+        stats.setdefault("misfit_hist", []).append(min(costs))
     return vec
 
 
@@ -408,7 +409,11 @@ class Executor:
         P = self._grad_plan(ids)
         if self.max_batch == 1:
             return P, [0]
-        P.set_active([i in set(ids) for i in range(self.n)])
+        live = set(ids)
+        mask = tuple(i in live for i in range(self.n))
+        if getattr(P, "_mask", None) != mask:
+            P.set_active(mask)
+            P._mask = mask
         return P, list(ids)
 
     def _fev(self, ids, pending):

@@ -1,0 +1,149 @@
+"""Continuous-batching inversion server on one GPU.
+
+Requests (``invert.Request``) arrive over time.  The server owns one plan
+whose slots cover a fixed mix of grid sizes; a request is admitted into a
+free slot of its size when its HBM footprint (its share of the plan:
+wavefields, checkpoints, coefficient fields, data; plus its LBFGS history)
+fits the budget.  Every loop iteration collects the pending propagator op
+of every in-flight request and runs them as one batched launch per op kind;
+requests that arrive meanwhile join at the next iteration (continuous
+batching, not a queue of whole inversions).  Per-request state (model,
+gradients, LBFGS pairs, observed data) stays on the device.
+
+The reference has no serving layer: it inverts samples one by one, one
+process per GPU (core/kaggle_support.py:482-495).
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Callable, Sequence
+
+import numpy as np
+import torch
+
+from ._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
+from .geometry import Geometry
+from .invert import Executor, Request, lbfgs_machine
+from .propagator import plan_bytes
+
+
+@dataclass
+class Served:
+    rid: int
+    arrival: float
+    admitted: float = 0.0
+    finished: float = 0.0
+    result: torch.Tensor | None = None
+    stats: dict = field(default_factory=dict)
+    failed: bool = False
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def request_footprint(g: Geometry, ckpt_every: int, history: int) -> int:
+    """Device bytes one request holds: its slot of the plan plus LBFGS pairs."""
+    slot = plan_bytes([g], ckpt_every=ckpt_every, caps=CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT)
+    n = g.n_params
+    lbfgs = 8 * n * (2 * history + 8)        # (s, y) pairs + x, d, g, prev g, trial points
+    return slot + lbfgs + 4 * g.n_data
+
+
+class InversionServer:
+    """Admits LBFGS inversion requests into fixed slots and batches their fevs."""
+
+    def __init__(self, slots: Sequence[Geometry], hbm_budget: int, ckpt_every: int = 50,
+                 max_iter: int = 30, history_size: int = 10, tolerance_grad: float = 1e-7,
+                 tolerance_change: float = 0.0, device="cuda"):
+        self.slots = list(slots)
+        self.ckpt_every = ckpt_every
+        self.kw = dict(max_iter=max_iter, history_size=history_size, tolerance_grad=tolerance_grad,
+                       tolerance_change=tolerance_change)
+        self.budget = int(hbm_budget)
+        self.ex = Executor(self.slots, ckpt_every=ckpt_every, device=device)
+        self._fp = {g: request_footprint(g, ckpt_every, history_size) for g in set(self.slots)}
+        self.used = 0
+        self.free: dict = {}
+        for i, g in enumerate(self.slots):
+            self.free.setdefault(g, []).append(i)
+        self.log: list = []
+
+    def footprint(self, req: Request) -> int:
+        return self._fp[req.geom]
+
+    def _try_admit(self, req: Request):
+        free = self.free.get(req.geom)
+        if not free:
+            return None
+        if self.used + self.footprint(req) > self.budget:
+            return None
+        slot = free.pop()
+        self.used += self.footprint(req)
+        return slot
+
+    def _release(self, slot: int, req: Request):
+        self.free[self.slots[slot]].append(slot)
+        self.used -= self.footprint(req)
+
+    def run(self, requests: Sequence[Request], arrivals: Sequence[float],
+            on_event: Callable | None = None) -> list:
+        """Serve `requests` arriving at `arrivals` (seconds from start).
+        Returns one Served record per request (same order)."""
+        order = np.argsort(arrivals, kind="stable")
+        queue = deque((requests[i], float(arrivals[i]), int(i)) for i in order)
+        out = [Served(rid=requests[i].rid, arrival=float(arrivals[i])) for i in range(len(requests))]
+        waiting: deque = deque()
+        active: dict = {}
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while queue or waiting or active:
+            t = now()
+            while queue and queue[0][1] <= t:
+                waiting.append(queue.popleft())
+            while waiting:
+                req, arr, idx = waiting[0]
+                slot = self._try_admit(req)
+                if slot is None:
+                    break
+                waiting.popleft()
+                self.ex.set_target(slot, req.target)
+                m = lbfgs_machine(req, **self.kw)
+                active[slot] = [req, idx, m, next(m)]
+                out[idx].admitted = now()
+            if not active:
+                if queue:
+                    time.sleep(max(0.0, queue[0][1] - now()))
+                continue
+            pending = {s: v[3] for s, v in active.items()}
+            results = self.ex.execute(pending)
+            self.log.append((now(), len(pending)))
+            for s, res in results.items():
+                req, idx, m, _ = active[s]
+                try:
+                    active[s][3] = m.send(res)
+                except StopIteration as e:
+                    rec = out[idx]
+                    rec.finished = now()
+                    rec.result = e.value
+                    rec.stats = dict(req.meta.get("stats", {}))
+                    rec.failed = bool(torch.isnan(e.value).any())
+                    del active[s]
+                    self._release(s, req)
+                    if on_event:
+                        on_event(rec)
+        return out
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
+    """Arrival times of a Poisson process with `rate` requests per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, n)) - 0.0
+
+
+def percentiles(latencies: Sequence[float]) -> dict:
+    a = np.asarray(latencies, dtype=np.float64)
+    return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)),
+            "mean": float(a.mean()), "max": float(a.max())}

@@ -1,0 +1,96 @@
+"""Host logic of the inversion loops, on the CPU: the product's machines
+(seismic_amd.invert) driven by the FP64 oracle propagator must reproduce the
+oracle's restatements of core/seis_numerics.py:307-448 (bfgs) and
+core/seis_invert.py:63-152 (GN, LBFGS) step for step."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd2d
+from oracle import invert as oinv
+from oracle import priors as opri
+from seismic_amd import invert, priors, synth
+from seismic_amd.geometry import Geometry
+
+KW = dict(nz=10, nx=12, nbc=8, nt=160)
+
+
+class OracleExecutor:
+    """Executes machine ops with the FP64 oracle propagator on the CPU."""
+
+    def __init__(self, og, target):
+        self.og, self.t = og, target
+
+    def execute(self, pending):
+        out = {}
+        for s, op in pending.items():
+            if op.kind == "fev":
+                seis, _, gr = fd2d.vel_to_seis(op.vec.numpy(), self.og, vec_adjoint=self.t,
+                                               adjoint_on_residual=True)
+                out[s] = (float(np.sum((seis - self.t) ** 2)), torch.tensor(gr))
+            elif op.kind == "gnp":
+                _, jdv, _ = fd2d.vel_to_seis(op.vec.numpy(), self.og, vec_diff=op.dvec.numpy())
+                out[s] = torch.tensor(fd2d.vel_to_seis(op.vec.numpy(), self.og, vec_adjoint=jdv)[2])
+            else:
+                out[s] = [float(np.sum((fd2d.vel_to_seis(v.numpy(), self.og)[0] - self.t) ** 2))
+                          for v in op.vecs]
+        return out
+
+
+def _setup(seed=0):
+    g, og = Geometry(**KW), fd2d.Geometry(**KW)
+    true = synth.make_model("flat", g.nz, g.nx, seed)
+    target = fd2d.vel_to_seis(synth.to_vec(true), og)[0]
+    start = synth.to_vec(synth.depth_linear_start(g.nz, g.nx, true.min(), true.max()), min_vel=true.min())
+    return g, og, target, start
+
+
+@pytest.mark.parametrize("hist", [None, 3])
+def test_lbfgs_machine_equals_oracle(hist):
+    g, og, target, start = _setup(1)
+    ref, nfev = oinv.lbfgs_invert(start, target, opri.RowTotalVariation(g.nz, g.nx), og, max_iter=6,
+                                  tolerance_change=1e-9, history_size=hist)
+    r = invert.Request(geom=g, target=torch.tensor(target), prior=priors.RowTotalVariation(g.nz, g.nx, device="cpu"),
+                       start=torch.tensor(start))
+    m = invert.lbfgs_machine(r, max_iter=6, tolerance_change=1e-9, history_size=hist)
+    out = invert.run_machines({0: m}, OracleExecutor(og, target))[0].numpy()
+    assert r.meta["stats"]["fev"] == nfev
+    np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-7)
+
+
+def test_gn_machine_equals_oracle():
+    g, og, target, start = _setup(2)
+    op = opri.L2Smoothness(g.nz, g.nx, lam=1e-9)
+    ref, _, _ = oinv.gauss_newton_cg(start, target, op, og, n_cg=4, n_scales=7)
+    r = invert.Request(geom=g, target=torch.tensor(target),
+                       prior=priors.L2Smoothness(g.nz, g.nx, lam=1e-9, device="cpu"), start=torch.tensor(start))
+    m = invert.gn_machine(r, n_outer=1, n_cg=4, n_scales=7)
+    out = invert.run_machines({0: m}, OracleExecutor(og, target))[0].numpy()
+    np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-7)
+    st = r.meta["stats"]
+    assert (st["fev"], st["gnp"], st["fwd"]) == (1, 4, 7) and st["misfit_hist"] == [st["misfit"]]
+
+
+def test_priors_match_oracle():
+    rng = np.random.default_rng(0)
+    for P, O in [(priors.RowTotalVariation, opri.RowTotalVariation), (priors.TotalVariation, opri.TotalVariation),
+                 (priors.L2Smoothness, opri.L2Smoothness)]:
+        p, o = P(7, 9, lam=3.0, device="cpu"), O(7, 9, lam=3.0)
+        x = rng.normal(0, 50, p.N)
+        c, gr = p.cost_grad(torch.tensor(x))
+        c2, g2 = o.cost_grad(x)
+        assert float(c) == pytest.approx(c2, rel=1e-12)
+        np.testing.assert_allclose(gr.numpy(), g2, rtol=1e-12, atol=1e-15)
+    se = priors.SquaredExponential(6, 8, lam=2.0, device="cpu")
+    x = torch.tensor(rng.normal(0, 50, 6 * 8 + 1))
+    d = 1e-6 * torch.tensor(rng.normal(0, 1, 6 * 8 + 1))
+    c, gr = se.cost_grad(x)
+    c2, _ = se.cost_grad(x + d)
+    assert abs(float((gr * d).sum()) - float(c2 - c)) / abs(float(c2 - c)) < 1e-5
+
+
+def test_l2_apply_P_matches_dense():
+    p = priors.L2Smoothness(5, 6, device="cpu")
+    o = opri.L2Smoothness(5, 6)
+    x = np.random.default_rng(1).normal(size=30)
+    np.testing.assert_allclose(p.apply_P(torch.tensor(x)).numpy(), o.P @ x, rtol=1e-12)
