@@ -146,8 +146,9 @@ enum {
 /* Launch configuration of one kernel kind: an rx x rz cell window (halo
  * included) is staged in LDS per workgroup and advanced k time steps per
  * launch; the window's interior, (rz-4k) x (rx-4k) cells, is written back.
- * Supported windows: 64x32, 64x64, 128x32; 1 <= k <= 8 with an interior of
- * at least 4 rows and 8 columns.  Results do not depend on the config
+ * Windows (rx x rz): forward 64x32, 128x32, 64x64, 128x64, 256x32; tangent and
+ * adjoint 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4
+ * rows and 8 columns.  Results do not depend on the config
  * (bit-identical wavefields).  No reference counterpart: the reference runs
  * one 32x32-thread launch per step (core/seis_forward2.py:191-193).
  * Setting a config synchronizes the device and drops captured graphs. */

@@ -191,12 +191,17 @@ __device__ __forceinline__ int rcv_index(int gx, const ModelDev& m) {
 // (1-kappa) in registers across the K steps; p (and dp) live in LDS.  No
 // global load is issued inside the step loop: the wavelet samples of the
 // launch are staged in LDS with the window.
-template <int RX, int RZ, bool TAN>
-__global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
-  constexpr int CPR = RX / 64, RPW = RZ / 4, NC = CPR * RPW;
+template <int RX, int RZ, int NT, bool TAN>
+__global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
+  constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
+  static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
-  __shared__ float P[RZ][RX];
-  __shared__ float D[TAN ? RZ : 1][TAN ? RX : 1];
+  // LDS windows carry a 2-cell apron on every side so that every owned
+  // cell's stencil stays inside the array; the apron is never read for a
+  // committed cell.  Logical window cell (r, c) lives at [r + 2][c + 2].
+  constexpr int PW = RX + 4, PH = RZ + 4;
+  __shared__ float P[PH][PW];
+  __shared__ float D[TAN ? PH : 1][TAN ? PW : 1];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ntiles = *a.n_tiles;
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-      const int r = wv + 4 * k;
+      const int r = wv + NW * k;
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << k;
 #pragma unroll
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
         const int cc = lane + 64 * c;
         const int gx = wrapi(ox + cc, NX);
         const int g = gz * NX + gx;
-        P[r][cc] = pin[g];
+        P[r + 2][cc + 2] = pin[g];
         q[i] = qin[g];
         al[i] = alpha[g];
         const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
         k1[i] = 1.0f - kscale * jj;
         if (g == w.src_cell) srcmask |= 1u << i;
         if constexpr (TAN) {
-          D[r][cc] = dpin[g];
+          D[r + 2][cc + 2] = dpin[g];
           dq[i] = dqin[g];
           dal[i] = a.dalpha[m.cell_off + g];
           dk[i] = dkscale * jj;
@@ -269,38 +274,44 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
       const float src = amp * sv;
       const float srcd = amp_d * sv;
       const int lo = 2 * (s + 1);
+      // Forward: every owned cell is computed (its stencil stays inside the
+      // padded window) and only the valid band [lo, RZ-lo) x [lo, RX-lo) is
+      // committed, which keeps the cell loop branch-free.  Tangent: rows
+      // outside the band are skipped (two stencils per cell would otherwise
+      // exceed the register budget).
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
-        const int r = wv + 4 * k;
-        if (r < lo || r >= RZ - lo) continue;  // wave-uniform
+        const int r = wv + NW * k;
+        const bool rok = r >= lo && r < RZ - lo;
+        if constexpr (TAN) {
+          if (!rok) continue;
+        }
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          // every column is computed (its stencil stays inside P); only the
-          // valid band [lo, RX-lo) is committed
-          const bool ok = cc >= lo && cc < RX - lo;
-          const float L = lap_full<RX>(P, r, cc);
+          const bool ok = rok && cc >= lo && cc < RX - lo;
+          const float L = lap_full<PW>(P, r + 2, cc + 2);
           float qn = fmaf(k1[i], q[i], al[i] * L);
           if ((srcmask >> i) & 1u) qn = qn + src;
           if constexpr (TAN) {
             // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
-            const float Ld = lap_full<RX>(D, r, cc);
+            const float Ld = lap_full<PW>(D, r + 2, cc + 2);
             float dqn = fmaf(k1[i], dq[i], al[i] * Ld);
             dqn = fmaf(dal[i], L, dqn);
             dqn = fmaf(-dk[i], q[i], dqn);
             if ((srcmask >> i) & 1u) dqn = dqn + srcd;
-            dpn[i] = ok ? D[r][cc] + dqn : dpn[i];
+            dpn[i] = ok ? D[r + 2][cc + 2] + dqn : dpn[i];
             dq[i] = ok ? dqn : dq[i];
           }
-          pn[i] = ok ? P[r][cc] + qn : pn[i];
+          pn[i] = ok ? P[r + 2][cc + 2] + qn : pn[i];
           q[i] = ok ? qn : q[i];
         }
       }
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
-        const int r = wv + 4 * k;
+        const int r = wv + NW * k;
         if (r < lo || r >= RZ - lo) continue;
         const int zz = oz + r;
         const bool zin = r >= H && r < RZ - H && zz < NZ;
@@ -309,8 +320,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
           const int i = k * CPR + c;
           const int cc = lane + 64 * c;
           if (cc < lo || cc >= RX - lo) continue;
-          P[r][cc] = pn[i];
-          if constexpr (TAN) D[r][cc] = dpn[i];
+          P[r + 2][cc + 2] = pn[i];
+          if constexpr (TAN) D[r + 2][cc + 2] = dpn[i];
           const int xx = ox + cc;
           if (!(zin && cc >= H && cc < RX - H && xx < NX)) continue;
           const int g = zz * NX + xx;
@@ -331,7 +342,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd(FwdArgs a) {
     float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-      const int r = wv + 4 * k;
+      const int r = wv + NW * k;
       const int zz = oz + r;
       if (r < H || r >= RZ - H || zz >= NZ) continue;
 #pragma unroll
@@ -385,11 +396,13 @@ struct AdjArgs {
 // halo) whose global loads are issued before the lam update of each step.
 // The accumulators see the same fused multiply-add sequence as a per-step
 // read-modify-write, so results do not depend on K.
-template <int RX, int RZ>
-__global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
-  constexpr int CPR = RX / 64, RPW = RZ / 4, NC = CPR * RPW;
+template <int RX, int RZ, int NT>
+__global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
+  constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
+  static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
-  __shared__ float A[RZ][RX];
+  constexpr int PW = RX + 4, PH = RZ + 4;     // alpha*lam window with a 2-cell apron
+  __shared__ float A[PH][PW];
   __shared__ float PT[2][RZ][RX];
   __shared__ float INJ[KMAX][RX];
   __shared__ float SW[KMAX];
@@ -423,7 +436,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
       SW[threadIdx.x] = ((int)threadIdx.x < a.K && t >= 0 && t < a.nt) ? a.wavelet[t] : 0.f;
     }
     // adjoint sources of this launch's steps along the window's columns
-    for (int idx = threadIdx.x; idx < a.K * RX; idx += NTHR) {
+    for (int idx = threadIdx.x; idx < a.K * RX; idx += NT) {
       const int s = idx / RX, cc = idx - (idx / RX) * RX;
       const int rr = rcv_index(wrapi(ox + cc, NX), m);
       float v = 0.f;
@@ -435,7 +448,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-      const int r = wv + 4 * k;
+      const int r = wv + NW * k;
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << k;
       const int zz = oz + r;
@@ -448,7 +461,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
         lam[i] = lin[g];
         mu[i] = min_[g];
         al[i] = alpha[g];
-        A[r][cc] = al[i] * lam[i];
+        A[r + 2][cc + 2] = al[i] * lam[i];
         const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
         k1[i] = 1.0f - kscale * (float)(j * j);
         if (g == w.src_cell) srcmask |= 1u << i;
@@ -473,7 +486,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
     if (src_here) ssum = a.wparam[te.w].ssum;
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-      const int r = wv + 4 * k;
+      const int r = wv + NW * k;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int cc = lane + 64 * c;
@@ -489,7 +502,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
       const float* pnext = pbase + (int64_t)(mm - 2 - a.p_base) * Pn;
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
-        const int r = wv + 4 * k;
+        const int r = wv + NW * k;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int cc = lane + 64 * c;
@@ -498,15 +511,16 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
-        const int r = wv + 4 * k;
-        if (r < lo || r >= RZ - lo) continue;  // wave-uniform
+        const int r = wv + NW * k;
+        const bool rok = r >= lo && r < RZ - lo;
+        if (!rok) continue;  // wave-uniform
         const bool rrow = (rcvmask >> k) & 1u;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          const bool ok = cc >= lo && cc < RX - lo;
-          const float La = lap_full<RX>(A, r, cc);
+          const bool ok = rok && cc >= lo && cc < RX - lo;
+          const float La = lap_full<PW>(A, r + 2, cc + 2);
           float mun = fmaf(k1[i], mu[i], La);
           if (rrow && rcol[c] >= 0) mun = mun + INJ[s][cc];
           ln[i] = ok ? lam[i] + mun : ln[i];
@@ -515,7 +529,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
-        const int r = wv + 4 * k;
+        const int r = wv + NW * k;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int cc = lane + 64 * c;
@@ -526,14 +540,14 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
       const float sw = SW[s];
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
-        const int r = wv + 4 * k;
+        const int r = wv + NW * k;
         if (r < lo || r >= RZ - lo) continue;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
           const int cc = lane + 64 * c;
           if (cc < lo || cc >= RX - lo) continue;
-          A[r][cc] = al[i] * ln[i];
+          A[r + 2][cc + 2] = al[i] * ln[i];
           lam[i] = ln[i];
           if (!((intmask >> i) & 1u)) continue;
           const int pr = r - H + 2, pc = cc - H + 2;
@@ -551,7 +565,7 @@ __global__ __launch_bounds__(NTHR) void k_adj(AdjArgs a) {
     float* mout = a.arena + w.off[R_MU] + a.mu_out_slot * Pn;
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-      const int r = wv + 4 * k;
+      const int r = wv + NW * k;
       const int zz = oz + r;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
@@ -788,12 +802,26 @@ struct KCfg {
   int rx, rz, k;
 };
 
-bool shape_ok(int rx, int rz) {
-  return (rx == 64 && rz == 32) || (rx == 64 && rz == 64) || (rx == 128 && rz == 32);
+// Instantiated windows per kernel kind: rx x rz cells on `nt` threads, 8
+// cells per thread for the forward and adjoint kernels.  The tangent and the
+// adjoint carry about twice the per-cell state, which caps them at 512 threads.
+struct Shape {
+  int rx, rz, nt;
+};
+const Shape kShapes[SEIS_KIND_COUNT][5] = {
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}},
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}},
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}}};
+
+int shape_threads(int kind, int rx, int rz) {
+  for (const Shape& sh : kShapes[kind])
+    if (sh.rx == rx && sh.rz == rz && sh.nt) return sh.nt;
+  return 0;
 }
 
-bool cfg_ok(const KCfg& c) {
-  return shape_ok(c.rx, c.rz) && c.k >= 1 && c.k <= 8 && c.rz - 4 * c.k >= 4 && c.rx - 4 * c.k >= 8;
+bool cfg_ok(int kind, const KCfg& c) {
+  return shape_threads(kind, c.rx, c.rz) && c.k >= 1 && c.k <= 8 && c.rz - 4 * c.k >= 4 &&
+         c.rx - 4 * c.k >= 8;
 }
 
 const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3}, {64, 32, 3}, {64, 32, 3}};
@@ -963,37 +991,39 @@ int dev_alloc(T** ptr, int64_t count) {
 }
 
 // ---- kernel dispatch over the instantiated window shapes
-template <int RX, int RZ>
-const void* kernel_ptr_t(int kind) {
-  if (kind == SEIS_KIND_FORWARD) return (const void*)k_fwd<RX, RZ, false>;
-  if (kind == SEIS_KIND_TANGENT) return (const void*)k_fwd<RX, RZ, true>;
-  return (const void*)k_adj<RX, RZ>;
-}
+#define SEIS_FWD_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512) X(128, 64, 1024) X(256, 32, 1024)
+#define SEIS_TAN_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
+#define SEIS_ADJ_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
 
 const void* kernel_ptr(int kind, const KCfg& c) {
-  if (c.rx == 64 && c.rz == 32) return kernel_ptr_t<64, 32>(kind);
-  if (c.rx == 64 && c.rz == 64) return kernel_ptr_t<64, 64>(kind);
-  return kernel_ptr_t<128, 32>(kind);
-}
-
-template <int RX, int RZ>
-void launch_fwd_t(bool tan, int grid, hipStream_t s, const FwdArgs& a) {
-  if (tan)
-    hipLaunchKernelGGL((k_fwd<RX, RZ, true>), dim3(grid), dim3(NTHR), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_fwd<RX, RZ, false>), dim3(grid), dim3(NTHR), 0, s, a);
+#define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_fwd<RX, RZ, NT, false>;
+#define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_fwd<RX, RZ, NT, true>;
+#define PTR_A(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_adj<RX, RZ, NT>;
+  if (kind == SEIS_KIND_FORWARD) { SEIS_FWD_SHAPES(PTR_F) }
+  else if (kind == SEIS_KIND_TANGENT) { SEIS_TAN_SHAPES(PTR_T) }
+  else { SEIS_ADJ_SHAPES(PTR_A) }
+#undef PTR_F
+#undef PTR_T
+#undef PTR_A
+  return nullptr;
 }
 
 void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
-  if (c.rx == 64 && c.rz == 32) launch_fwd_t<64, 32>(tan, grid, s, a);
-  else if (c.rx == 64 && c.rz == 64) launch_fwd_t<64, 64>(tan, grid, s, a);
-  else launch_fwd_t<128, 32>(tan, grid, s, a);
+#define LF(RX, RZ, NT) \
+  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_fwd<RX, RZ, NT, false>), dim3(grid), dim3(NT), 0, s, a); return; }
+#define LT(RX, RZ, NT) \
+  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_fwd<RX, RZ, NT, true>), dim3(grid), dim3(NT), 0, s, a); return; }
+  if (tan) { SEIS_TAN_SHAPES(LT) }
+  else { SEIS_FWD_SHAPES(LF) }
+#undef LF
+#undef LT
 }
 
 void launch_adj(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
-  if (c.rx == 64 && c.rz == 32) hipLaunchKernelGGL((k_adj<64, 32>), dim3(grid), dim3(NTHR), 0, s, a);
-  else if (c.rx == 64 && c.rz == 64) hipLaunchKernelGGL((k_adj<64, 64>), dim3(grid), dim3(NTHR), 0, s, a);
-  else hipLaunchKernelGGL((k_adj<128, 32>), dim3(grid), dim3(NTHR), 0, s, a);
+#define LA(RX, RZ, NT) \
+  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_adj<RX, RZ, NT>), dim3(grid), dim3(NT), 0, s, a); return; }
+  SEIS_ADJ_SHAPES(LA)
+#undef LA
 }
 
 // Tiles of the active wavefields for window config c, laid out so that the
@@ -1047,7 +1077,8 @@ int setup_kind(seis_plan* p, int kind) {
     if (rc) return rc;
   }
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kind, p->cfg[kind]), NTHR, 0));
+  const int nthr = shape_threads(kind, p->cfg[kind].rx, p->cfg[kind].rz);
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kind, p->cfg[kind]), nthr, 0));
   int gmax = std::max(1, per_cu) * p->num_cu;
   gmax = std::max(NXCD, gmax / NXCD * NXCD);
   t.grid = std::max(NXCD, std::min(cap, gmax));
@@ -1320,7 +1351,7 @@ int seis_plan_layout(const seis_plan_t* plan, int64_t* vec_off, int64_t* seis_of
 int seis_plan_set_config(seis_plan_t* p, int32_t kind, int32_t rx, int32_t rz, int32_t k) {
   if (!p || kind < 0 || kind >= SEIS_KIND_COUNT) return fail(SEIS_EINVAL, "bad plan or kind");
   const KCfg c{rx, rz, k};
-  if (!cfg_ok(c))
+  if (!cfg_ok(kind, c))
     return fail(SEIS_EINVAL, "unsupported config: window " + std::to_string(rx) + "x" + std::to_string(rz) +
                                  ", k=" + std::to_string(k));
   HIP_TRY(hipDeviceSynchronize());

@@ -23,6 +23,7 @@ from typing import Any, Callable, Sequence
 import numpy as np
 import torch
 
+from . import autotune
 from ._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
 from .geometry import Geometry
 from .propagator import Propagator, vel_to_seis
@@ -357,7 +358,9 @@ class Executor:
     # plan cache --------------------------------------------------------
     def _plan(self, key, geoms, caps):
         if key not in self._plans:
-            self._plans[key] = Propagator(geoms, ckpt_every=self.ckpt_every, caps=caps, device=self.device)
+            P = Propagator(geoms, ckpt_every=self.ckpt_every, caps=caps, device=self.device)
+            autotune.apply(P)
+            self._plans[key] = P
         return self._plans[key]
 
     def _grad_plan(self, slot_ids):
@@ -464,7 +467,12 @@ def run_machines(machines: dict, executor: Executor, on_finish: Callable | None 
     pending = {}
     results = {}
     for s, m in machines.items():
-        pending[s] = next(m)
+        try:
+            pending[s] = next(m)
+        except StopIteration as e:     # nothing to evaluate (e.g. zero iterations)
+            results[s] = e.value
+            if on_finish:
+                on_finish(s, e.value)
     while pending:
         res = executor.execute(pending)
         for s, r in res.items():

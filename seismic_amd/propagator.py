@@ -198,7 +198,10 @@ def default_propagator(geometry: Geometry | None = None, ckpt_every: int = 0) ->
     g = geometry or Geometry()
     key = (g, ckpt_every)
     if key not in _default:
-        _default[key] = Propagator([g], ckpt_every=ckpt_every)
+        from . import autotune
+        P = Propagator([g], ckpt_every=ckpt_every)
+        autotune.apply(P)
+        _default[key] = P
     return _default[key]
 
 

@@ -88,6 +88,13 @@ class InversionServer:
         self.free[self.slots[slot]].append(slot)
         self.used -= self.footprint(req)
 
+    @staticmethod
+    def _finish(rec: Served, req: Request, value, t: float):
+        rec.finished = t
+        rec.result = value
+        rec.stats = dict(req.meta.get("stats", {}))
+        rec.failed = bool(torch.isnan(value).any())
+
     def run(self, requests: Sequence[Request], arrivals: Sequence[float],
             on_event: Callable | None = None) -> list:
         """Serve `requests` arriving at `arrivals` (seconds from start).
@@ -111,8 +118,14 @@ class InversionServer:
                 waiting.popleft()
                 self.ex.set_target(slot, req.target)
                 m = lbfgs_machine(req, **self.kw)
-                active[slot] = [req, idx, m, next(m)]
                 out[idx].admitted = now()
+                try:
+                    active[slot] = [req, idx, m, next(m)]
+                except StopIteration as e:
+                    self._finish(out[idx], req, e.value, now())
+                    self._release(slot, req)
+                    if on_event:
+                        on_event(out[idx])
             if not active:
                 if queue:
                     time.sleep(max(0.0, queue[0][1] - now()))
@@ -126,10 +139,7 @@ class InversionServer:
                     active[s][3] = m.send(res)
                 except StopIteration as e:
                     rec = out[idx]
-                    rec.finished = now()
-                    rec.result = e.value
-                    rec.stats = dict(req.meta.get("stats", {}))
-                    rec.failed = bool(torch.isnan(e.value).any())
+                    self._finish(rec, req, e.value, now())
                     del active[s]
                     self._release(s, req)
                     if on_event:

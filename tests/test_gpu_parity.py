@@ -126,6 +126,18 @@ def test_tile_configs_bit_identical(small_geoms):
             assert out[3] == ref[3]
     with pytest.raises(ValueError):
         P.set_config("forward", 64, 32, 8)   # interior would be empty
+    with pytest.raises(ValueError):
+        P.set_config("adjoint", 128, 64, 2)  # forward-only window
+
+
+@pytest.mark.parametrize("cfg", [(128, 64, 3), (128, 64, 8), (256, 32, 2), (256, 32, 5)])
+def test_large_forward_windows_bit_identical(small_geoms, cfg):
+    g, og = small_geoms
+    _, vec = _case(g, "curved", 22)
+    P = Propagator([g], ckpt_every=0)
+    ref = _np(P.forward(torch.tensor(vec)))
+    P.set_config("forward", *cfg)
+    assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref)
 
 
 def test_dot_product(small_geoms):
