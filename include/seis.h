@@ -129,6 +129,20 @@ int seis_gradient(seis_plan_t* plan, const float* vec, const float* dd,
                   int32_t on_residual, float* seis, double* misfit,
                   double* grad, seis_stream_t stream);
 
+/* Split gradient for callers that differentiate through the propagator
+ * (autograd): seis_forward_save runs the forward of seis_gradient and keeps
+ * its checkpoints (or full history); seis_adjoint then runs only the reverse
+ * pass from them, with the same meaning of dd / on_residual / misfit / grad
+ * as seis_gradient.  vec must be the vector of the saving forward, and no
+ * other pass may run on the plan in between (the caller tracks this).
+ * Together they replace vel_to_seis(vec) followed by
+ * vel_to_seis(vec, vec_adjoint=dd) (core/seis_forward2.py:146-331). */
+int seis_forward_save(seis_plan_t* plan, const float* vec, float* seis,
+                      seis_stream_t stream);
+int seis_adjoint(seis_plan_t* plan, const float* vec, const float* dd,
+                 int32_t on_residual, double* misfit, double* grad,
+                 seis_stream_t stream);
+
 /* Gauss-Newton product: dseis = J dvec and grad = J^T (J dvec), both at vec.
  * Replaces the dense column-by-column J of core/seis_invert.py:84-90 with
  * one tangent + one adjoint pass per product. */

@@ -39,6 +39,8 @@ EXPORTS = {
     "seis_tangent": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_gradient": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                                 c_void_p]),
+    "seis_forward_save": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "seis_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "seis_gn_product": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_plan_device_bytes": (c_int64, [c_void_p]),
     "seis_plan_set_config": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32]),

@@ -1400,6 +1400,30 @@ int seis_forward_misfit(seis_plan_t* p, const float* vec, const float* dd, float
   return copy_out(p, s, p->d_seis, seis);
 }
 
+int seis_forward_save(seis_plan_t* p, const float* vec, float* seis, seis_stream_t stream) {
+  if (!p || !vec) return fail(SEIS_EINVAL, "NULL argument");
+  if (!(p->caps & SEIS_CAP_ADJOINT)) return fail(SEIS_ESTATE, "plan lacks SEIS_CAP_ADJOINT");
+  hipStream_t s = (hipStream_t)stream;
+  TRY(run_prep(p, s, vec, nullptr));
+  TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
+  HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
+  return copy_out(p, s, p->d_seis, seis);
+}
+
+int seis_adjoint(seis_plan_t* p, const float* vec, const float* dd, int32_t on_residual, double* misfit,
+                 double* grad, seis_stream_t stream) {
+  if (!p || !vec || !dd || !grad) return fail(SEIS_EINVAL, "NULL argument");
+  if (!(p->caps & SEIS_CAP_ADJOINT)) return fail(SEIS_ESTATE, "plan lacks SEIS_CAP_ADJOINT");
+  hipStream_t s = (hipStream_t)stream;
+  TRY(run_prep(p, s, vec, nullptr));   // same alpha as the saving forward; clears the source correlation
+  HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
+  if (on_residual && misfit) launch_misfit(p, s, misfit);
+  const int kind = on_residual ? 0 : 1;
+  TRY(get_graph(p, &p->g_adj[kind], [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
+  HIP_TRY(hipGraphLaunch(p->g_adj[kind], s));
+  return finish_grad(p, s, vec, grad);
+}
+
 int seis_tangent(seis_plan_t* p, const float* vec, const float* dvec, float* seis, float* dseis,
                  seis_stream_t stream) {
   if (!p || !vec || !dvec) return fail(SEIS_EINVAL, "NULL argument");

@@ -99,6 +99,8 @@ class Propagator:
         self.n_vec = int(self.vec_off[-1])
         self.n_seis = int(self.seis_off[-1])
         self.device_bytes = int(L.seis_plan_device_bytes(self._h))
+        self._gen = 0          # bumped by every call that rewrites wavefield state
+        self._saved_gen = -1   # generation of the last forward_save
 
     def close(self):
         if getattr(self, "_h", None):
@@ -148,6 +150,7 @@ class Propagator:
 
     # ------------------------------------------------------------- passes
     def forward(self, vec, out: torch.Tensor | None = None) -> torch.Tensor:
+        self._gen += 1
         v = self._vec(vec)
         seis = out if out is not None else torch.empty(self.n_seis, device=self.device)
         _lib.check(_lib.lib().seis_forward(self._h, _ptr(v), _ptr(seis), _stream()), "seis_forward")
@@ -155,6 +158,7 @@ class Propagator:
 
     def forward_misfit(self, vec, dd, seis_out: torch.Tensor | None = None) -> torch.Tensor:
         """misfit[m] = sum((F(vec) - dd)^2) per model, float64 (n_models,)."""
+        self._gen += 1
         v, d = self._vec(vec), self._data(dd)
         misfit = torch.empty(self.n_models, device=self.device, dtype=torch.float64)
         _lib.check(_lib.lib().seis_forward_misfit(self._h, _ptr(v), _ptr(d), _ptr(seis_out), _ptr(misfit),
@@ -162,6 +166,7 @@ class Propagator:
         return misfit
 
     def tangent(self, vec, dvec):
+        self._gen += 1
         v, dv = self._vec(vec), self._vec(dvec)
         seis = torch.empty(self.n_seis, device=self.device)
         dseis = torch.empty(self.n_seis, device=self.device)
@@ -171,6 +176,7 @@ class Propagator:
 
     def gradient(self, vec, dd, on_residual: bool = True):
         """(seis, misfit[n_models] float64 or None, grad float64)."""
+        self._gen += 1
         v, d = self._vec(vec), self._data(dd)
         seis = torch.empty(self.n_seis, device=self.device)
         grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)
@@ -179,8 +185,34 @@ class Propagator:
                                             _ptr(misfit), _ptr(grad), _stream()), "seis_gradient")
         return seis, misfit, grad
 
+    def forward_save(self, vec) -> tuple:
+        """Forward that keeps its checkpoints for a later ``adjoint``.
+        Returns (seis, token); the token must be passed to ``adjoint``."""
+        self._gen += 1
+        v = self._vec(vec)
+        seis = torch.empty(self.n_seis, device=self.device)
+        _lib.check(_lib.lib().seis_forward_save(self._h, _ptr(v), _ptr(seis), _stream()), "seis_forward_save")
+        self._saved_gen = self._gen
+        return seis, self._gen
+
+    def saved_valid(self, token: int) -> bool:
+        return token == self._saved_gen == self._gen
+
+    def adjoint(self, vec, dd, token: int, on_residual: bool = False):
+        """Reverse pass from the forward_save identified by ``token``:
+        (misfit or None, grad float64).  Raises if another pass ran since."""
+        if not self.saved_valid(token):
+            raise RuntimeError("the saved forward was overwritten by a later pass on this plan")
+        v, d = self._vec(vec), self._data(dd)
+        grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)
+        misfit = torch.empty(self.n_models, device=self.device, dtype=torch.float64) if on_residual else None
+        _lib.check(_lib.lib().seis_adjoint(self._h, _ptr(v), _ptr(d), int(bool(on_residual)), _ptr(misfit),
+                                           _ptr(grad), _stream()), "seis_adjoint")
+        return misfit, grad   # the reverse pass leaves the saved forward intact
+
     def gn_product(self, vec, dvec):
         """(J dvec, J^T J dvec) at vec."""
+        self._gen += 1
         v, dv = self._vec(vec), self._vec(dvec)
         dseis = torch.empty(self.n_seis, device=self.device)
         grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)

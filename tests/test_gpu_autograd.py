@@ -1,0 +1,116 @@
+"""The differentiable propagator (torch.autograd.Function with the adjoint as
+backward, checkpoint/recompute) against the FP64 oracle: gradient of the data
+misfit vs the oracle adjoint (1e-4 relative), and the full loss (misfit +
+small-CNN prior) vs FP64 central finite differences on 20 directions."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel
+from oracle import fd2d
+from seismic_amd import synth
+from seismic_amd.autograd import SmallCNNPrior, propagate
+from seismic_amd.geometry import Geometry
+from seismic_amd.propagator import Propagator
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(nz=20, nx=24, nbc=16, nt=300)
+B = 4
+
+
+def _batch():
+    g, og = Geometry(**KW), fd2d.Geometry(**KW)
+    vel = np.stack([synth.to_vec(synth.make_model(s, g.nz, g.nx, 40 + i))
+                    for i, s in enumerate(["flat", "curved", "faulted", "grf"])])
+    obs = np.stack([fd2d.vel_to_seis(synth.to_vec(synth.make_model("curved", g.nz, g.nx, 60 + i)), og)[0]
+                    for i in range(B)])
+    return g, og, vel, obs
+
+
+def _smooth_field(nz, nx, rng, sigma, ell=6.0):
+    """Zero-mean Gaussian random field (correlation ~ell cells), std sigma m/s."""
+    kz = np.fft.fftfreq(nz)[:, None]
+    kx = np.fft.fftfreq(nx)[None, :]
+    f = np.real(np.fft.ifft2((rng.normal(size=(nz, nx)) + 1j * rng.normal(size=(nz, nx)))
+                             * np.exp(-(np.pi * ell) ** 2 * (kz ** 2 + kx ** 2))))
+    f -= f.mean()
+    return (f / (f.std() + 1e-30) * sigma).ravel()
+
+
+def _oracle_misfit_grad(og, vel, obs):
+    """Gradient of sum_b mean((s_b - obs_b)^2): (2/N) J^T (s - obs) per model."""
+    out = []
+    for v, o in zip(vel, obs):
+        _, _, gr = fd2d.vel_to_seis(v, og, vec_adjoint=o, adjoint_on_residual=True)
+        out.append(2.0 / len(o) * gr)
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("ckpt", [0, 50, 13])
+def test_autograd_matches_oracle_adjoint(ckpt):
+    g, og, vel, obs = _batch()
+    P = Propagator([g] * B, ckpt_every=ckpt)
+    v = torch.tensor(vel, dtype=torch.float32, device="cuda", requires_grad=True)
+    o = torch.tensor(obs, dtype=torch.float32, device="cuda")
+    loss = ((propagate(v, P) - o) ** 2).mean(dim=1).sum()
+    loss.backward()
+    ref = _oracle_misfit_grad(og, vel, obs)
+    got = v.grad.double().cpu().numpy()
+    print("autograd grad rel", rel(got, ref))
+    assert rel(got, ref) < 1e-4
+    for b in range(B):
+        assert rel(got[b], ref[b]) < 1e-4
+
+
+def test_autograd_fd_20_directions():
+    g, og, vel, obs = _batch()
+    torch.manual_seed(0)
+    prior = SmallCNNPrior(g.nz, g.nx)
+    lam = 0.5
+    P = Propagator([g] * B, ckpt_every=50)
+    v = torch.tensor(vel, dtype=torch.float32, device="cuda", requires_grad=True)
+    o = torch.tensor(obs, dtype=torch.float32, device="cuda")
+    # the prior runs in float64 (torch convolutions in float32 may use
+    # reduced-precision math); the propagator stays float32
+    prior_gpu = SmallCNNPrior(g.nz, g.nx).cuda().double()
+    loss = ((propagate(v, P) - o) ** 2).mean(dim=1).sum() + lam * prior_gpu(v.double()).sum()
+    loss.backward()
+    grad = v.grad.double().cpu().numpy().ravel()
+    prior64 = prior.double()
+
+    def loss64(x):
+        x = x.reshape(B, -1)
+        mis = sum(np.mean((fd2d.vel_to_seis(x[b], og)[0] - obs[b]) ** 2) for b in range(B))
+        with torch.no_grad():
+            pr = float(prior64(torch.tensor(x)).sum())
+        return mis + lam * pr
+
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for k in range(20):
+        d = np.stack([np.concatenate([_smooth_field(g.nz, g.nx, rng, 20.0), [rng.normal(0, 5)]])
+                      for _ in range(B)]).ravel()
+        h = 1e-2
+        fd = (loss64(vel.ravel() + h * d) - loss64(vel.ravel() - h * d)) / (2 * h)
+        an = float(grad @ d)
+        err = abs(an - fd) / abs(fd)
+        worst = max(worst, err)
+        print(f"direction {k}: analytic {an:.6e} fd {fd:.6e} rel {err:.2e}")
+        assert err < 1e-4
+    print("worst directional error", worst)
+
+
+def test_backward_after_intervening_pass_recomputes():
+    g, og, vel, obs = _batch()
+    P = Propagator([g] * B, ckpt_every=13)
+    o = torch.tensor(obs, dtype=torch.float32, device="cuda")
+    grads = []
+    for intervene in (False, True):
+        v = torch.tensor(vel, dtype=torch.float32, device="cuda", requires_grad=True)
+        loss = ((propagate(v, P) - o) ** 2).mean(dim=1).sum()
+        if intervene:
+            P.forward(torch.tensor(vel[::-1].copy().reshape(-1)))   # overwrites the checkpoints
+        loss.backward()
+        grads.append(v.grad.clone())
+    assert torch.equal(grads[0], grads[1])
