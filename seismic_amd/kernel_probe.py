@@ -18,7 +18,7 @@ import math
 import numpy as np
 import torch
 
-from . import synth
+from . import autotune, synth
 from .geometry import Geometry
 from .propagator import Propagator
 
@@ -32,6 +32,7 @@ def main():
     g = Geometry(nt=args.nt)
     B = args.batch
     P = Propagator([g] * B, ckpt_every=0)
+    autotune.apply(P)
     vec = torch.tensor(np.concatenate([synth.to_vec(synth.curve_fault_model(70, 70, i)) for i in range(B)]),
                        dtype=torch.float32, device="cuda")
     dd = P.forward(vec).clone()
