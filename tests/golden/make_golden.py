@@ -3,11 +3,12 @@
 Reference geometry (core/seis_forward2.py:69-101): 70x70, dx 10 m, nbc 120,
 nt 999, dt 1 ms, 15 Hz Ricker, shots at x = 0, 17, 34, 52, 69, 70 receivers.
 Model: seeded curved layers + dipping fault (seismic_amd.synth.curve_fault_model,
-seed 7).  Target: the forward of the same model with +2 % velocity.
+seed 7).  Target: the forward of the same model with +2 % velocity, rounded
+to float32 (the precision observed data is handed over in).
 
   ref70_fwd.npz   seismograms of shots 0 and 2 (float32, (2, 999, 70))
-  ref70_grad.npz  J^T (F(v) - target) for all 5 shots (float64, 4901) and the
-                  misfit sum((F(v) - target)^2)
+  ref70_grad.npz  the float32 target (349650), J^T (F(v) - target) for all 5
+                  shots (float64, 4901) and the misfit sum((F(v) - target)^2)
 
 Run from the repo root: python tests/golden/make_golden.py  (~3 minutes).
 """
@@ -29,12 +30,13 @@ def main():
     g = fd2d.Geometry()
     v = synth.curve_fault_model(70, 70, seed=7)
     vec = synth.to_vec(v)
-    target = fd2d.forward_only(synth.to_vec(v * 1.02), g).reshape(-1)
+    target32 = fd2d.forward_only(synth.to_vec(v * 1.02), g).reshape(-1).astype(np.float32)
+    target = target32.astype(np.float64)
     seis, _, grad = fd2d.vel_to_seis(vec, g, vec_adjoint=target, adjoint_on_residual=True)
     s3 = seis.reshape(5, g.nt, g.n_rcv)
     np.savez_compressed(os.path.join(HERE, "ref70_fwd.npz"), seis=s3[[0, 2]].astype(np.float32),
                         shots=np.array([0, 2]))
-    np.savez_compressed(os.path.join(HERE, "ref70_grad.npz"), grad=grad,
+    np.savez_compressed(os.path.join(HERE, "ref70_grad.npz"), grad=grad, target=target32,
                         misfit=np.array(np.sum((seis - target) ** 2)))
     print("written", np.abs(s3).max(), np.linalg.norm(grad))
 

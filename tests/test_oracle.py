@@ -7,11 +7,14 @@ exact adjoint identities, brute force on tiny grids, physical symmetries and a
 library special case (torch.optim.LBFGS).  The reference's own recipes are
 restated where they apply (dot test and Taylor test, core/seis_test.py:26-44;
 prior and cost FD tests, :48-89)."""
+import os
+
 import numpy as np
 import pytest
 import torch
 
 from oracle import fd2d, invert, priors
+from seismic_amd import synth
 
 TINY = dict(nz=4, nx=5, nbc=3, nt=150)     # 10 x 11 padded, 2 shots
 SMALL = dict(nz=12, nx=16, nbc=8, nt=200)
@@ -319,3 +322,19 @@ def test_gn_cg_equals_dense_gn_small_basis():
     v_cg, s_cg, _ = invert.gauss_newton_cg(start, target, prior, g, n_cg=B.shape[1] + 4, n_scales=11)
     assert s_dense == s_cg
     np.testing.assert_allclose(v_cg, v_dense, rtol=1e-6)
+
+
+def test_golden_fixtures_match_oracle():
+    """The committed full-size fixtures are what the oracle computes now
+    (shot 0 forward, and the fixture's misfit from the oracle's 5-shot forward)."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    f = np.load(os.path.join(gold, "ref70_fwd.npz"))
+    d = np.load(os.path.join(gold, "ref70_grad.npz"))
+    g = fd2d.Geometry()
+    vec = synth.to_vec(synth.curve_fault_model(70, 70, seed=7))
+    s0 = fd2d.forward_only(vec, g, shots=[0])
+    assert np.array_equal(s0.reshape(g.nt, g.n_rcv).astype(np.float32), f["seis"][0])
+    full = fd2d.forward_only(vec, g).reshape(-1)
+    m = float(np.sum((full - d["target"].astype(np.float64)) ** 2))
+    assert abs(m - float(d["misfit"])) <= 1e-12 * float(d["misfit"])
+    assert d["grad"].shape == (g.nz * g.nx + 1,) and np.all(np.isfinite(d["grad"]))

@@ -1,0 +1,139 @@
+"""Serving loop host logic on the CPU with a fake executor: admission by slot
+and HBM footprint, continuous batching (late arrivals join running batches),
+per-request results independent of the mix, NaN fallback to the start model
+(core/seis_invert.py:146-149)."""
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from seismic_amd import invert, serve
+from seismic_amd.geometry import Geometry
+
+
+class QuadPrior:
+    """Zero prior on the identity basis (enough to drive the LBFGS machine)."""
+
+    basis = None
+    lam = 0.0
+    device = torch.device("cpu")
+
+    def __init__(self, n):
+        self.n = n
+
+    def to_basis(self, v):
+        return v.clone()
+
+    def from_basis(self, x):
+        return x
+
+    def basis_T(self, g):
+        return g
+
+    def cost_grad(self, x):
+        return torch.tensor(0.0, dtype=torch.float64), torch.zeros_like(x)
+
+
+class FakeExecutor:
+    """'fev' of a separable quadratic: misfit = sum (x - c)^2 * N, J^T r = N (x - c)."""
+
+    def __init__(self, nan_rids=(), seconds=0.01):
+        self.centers = {}
+        self.nan_targets = set(nan_rids)
+        self.nan = set()
+        self.calls = []
+        self.seconds = seconds      # stands in for the batched launch time
+
+    def set_target(self, slot, target):
+        # a request's "observed data" is its quadratic's centre; the last entry tags NaN requests
+        self.centers[slot] = target[:-1]
+        self.nan.discard(slot)
+        if int(target[-1]) in self.nan_targets:
+            self.nan.add(slot)
+
+    def execute(self, pending):
+        self.calls.append(sorted(pending))
+        time.sleep(self.seconds)
+        out = {}
+        for s, op in pending.items():
+            assert op.kind == "fev"
+            c = self.centers[s]
+            r = op.vec - c
+            N = float(c.numel())
+            f = float((r * r).sum()) * N
+            g = r * N
+            if s in self.nan:
+                f, g = float("nan"), g * float("nan")
+            out[s] = (f, g)
+        return out
+
+
+def _server(slots, budget, fake, footprint=100):
+    srv = serve.InversionServer.__new__(serve.InversionServer)
+    srv.slots = list(slots)
+    srv.ckpt_every = 0
+    srv.kw = dict(max_iter=5, history_size=3, tolerance_grad=1e-12, tolerance_change=0.0)
+    srv.budget = budget
+    srv.ex = fake
+    srv._fp = {g: footprint for g in set(slots)}
+    srv.used = 0
+    srv.free = {}
+    for i, g in enumerate(slots):
+        srv.free.setdefault(g, []).append(i)
+    srv.log = []
+    return srv
+
+
+def _req(g, rid, n=7, center=None):
+    c = torch.ones(n, dtype=torch.float64) if center is None else center
+    return invert.Request(geom=g, target=torch.cat([c, torch.tensor([float(rid)], dtype=torch.float64)]),
+                          prior=QuadPrior(n), start=torch.full((n,), 3.0, dtype=torch.float64), rid=rid)
+
+
+def test_admission_respects_slots_and_budget():
+    g1, g2 = Geometry(nz=4, nx=4, nbc=2, nt=150), Geometry(nz=5, nx=5, nbc=2, nt=150)
+    srv = _server([g1, g1, g2], budget=250, fake=None)
+    a, b, c, d = _req(g1, 0), _req(g1, 1), _req(g2, 2), _req(g1, 3)
+    assert srv._try_admit(a) is not None
+    assert srv._try_admit(b) is not None
+    assert srv._try_admit(d) is None          # no free 4x4 slot
+    assert srv._try_admit(c) is None          # 5x5 slot free but the budget is used up
+    srv._release(0, a)
+    assert srv._try_admit(c) is not None
+
+
+def test_continuous_batching_and_results_match_alone():
+    g = Geometry(nz=4, nx=4, nbc=2, nt=150)
+    n = 7
+    centers = {i: torch.linspace(0, 1, n, dtype=torch.float64) * (i + 1) for i in range(4)}
+    fake = FakeExecutor()
+    srv = _server([g] * 4, budget=10 ** 9, fake=fake)
+    reqs = [_req(g, i, n, centers[i]) for i in range(4)]
+    arrivals = [0.0, 0.0, 0.015, 0.025]
+    recs = srv.run(reqs, arrivals)
+    assert all(not r.failed for r in recs)
+    assert any(len(c) >= 3 for c in fake.calls)   # late arrivals merged into running batches
+    for i, r in enumerate(recs):
+        s1 = _server([g], budget=10 ** 9, fake=FakeExecutor())
+        r1 = s1.run([_req(g, i, n, centers[i])], [0.0])[0]
+        assert torch.equal(r.result, r1.result)
+        assert r.latency >= 0 and r.finished >= r.admitted >= r.arrival - 1e-9
+
+
+def test_nan_request_falls_back_to_start():
+    g = Geometry(nz=4, nx=4, nbc=2, nt=150)
+    fake = FakeExecutor(nan_rids=[1])
+    srv = _server([g, g], budget=10 ** 9, fake=fake)
+    reqs = [_req(g, 0), _req(g, 1)]
+    recs = srv.run(reqs, [0.0, 0.0])
+    assert torch.equal(recs[1].result, reqs[1].start)
+    assert not torch.isnan(recs[0].result).any()
+
+
+def test_poisson_arrivals_and_percentiles():
+    a = serve.poisson_arrivals(2000, 100.0, seed=1)
+    assert np.all(np.diff(a) > 0)
+    assert abs(len(a) / a[-1] - 100.0) < 10.0
+    p = serve.percentiles([1.0, 2.0, 3.0, 4.0])
+    assert p["p50"] == pytest.approx(2.5) and p["max"] == 4.0
