@@ -149,20 +149,38 @@ int seis_adjoint(seis_plan_t* plan, const float* vec, const float* dd,
 int seis_gn_product(seis_plan_t* plan, const float* vec, const float* dvec,
                     float* dseis, double* grad, seis_stream_t stream);
 
+/* Tangent and Gauss-Newton product at the linearization point of the last
+ * history-saving forward (seis_gradient, seis_forward_save or
+ * seis_gn_product) of each active model, without recomputing the forward:
+ * that forward stored Lf(p[n]) and q[n] = p[n] - p[n-1] for every step, and
+ * the Born step kernel reads them instead of advancing p.  Results are
+ * bit-identical to seis_tangent's dseis and seis_gn_product's outputs at the
+ * same vec.  vec must be the vector of that saving forward (the caller
+ * tracks this, as core/seis_invert.py:84-90 keeps x_guess fixed over the
+ * columns of J).  Requires a full-history plan (ckpt_every 0) with
+ * SEIS_CAP_TANGENT | SEIS_CAP_ADJOINT, and a saving forward since creation on
+ * every active model; otherwise SEIS_ESTATE.  Replaces the forward
+ * recomputation inside vel_to_seis(vec, vec_diff=...) (:252-283). */
+int seis_tangent_saved(seis_plan_t* plan, const float* vec, const float* dvec,
+                       float* dseis, seis_stream_t stream);
+int seis_gn_product_saved(seis_plan_t* plan, const float* vec, const float* dvec,
+                          float* dseis, double* grad, seis_stream_t stream);
+
 /* Kernel kinds with a tunable launch configuration. */
 enum {
   SEIS_KIND_FORWARD = 0, /* forward step kernel (also segment recompute)     */
   SEIS_KIND_TANGENT = 1, /* fused forward + tangent step kernel               */
   SEIS_KIND_ADJOINT = 2, /* adjoint step kernel with fused imaging            */
-  SEIS_KIND_COUNT = 3
+  SEIS_KIND_BORN = 3,    /* tangent step kernel on the saved forward history  */
+  SEIS_KIND_COUNT = 4
 };
 
 /* Launch configuration of one kernel kind: an rx x rz cell window (halo
  * included) is staged in LDS per workgroup and advanced k time steps per
  * launch; the window's interior, (rz-4k) x (rx-4k) cells, is written back.
- * Windows (rx x rz): forward 64x32, 128x32, 64x64, 128x64, 256x32; tangent and
- * adjoint 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4
- * rows and 8 columns.  Results do not depend on the config
+ * Windows (rx x rz): forward and Born 64x32, 128x32, 64x64, 128x64, 256x32;
+ * tangent and adjoint 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of
+ * at least 4 rows and 8 columns.  Results do not depend on the config
  * (bit-identical wavefields).  No reference counterpart: the reference runs
  * one 32x32-thread launch per step (core/seis_forward2.py:191-193).
  * Setting a config synchronizes the device and drops captured graphs. */

@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libseis.so"
 CAP_FORWARD = 1
 CAP_TANGENT = 2
 CAP_ADJOINT = 4
-KIND_FORWARD, KIND_TANGENT, KIND_ADJOINT = 0, 1, 2
+KIND_FORWARD, KIND_TANGENT, KIND_ADJOINT, KIND_BORN = 0, 1, 2, 3
 MAX_SHOTS = 16
 
 ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "HIP error",
@@ -42,6 +42,8 @@ EXPORTS = {
     "seis_forward_save": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "seis_gn_product": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "seis_tangent_saved": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "seis_gn_product_saved": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_plan_device_bytes": (c_int64, [c_void_p]),
     "seis_plan_set_config": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32]),
     "seis_plan_get_config": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),

@@ -1,6 +1,7 @@
 """Autotuner for the step kernels' launch configuration.
 
-Each kernel kind (forward, fused tangent, adjoint) runs an rx x rz cell
+Each kernel kind (forward, fused tangent, adjoint, Born on the saved
+forward) runs an rx x rz cell
 window per workgroup and advances k time steps per launch (libseis
 ``seis_plan_set_config``).  The best choice depends on how many cells one
 time step covers (batch size x model size: a single 70x70 model leaves most
@@ -31,7 +32,20 @@ CANDIDATES = {
                 (64, 64, 4), (64, 64, 6), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3)],
     "tangent": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
     "adjoint": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
+    "born": [(64, 32, 1), (64, 32, 2), (64, 32, 3), (128, 32, 2), (128, 32, 3), (64, 64, 2), (64, 64, 3),
+             (128, 64, 2), (128, 64, 3), (128, 64, 4), (256, 32, 2), (256, 32, 3)],
 }
+KINDS = ("forward", "tangent", "adjoint", "born")
+
+
+def _has(P, kind: str) -> bool:
+    if kind == "tangent":
+        return bool(P.caps & CAP_TANGENT)
+    if kind == "adjoint":
+        return bool(P.caps & CAP_ADJOINT)
+    if kind == "born":
+        return bool(P.caps & CAP_TANGENT) and P.full_history
+    return True
 
 
 def cells_per_step(P) -> int:
@@ -55,10 +69,8 @@ def apply(P, table: dict | None = None) -> dict:
     mode); returns what was applied."""
     table = load_table() if table is None else table
     applied = {}
-    for kind in ("forward", "tangent", "adjoint"):
-        if kind == "tangent" and not (P.caps & CAP_TANGENT):
-            continue
-        if kind == "adjoint" and not (P.caps & CAP_ADJOINT):
+    for kind in KINDS:
+        if not _has(P, kind):
             continue
         k = key(P, kind)
         kk, b, mode = k.split(":")
@@ -82,20 +94,20 @@ def _timed(fn, reps):
     return (time.perf_counter() - t0) / reps
 
 
-def tune(P, kinds=("forward", "tangent", "adjoint"), reps: int = 2, verbose: bool = False) -> dict:
+def tune(P, kinds=KINDS, reps: int = 2, verbose: bool = False) -> dict:
     """Time every candidate config of each kind on plan P (synthetic models of
     the plan's shapes) and leave P on the fastest.  Returns {key: (cfg, s)}."""
     vec = torch.tensor(np.concatenate([synth.to_vec(synth.make_model("curved", g.nz, g.nx, i % 7))
                                        for i, g in enumerate(P.geoms)]), dtype=torch.float32, device="cuda")
     dd = P.forward(vec).clone()
     run = {"forward": lambda: P.forward(vec), "tangent": lambda: P.tangent(vec, vec),
-           "adjoint": lambda: P.gradient(vec, dd)}
+           "adjoint": lambda: P.gradient(vec, dd), "born": lambda: P.tangent_saved(vec, vec, check=False)}
     out = {}
     for kind in kinds:
-        if kind == "tangent" and not (P.caps & CAP_TANGENT):
+        if not _has(P, kind):
             continue
-        if kind == "adjoint" and not (P.caps & CAP_ADJOINT):
-            continue
+        if kind == "born":
+            P.gradient(vec, dd)   # the saved forward the Born pass reads
         res = {}
         for cfg in CANDIDATES[kind]:
             try:

@@ -16,6 +16,9 @@
 // with the imaging terms reduced to two per-cell accumulators
 //   GA = sum lam[m]*Lf(p[m-1])         (alpha_adjoint + 2*c1*temp1_adjoint)
 //   GK = sum lam[m]*(p[m-2] - p[m-1])  (kappa_adjoint = -temp1_adjoint - temp2_adjoint)
+// A saving forward stores the pair (Lf(p[n]), q[n]) per cell and step; the
+// adjoint's imaging terms and the tangent's source terms dalpha*Lf(p) and
+// dkappa*q are pointwise in it, so both run one stencil per step.
 // and prep_run_adjoint (:361-377) + unpad_edge_padded_gradient
 // (core/seis_numerics.py:14-79) folded into one finishing kernel.
 //
@@ -47,9 +50,10 @@ constexpr int NXCD = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
 
 // Per-wavefield state regions in the float arena.
 enum Region {
-  R_HIST = 0,  // full history (nt+2 slots) or recompute segment (K+1 slots)
-  R_PP,        // forward ping-pong p (2 slots) in checkpoint mode
-  R_Q,         // q = p[n] - p[n-1]
+  R_HIST = 0,  // (Lf(p[n]), q[n]) float pairs: full history n = 0..nt (nt+1
+               // slots) or a recompute segment (Kc+1 slots); 2 floats per cell
+  R_PP,        // forward p ping-pong (2 slots)
+  R_Q,         // q = p[n] - p[n-1] ping-pong (2 slots)
   R_DP,        // tangent dp ping-pong (2 slots)
   R_DQ,        // tangent dq
   R_LAM,       // adjoint lambda ping-pong (2 slots)
@@ -57,7 +61,7 @@ enum Region {
   R_GA,        // imaging accumulator for alpha
   R_GK,        // imaging accumulator for kappa
   R_CKP,       // checkpoints of p[jK] (J slots)
-  R_CKQ,       // checkpoints of q[jK+1] (J slots)
+  R_CKQ,       // checkpoints of q[jK] (J slots)
   R_COUNT
 };
 
@@ -167,8 +171,8 @@ struct FwdArgs {
   int nt;                // time samples
   int n0, K;             // advance state n0 -> n0+K; step n produces p[n+1] with s[n-1]
   int Kt;                // steps the tile list was built for (halo 2*Kt); K <= Kt
-  int p_in_reg, p_in_slot;
-  int p_out_reg, p_out_slot0, hist_all;  // hist_all: p[n0+s+1] -> slot p_out_slot0+s
+  int p_in_slot, p_out_slot;  // R_PP slots holding p[n0] and receiving p[n0+K]
+  int hist_slot0;        // >= 0: (Lf(p[n0+s]), q[n0+s]) -> R_HIST slot hist_slot0 + s
   int q_in_slot, q_out_slot;
   int dp_in_slot, dp_out_slot, dq_in_slot, dq_out_slot;
   int ck_slot;           // >= 0: also store the final (p, q) as checkpoint ck_slot
@@ -190,7 +194,9 @@ __device__ __forceinline__ int rcv_index(int gx, const ModelDev& m) {
 // for every tile of the work list.  Thread-owned cells keep q, alpha and
 // (1-kappa) in registers across the K steps; p (and dp) live in LDS.  No
 // global load is issued inside the step loop: the wavelet samples of the
-// launch are staged in LDS with the window.
+// launch are staged in LDS with the window.  A saving forward also stores,
+// per interior cell and step, the pair (Lf(p[n]), q[n]) that the tangent and
+// adjoint recursions consume.
 template <int RX, int RZ, int NT, bool TAN>
 __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
@@ -213,12 +219,16 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
     const ModelDev& m = a.models[w.model];
     const int NZ = w.NZ, NX = w.NX;
     const int64_t Pn = (int64_t)NZ * NX;
-    const float* pin = a.arena + w.off[a.p_in_reg] + a.p_in_slot * Pn;
+    const float* pin = a.arena + w.off[R_PP] + a.p_in_slot * Pn;
     const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
+    float2* hist = a.hist_slot0 >= 0
+                       ? reinterpret_cast<float2*>(a.arena + w.off[R_HIST]) + a.hist_slot0 * Pn
+                       : nullptr;
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float amp = a.wparam[te.w].amp;
     const int oz = te.z0 - H, ox = te.x0 - H;
+    const int gint0 = oz * NX + ox;  // unwrapped index of window cell (0, 0)
     float q[NC], al[NC], k1[NC], pn[NC];
     float dq[TAN ? NC : 1], dal[TAN ? NC : 1], dk[TAN ? NC : 1], dpn[TAN ? NC : 1];
     float dkscale = 0.f, amp_d = 0.f;
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       const int it = a.n0 + (int)threadIdx.x - 1;
       SW[threadIdx.x] = ((int)threadIdx.x < a.K && it >= 0 && it < a.nt) ? a.wavelet[it] : 0.f;
     }
-    unsigned srcmask = 0, rcvmask = 0;
+    unsigned srcmask = 0, rcvmask = 0, intmask = 0;
     int rcol[CPR];
 #pragma unroll
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
@@ -256,6 +266,8 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
         const float jj = (float)(j * j);
         k1[i] = 1.0f - kscale * jj;
         if (g == w.src_cell) srcmask |= 1u << i;
+        if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
+          intmask |= 1u << i;
         if constexpr (TAN) {
           D[r + 2][cc + 2] = dpin[g];
           dq[i] = dqin[g];
@@ -274,11 +286,12 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       const float src = amp * sv;
       const float srcd = amp_d * sv;
       const int lo = 2 * (s + 1);
+      float2* hs = hist ? hist + s * Pn : nullptr;
       // Forward: every owned cell is computed (its stencil stays inside the
       // padded window) and only the valid band [lo, RZ-lo) x [lo, RX-lo) is
       // committed, which keeps the cell loop branch-free.  Tangent: rows
       // outside the band are skipped (two stencils per cell would otherwise
-      // exceed the register budget).
+      // exceed the register budget).  Interior rows are always in the band.
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
         const int r = wv + NW * k;
@@ -292,6 +305,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
           const int cc = lane + 64 * c;
           const bool ok = rok && cc >= lo && cc < RX - lo;
           const float L = lap_full<PW>(P, r + 2, cc + 2);
+          if (hs && ((intmask >> i) & 1u)) hs[gint0 + r * NX + cc] = make_float2(L, q[i]);
           float qn = fmaf(k1[i], q[i], al[i] * L);
           if ((srcmask >> i) & 1u) qn = qn + src;
           if constexpr (TAN) {
@@ -313,8 +327,6 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       for (int k = 0; k < RPW; ++k) {
         const int r = wv + NW * k;
         if (r < lo || r >= RZ - lo) continue;
-        const int zz = oz + r;
-        const bool zin = r >= H && r < RZ - H && zz < NZ;
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
           const int i = k * CPR + c;
@@ -322,10 +334,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
           if (cc < lo || cc >= RX - lo) continue;
           P[r + 2][cc + 2] = pn[i];
           if constexpr (TAN) D[r + 2][cc + 2] = dpn[i];
-          const int xx = ox + cc;
-          if (!(zin && cc >= H && cc < RX - H && xx < NX)) continue;
-          const int g = zz * NX + xx;
-          if (a.hist_all) a.arena[w.off[a.p_out_reg] + (a.p_out_slot0 + s) * Pn + g] = pn[i];
+          if (!((intmask >> i) & 1u)) continue;
           if (a.write_seis && live && ((rcvmask >> k) & 1u) && rcol[c] >= 0) {
             const int64_t si = w.seis_off + (int64_t)it * m.n_rcv + rcol[c];
             a.seis[si] = pn[i];
@@ -337,23 +346,19 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
     }
     // state (p, q[, dp, dq]) at n0+K on the interior
     float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
-    float* pout = a.arena + w.off[a.p_out_reg] + a.p_out_slot0 * Pn;
+    float* pout = a.arena + w.off[R_PP] + a.p_out_slot * Pn;
     float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
     float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int r = wv + NW * k;
-      const int zz = oz + r;
-      if (r < H || r >= RZ - H || zz >= NZ) continue;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = k * CPR + c;
-        const int cc = lane + 64 * c;
-        const int xx = ox + cc;
-        if (cc < H || cc >= RX - H || xx >= NX) continue;
-        const int g = zz * NX + xx;
+        if (!((intmask >> i) & 1u)) continue;
+        const int g = gint0 + r * NX + lane + 64 * c;
         qout[g] = q[i];
-        if (!a.hist_all) pout[g] = pn[i];
+        pout[g] = pn[i];
         if (ckp) {
           ckp[g] = pn[i];
           ckq[g] = q[i];
@@ -362,6 +367,173 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
           a.arena[w.off[R_DP] + a.dp_out_slot * Pn + g] = dpn[i];
           a.arena[w.off[R_DQ] + a.dq_out_slot * Pn + g] = dq[i];
         }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- tangent on the saved history
+struct BornArgs {
+  const TileEnt* tiles;
+  const int* n_tiles;
+  const WaveDev* waves;
+  const ModelDev* models;
+  const ParamDev* params;
+  const WaveParam* wparam;
+  const float* alpha;
+  const float* dalpha;
+  const float* wavelet;
+  float* arena;
+  float* dseis;
+  int nt;
+  int n0, K;             // advance (dp, dq) n0 -> n0+K
+  int Kt;                // steps the tile list was built for (halo 2*Kt); K <= Kt
+  int dp_in_slot, dp_out_slot, dq_in_slot, dq_out_slot;
+};
+
+// K steps of the Born recursion (update_p_diff :470-509) with the forward
+// wavefield taken from the saved history: (Lf(p[n]), q[n]) of every band cell
+// is read from HBM instead of advancing p, which leaves one stencil per step.
+// The next step's pairs are requested as soon as the current step's last use
+// of the registers they land in is issued, so the commit phase and barriers
+// cover their latency.  The
+// fused multiply-add sequence is k_fwd<TAN>'s, so dseis and (dp, dq) are
+// bit-identical to the fused forward + tangent pass.
+template <int RX, int RZ, int NT>
+__global__ __launch_bounds__(NT) void k_born(BornArgs a) {
+  constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
+  static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
+  static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
+  constexpr int PW = RX + 4, PH = RZ + 4;
+  __shared__ float D[PH][PW];
+  __shared__ float SW[KMAX];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ntiles = *a.n_tiles;
+  const int H = 2 * a.Kt;
+  for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    const TileEnt te = a.tiles[e];
+    if (te.w < 0) continue;
+    const WaveDev& w = a.waves[te.w];
+    const ModelDev& m = a.models[w.model];
+    const int NZ = w.NZ, NX = w.NX;
+    const int64_t Pn = (int64_t)NZ * NX;
+    const float* dpin = a.arena + w.off[R_DP] + a.dp_in_slot * Pn;
+    const float* dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
+    const float2* hist = reinterpret_cast<const float2*>(a.arena + w.off[R_HIST]);
+    const float* alpha = a.alpha + m.cell_off;
+    const float* dalpha = a.dalpha + m.cell_off;
+    const float kscale = a.params[w.model].kscale;
+    const float dkscale = a.params[w.model].dkscale;
+    const float amp_d = a.wparam[te.w].amp_d;
+    const int oz = te.z0 - H, ox = te.x0 - H;
+    const int gint0 = oz * NX + ox;
+    float dq[NC], al[NC], dal[NC], jj[NC], dpn[NC];
+    float2 hc[NC];
+    int gw[NC];
+    if (threadIdx.x < KMAX) {
+      const int it = a.n0 + (int)threadIdx.x - 1;
+      SW[threadIdx.x] = ((int)threadIdx.x < a.K && it >= 0 && it < a.nt) ? a.wavelet[it] : 0.f;
+    }
+    unsigned srcmask = 0, rcvmask = 0, intmask = 0;
+    int rcol[CPR];
+#pragma unroll
+    for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
+    const float2* h0 = hist + a.n0 * Pn;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + NW * k;
+      const int gz = wrapi(oz + r, NZ);
+      if (gz == m.rcv_row) rcvmask |= 1u << k;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int i = k * CPR + c;
+        const int cc = lane + 64 * c;
+        const int gx = wrapi(ox + cc, NX);
+        const int g = gz * NX + gx;
+        gw[i] = g;
+        D[r + 2][cc + 2] = dpin[g];
+        dq[i] = dqin[g];
+        al[i] = alpha[g];
+        dal[i] = dalpha[g];
+        const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
+        jj[i] = (float)(j * j);
+        if (g == w.src_cell) srcmask |= 1u << i;
+        if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
+          intmask |= 1u << i;
+        dpn[i] = 0.f;
+        hc[i] = (r >= 2 && r < RZ - 2 && cc >= 2 && cc < RX - 2) ? h0[g] : make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < a.K; ++s) {
+      const int it = a.n0 + s - 1;
+      const bool live = it >= 0 && it < a.nt;
+      const float srcd = amp_d * SW[s];
+      const int lo = 2 * (s + 1);
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + NW * k;
+        if (r < lo || r >= RZ - lo) continue;  // wave-uniform
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int i = k * CPR + c;
+          const int cc = lane + 64 * c;
+          const bool ok = cc >= lo && cc < RX - lo;
+          const float Ld = lap_full<PW>(D, r + 2, cc + 2);
+          const float k1 = 1.0f - kscale * jj[i];
+          const float dk = dkscale * jj[i];
+          float dqn = fmaf(k1, dq[i], al[i] * Ld);
+          dqn = fmaf(dal[i], hc[i].x, dqn);
+          dqn = fmaf(-dk, hc[i].y, dqn);
+          if ((srcmask >> i) & 1u) dqn = dqn + srcd;
+          dpn[i] = ok ? D[r + 2][cc + 2] + dqn : dpn[i];
+          dq[i] = ok ? dqn : dq[i];
+        }
+      }
+      if (s + 1 < a.K) {  // pairs of the next step's band
+        const float2* hx = hist + (int64_t)(a.n0 + s + 1) * Pn;
+        const int l1 = lo + 2;
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+          const int r = wv + NW * k;
+          if (r < l1 || r >= RZ - l1) continue;
+#pragma unroll
+          for (int c = 0; c < CPR; ++c) {
+            const int cc = lane + 64 * c;
+            if (cc >= l1 && cc < RX - l1) hc[k * CPR + c] = hx[gw[k * CPR + c]];
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        const int r = wv + NW * k;
+        if (r < lo || r >= RZ - lo) continue;
+#pragma unroll
+        for (int c = 0; c < CPR; ++c) {
+          const int i = k * CPR + c;
+          const int cc = lane + 64 * c;
+          if (cc < lo || cc >= RX - lo) continue;
+          D[r + 2][cc + 2] = dpn[i];
+          if (!((intmask >> i) & 1u)) continue;
+          if (live && ((rcvmask >> k) & 1u) && rcol[c] >= 0)
+            a.dseis[w.seis_off + (int64_t)it * m.n_rcv + rcol[c]] = dpn[i];
+        }
+      }
+      __syncthreads();
+    }
+    float* dpout = a.arena + w.off[R_DP] + a.dp_out_slot * Pn;
+    float* dqout = a.arena + w.off[R_DQ] + a.dq_out_slot * Pn;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int r = wv + NW * k;
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        const int i = k * CPR + c;
+        if (!((intmask >> i) & 1u)) continue;
+        const int g = gint0 + r * NX + lane + 64 * c;
+        dpout[g] = dpn[i];
+        dqout[g] = dq[i];
       }
     }
   }
@@ -384,18 +556,18 @@ struct AdjArgs {
   int M0, K;            // produces lam[M0-1] .. lam[M0-K]
   int Kt;               // steps the tile list was built for (halo 2*Kt); K <= Kt
   int lam_in_slot, lam_out_slot, mu_in_slot, mu_out_slot;
-  int p_reg, p_base;    // forward p[n] sits at slot n - p_base of region p_reg
+  int h_base;           // the forward's (Lf(p[n]), q[n]) sits at R_HIST slot n - h_base
 };
 
 // K reverse steps of update_p_adjoint (:517-579) in gather form for every
 // tile, with the imaging accumulators (alpha / kappa adjoints) and the
 // source-wavelet correlation (:314) fused on the tile interior.  lam, mu,
 // alpha, (1-kappa) and the imaging accumulators of owned cells stay in
-// registers; alpha*lam is staged in LDS for the stencil, and the forward
-// wavefield p[m-1], p[m-2] streams through two LDS windows (interior + 2
-// halo) whose global loads are issued before the lam update of each step.
-// The accumulators see the same fused multiply-add sequence as a per-step
-// read-modify-write, so results do not depend on K.
+// registers; alpha*lam is staged in LDS for the stencil.  The forward enters
+// only through the saved pairs (Lf(p[m-1]), q[m-1]) of interior cells; the
+// next step's pairs are requested right after the current step's last use.  The accumulators see the same fused
+// multiply-add sequence as a per-step read-modify-write, so results do not
+// depend on K.
 template <int RX, int RZ, int NT>
 __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
@@ -403,7 +575,6 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
   constexpr int PW = RX + 4, PH = RZ + 4;     // alpha*lam window with a 2-cell apron
   __shared__ float A[PH][PW];
-  __shared__ float PT[2][RZ][RX];
   __shared__ float INJ[KMAX][RX];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -420,13 +591,13 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     const float* min_ = a.arena + w.off[R_MU] + a.mu_in_slot * Pn;
     float* GA = a.arena + w.off[R_GA];
     float* GK = a.arena + w.off[R_GK];
-    const float* pbase = a.arena + w.off[a.p_reg];
+    const float2* hist = reinterpret_cast<const float2*>(a.arena + w.off[R_HIST]);
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const int oz = te.z0 - H, ox = te.x0 - H;
-    const int pz = te.z0 - 2, px = te.x0 - 2;        // p windows: interior + 2 halo
-    const int TH = RZ - 2 * H + 4, TW = RX - 2 * H + 4;
-    float lam[NC], mu[NC], al[NC], k1[NC], ln[NC], ga[NC], gk[NC], pv[NC];
+    const int gint0 = oz * NX + ox;
+    float lam[NC], mu[NC], al[NC], k1[NC], ln[NC], ga[NC], gk[NC];
+    float2 hc[NC];
     unsigned srcmask = 0, rcvmask = 0, intmask = 0;
     int rcol[CPR];
 #pragma unroll
@@ -446,6 +617,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       }
       INJ[s][cc] = v;
     }
+    const float2* h0 = hist + (int64_t)(a.M0 - 2 - a.h_base) * Pn;  // pairs of n = M0-2
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int r = wv + NW * k;
@@ -469,46 +641,24 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         const bool inner = r >= H && r < RZ - H && zz < NZ && cc >= H && cc < RX - H && xx < NX;
         ga[i] = 0.f;
         gk[i] = 0.f;
+        hc[i] = make_float2(0.f, 0.f);
         if (inner) {
           intmask |= 1u << i;
-          ga[i] = GA[zz * NX + xx];
-          gk[i] = GK[zz * NX + xx];
+          const int gi = zz * NX + xx;
+          ga[i] = GA[gi];
+          gk[i] = GK[gi];
+          hc[i] = h0[gi];
         }
         ln[i] = 0.f;
-        // p[M0-2] window (row r, column cc of the p window)
-        pv[i] = (r < TH && cc < TW) ? pbase[(int64_t)(a.M0 - 2 - a.p_base) * Pn +
-                                            wrapi(pz + r, NZ) * NX + wrapi(px + cc, NX)]
-                                      : 0.f;
       }
     }
     double ssum = 0.0;
     const bool src_here = (srcmask & intmask) != 0;
     if (src_here) ssum = a.wparam[te.w].ssum;
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
-#pragma unroll
-      for (int c = 0; c < CPR; ++c) {
-        const int cc = lane + 64 * c;
-        if (r < TH && cc < TW) PT[0][r][cc] = pv[k * CPR + c];
-      }
-    }
     __syncthreads();
     for (int s = 0; s < a.K; ++s) {
       const int mm = a.M0 - 1 - s;
       const int lo = 2 * (s + 1);
-      const int cur = s & 1, nxt = cur ^ 1;
-      // issue the loads of p[mm-2] before the lam update
-      const float* pnext = pbase + (int64_t)(mm - 2 - a.p_base) * Pn;
-#pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-#pragma unroll
-        for (int c = 0; c < CPR; ++c) {
-          const int cc = lane + 64 * c;
-          if (r < TH && cc < TW) pv[k * CPR + c] = pnext[wrapi(pz + r, NZ) * NX + wrapi(px + cc, NX)];
-        }
-      }
 #pragma unroll
       for (int k = 0; k < RPW; ++k) {
         const int r = wv + NW * k;
@@ -527,15 +677,6 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
           mu[i] = ok ? mun : mu[i];
         }
       }
-#pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-#pragma unroll
-        for (int c = 0; c < CPR; ++c) {
-          const int cc = lane + 64 * c;
-          if (r < TH && cc < TW) PT[nxt][r][cc] = pv[k * CPR + c];
-        }
-      }
       __syncthreads();
       const float sw = SW[s];
 #pragma unroll
@@ -550,13 +691,19 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
           A[r + 2][cc + 2] = al[i] * ln[i];
           lam[i] = ln[i];
           if (!((intmask >> i) & 1u)) continue;
-          const int pr = r - H + 2, pc = cc - H + 2;
-          const float Lp = lap_full<RX>(PT[cur], pr, pc);
-          const float p1 = PT[cur][pr][pc];
-          const float p0 = PT[nxt][pr][pc];
-          ga[i] = fmaf(ln[i], Lp, ga[i]);
-          gk[i] = fmaf(ln[i], p0 - p1, gk[i]);
+          // GA += lam[m] Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
+          ga[i] = fmaf(ln[i], hc[i].x, ga[i]);
+          gk[i] = fmaf(ln[i], -hc[i].y, gk[i]);
           if ((srcmask >> i) & 1u) ssum += (double)ln[i] * (double)sw;
+        }
+      }
+      if (s + 1 < a.K) {  // pairs of n = mm - 2 for the next step
+        const float2* hx = hist + (int64_t)(mm - 2 - a.h_base) * Pn;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+          const int r = wv + NW * (i / CPR);
+          const int cc = lane + 64 * (i % CPR);
+          if ((intmask >> i) & 1u) hc[i] = hx[gint0 + r * NX + cc];
         }
       }
       __syncthreads();
@@ -566,19 +713,18 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int r = wv + NW * k;
-      const int zz = oz + r;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = k * CPR + c;
         if (!((intmask >> i) & 1u)) continue;
-        const int g = zz * NX + ox + lane + 64 * c;
+        const int g = gint0 + r * NX + lane + 64 * c;
         lout[g] = lam[i];
         mout[g] = mu[i];
         GA[g] = ga[i];
         GK[g] = gk[i];
       }
     }
-    if (src_here && ((srcmask & intmask) != 0)) a.wparam[te.w].ssum = ssum;
+    if (src_here) a.wparam[te.w].ssum = ssum;
     __syncthreads();
   }
 }
@@ -599,7 +745,7 @@ __global__ __launch_bounds__(NTHR) void k_fill(const WaveDev* waves, const Model
   }
 }
 
-// Restore checkpoint j (state p[jK], q[jK]) into segment slot 0 and q slot 0
+// Restore checkpoint j (state p[jK], q[jK]) into p slot 0 and q slot 0
 // before the segment is recomputed.
 __global__ __launch_bounds__(NTHR) void k_restore(const WaveDev* waves, const ModelDev* models,
                                                    int n_waves, float* arena, int j) {
@@ -609,7 +755,7 @@ __global__ __launch_bounds__(NTHR) void k_restore(const WaveDev* waves, const Mo
     const int64_t P = (int64_t)w.NZ * w.NX;
     const float* ckp = arena + w.off[R_CKP] + j * P;
     const float* ckq = arena + w.off[R_CKQ] + j * P;
-    float* s0 = arena + w.off[R_HIST];
+    float* s0 = arena + w.off[R_PP];
     float* q = arena + w.off[R_Q];
     for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < P; i += (int64_t)gridDim.x * NTHR) {
       s0[i] = ckp[i];
@@ -803,15 +949,17 @@ struct KCfg {
 };
 
 // Instantiated windows per kernel kind: rx x rz cells on `nt` threads, 8
-// cells per thread for the forward and adjoint kernels.  The tangent and the
-// adjoint carry about twice the per-cell state, which caps them at 512 threads.
+// cells per thread.  The fused tangent and the adjoint carry about twice the
+// per-cell state of the forward and Born kernels, which caps them at 512
+// threads.
 struct Shape {
   int rx, rz, nt;
 };
 const Shape kShapes[SEIS_KIND_COUNT][5] = {
     {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}},
     {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}},
-    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}}};
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}},
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}}};
 
 int shape_threads(int kind, int rx, int rz) {
   for (const Shape& sh : kShapes[kind])
@@ -824,7 +972,7 @@ bool cfg_ok(int kind, const KCfg& c) {
          c.rx - 4 * c.k >= 8;
 }
 
-const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3}, {64, 32, 3}, {64, 32, 3}};
+const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3}, {64, 32, 3}, {64, 32, 3}, {64, 32, 2}};
 
 struct Layout {
   std::vector<ModelDev> models;
@@ -895,8 +1043,8 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
         w.off[r] = L.arena;
         L.arena += align64(P) * slots;
       };
-      region(R_HIST, L.fwd_hist ? nt + 2 : (adj ? L.Kc + 1 : 0));
-      region(R_PP, L.fwd_hist ? 0 : 2);
+      region(R_HIST, 2 * (L.fwd_hist ? nt + 1 : (adj ? L.Kc + 1 : 0)));
+      region(R_PP, 2);
       region(R_Q, 2);
       region(R_DP, tan ? 2 : 0);
       region(R_DQ, tan ? 2 : 0);
@@ -951,6 +1099,8 @@ struct seis_plan {
   hipStream_t cap = nullptr;
   hipGraphExec_t g_fwd[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraphExec_t g_adj[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t g_born = nullptr;
+  std::vector<char> hist_ok;  // per model: a saving forward has filled R_HIST
 };
 
 namespace {
@@ -961,6 +1111,7 @@ void drop_graphs(seis_plan* p) {
       if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   for (auto& g : p->g_adj)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  if (p->g_born) { (void)hipGraphExecDestroy(p->g_born); p->g_born = nullptr; }
 }
 
 void plan_free(seis_plan* p) {
@@ -994,17 +1145,21 @@ int dev_alloc(T** ptr, int64_t count) {
 #define SEIS_FWD_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512) X(128, 64, 1024) X(256, 32, 1024)
 #define SEIS_TAN_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
 #define SEIS_ADJ_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
+#define SEIS_BORN_SHAPES(X) SEIS_FWD_SHAPES(X)
 
 const void* kernel_ptr(int kind, const KCfg& c) {
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_fwd<RX, RZ, NT, false>;
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_fwd<RX, RZ, NT, true>;
 #define PTR_A(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_adj<RX, RZ, NT>;
+#define PTR_B(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_born<RX, RZ, NT>;
   if (kind == SEIS_KIND_FORWARD) { SEIS_FWD_SHAPES(PTR_F) }
   else if (kind == SEIS_KIND_TANGENT) { SEIS_TAN_SHAPES(PTR_T) }
-  else { SEIS_ADJ_SHAPES(PTR_A) }
+  else if (kind == SEIS_KIND_ADJOINT) { SEIS_ADJ_SHAPES(PTR_A) }
+  else { SEIS_BORN_SHAPES(PTR_B) }
 #undef PTR_F
 #undef PTR_T
 #undef PTR_A
+#undef PTR_B
   return nullptr;
 }
 
@@ -1024,6 +1179,13 @@ void launch_adj(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
   if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_adj<RX, RZ, NT>), dim3(grid), dim3(NT), 0, s, a); return; }
   SEIS_ADJ_SHAPES(LA)
 #undef LA
+}
+
+void launch_born(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
+#define LB(RX, RZ, NT) \
+  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_born<RX, RZ, NT>), dim3(grid), dim3(NT), 0, s, a); return; }
+  SEIS_BORN_SHAPES(LB)
+#undef LB
 }
 
 // Tiles of the active wavefields for window config c, laid out so that the
@@ -1104,15 +1266,17 @@ FwdArgs fwd_args(seis_plan* p, int kind) {
 
 // Forward time loop from the zero state at n = 0 through n = nt (p[1] .. p[nt+1],
 // core/seis_forward2.py:211-247), fused with the tangent when `tan`
-// (:255-283).  In checkpoint mode with `save`, launches stop at every
-// checkpoint boundary n = j*Kc and store the state there.
+// (:255-283).  A saving forward (`save`) on a full-history plan stores the
+// pairs (Lf(p[n]), q[n]) of n = 0..nt; in checkpoint mode its launches stop at
+// every checkpoint boundary n = j*Kc and store the state there.
 void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   const Layout& L = p->L;
   const int kind = tan ? SEIS_KIND_TANGENT : SEIS_KIND_FORWARD;
   const KCfg& c = p->cfg[kind];
+  const bool hist = save && L.fwd_hist;
   const bool ck = save && !L.full && (p->caps & SEIS_CAP_ADJOINT);
   fill(p, s, R_Q, 0, 1);
-  fill(p, s, L.fwd_hist ? R_HIST : R_PP, 0, 1);
+  fill(p, s, R_PP, 0, 1);
   if (ck) {
     fill(p, s, R_CKP, 0, 1);
     fill(p, s, R_CKQ, 0, 1);
@@ -1129,15 +1293,9 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
     if (ck) K = std::min(K, (n / L.Kc + 1) * L.Kc - n);
     a.n0 = n;
     a.K = K;
-    if (L.fwd_hist) {
-      a.p_in_reg = R_HIST; a.p_in_slot = n;
-      a.p_out_reg = R_HIST; a.p_out_slot0 = n + 1; a.hist_all = 1;
-    } else {
-      a.p_in_reg = R_PP; a.p_in_slot = li & 1;
-      a.p_out_reg = R_PP; a.p_out_slot0 = (li + 1) & 1; a.hist_all = 0;
-    }
-    a.q_in_slot = a.dp_in_slot = a.dq_in_slot = li & 1;
-    a.q_out_slot = a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
+    a.p_in_slot = a.q_in_slot = a.dp_in_slot = a.dq_in_slot = li & 1;
+    a.p_out_slot = a.q_out_slot = a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
+    a.hist_slot0 = hist ? n : -1;
     a.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
     launch_fwd(c, tan, p->tl[kind].grid, s, a);
     n += K;
@@ -1145,9 +1303,35 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   }
 }
 
+// Tangent time loop n = 0..nt on the saved history of a full-history plan
+// (same recursion and outputs as the fused pass of enqueue_forward).
+void enqueue_born(seis_plan* p, hipStream_t s) {
+  fill(p, s, R_DP, 0, 1);
+  fill(p, s, R_DQ, 0, 1);
+  const KCfg& c = p->cfg[SEIS_KIND_BORN];
+  BornArgs a{};
+  a.tiles = p->tl[SEIS_KIND_BORN].d; a.n_tiles = p->tl[SEIS_KIND_BORN].d_count;
+  a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params; a.wparam = p->d_wparam;
+  a.alpha = p->d_alpha; a.dalpha = p->d_dalpha; a.wavelet = p->d_wavelet;
+  a.arena = p->d_arena; a.dseis = p->d_dseis; a.nt = p->nt;
+  a.Kt = c.k;
+  int n = 0, li = 0;
+  while (n < p->nt + 1) {
+    const int K = std::min(c.k, p->nt + 1 - n);
+    a.n0 = n;
+    a.K = K;
+    a.dp_in_slot = a.dq_in_slot = li & 1;
+    a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
+    launch_born(c, p->tl[SEIS_KIND_BORN].grid, s, a);
+    n += K;
+    ++li;
+  }
+}
+
 // Reverse time loop: lam[m] for m = nt+1 .. 2 (update_p_adjoint graph,
-// core/seis_forward2.py:292-313).  In checkpoint mode each segment of Kc
-// steps is first recomputed from its checkpoint into the segment buffer.
+// core/seis_forward2.py:292-313), reading the pairs of n = m-1.  In checkpoint
+// mode each segment is first recomputed from its checkpoint: steps
+// n = j*Kc .. min(j*Kc + Kc, nt) store their pairs into the segment buffer.
 void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
   const Layout& L = p->L;
   fill(p, s, R_LAM, 0, 1);
@@ -1165,7 +1349,7 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
   else { a.inj_a = p->d_dseis; a.inj_b = nullptr; }
   const int grid = p->tl[SEIS_KIND_ADJOINT].grid;
   int li = 0;
-  auto run_range = [&](int m_hi, int m_lo, int reg, int base) {
+  auto run_range = [&](int m_hi, int m_lo, int base) {
     int M0 = m_hi + 1;
     while (M0 - 1 >= m_lo) {
       const int K = std::min(ca.k, M0 - m_lo);
@@ -1173,42 +1357,57 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
       a.K = K;
       a.lam_in_slot = a.mu_in_slot = li & 1;
       a.lam_out_slot = a.mu_out_slot = (li + 1) & 1;
-      a.p_reg = reg;
-      a.p_base = base;
+      a.h_base = base;
       launch_adj(ca, grid, s, a);
       M0 -= K;
       ++li;
     }
   };
   if (L.full) {
-    run_range(p->nt + 1, 2, R_HIST, 0);
+    run_range(p->nt + 1, 2, 0);
     return;
   }
   const KCfg& cf = p->cfg[SEIS_KIND_FORWARD];
   FwdArgs f = fwd_args(p, SEIS_KIND_FORWARD);
   f.write_seis = 0;
-  f.p_in_reg = f.p_out_reg = R_HIST;
-  f.hist_all = 1;
   for (int j = L.J - 1; j >= 0; --j) {
     const int lo = j * L.Kc;
-    const int hi = std::min(lo + L.Kc, p->nt);  // largest p index the segment needs
+    const int hi = std::min(lo + L.Kc, p->nt);  // largest n whose pair the segment needs
     hipLaunchKernelGGL(k_restore, dim3(64, std::min(p->n_waves, 256)), dim3(NTHR), 0, s, p->d_waves,
                        p->d_models, p->n_waves, p->d_arena, j);
     int n = lo, fi = 0;
-    while (n < hi) {
-      const int K = std::min(cf.k, hi - n);
+    while (n <= hi) {
+      const int K = std::min(cf.k, hi + 1 - n);
       f.n0 = n;
       f.K = K;
-      f.p_in_slot = n - lo;
-      f.p_out_slot0 = n + 1 - lo;
-      f.q_in_slot = fi & 1;
-      f.q_out_slot = (fi + 1) & 1;
+      f.p_in_slot = f.q_in_slot = fi & 1;
+      f.p_out_slot = f.q_out_slot = (fi + 1) & 1;
+      f.hist_slot0 = n - lo;
       launch_fwd(cf, false, p->tl[SEIS_KIND_FORWARD].grid, s, f);
       n += K;
       ++fi;
     }
-    run_range(std::min(lo + L.Kc + 1, p->nt + 1), lo + 2, R_HIST, lo);
+    run_range(std::min(lo + L.Kc + 1, p->nt + 1), lo + 2, lo);
   }
+}
+
+// Record that the active models' R_HIST now holds a complete saved forward.
+void mark_saved(seis_plan* p) {
+  if (!p->L.fwd_hist) return;
+  for (int i = 0; i < p->n_models; ++i)
+    if (p->L.models[i].active) p->hist_ok[i] = 1;
+}
+
+int check_saved(seis_plan* p) {
+  if ((p->caps & (SEIS_CAP_TANGENT | SEIS_CAP_ADJOINT)) != (SEIS_CAP_TANGENT | SEIS_CAP_ADJOINT))
+    return fail(SEIS_ESTATE, "plan lacks SEIS_CAP_TANGENT | SEIS_CAP_ADJOINT");
+  if (!p->L.fwd_hist) return fail(SEIS_ESTATE, "saved-history passes need a full-history plan (ckpt_every 0)");
+  for (int i = 0; i < p->n_models; ++i)
+    if (p->L.models[i].active && !p->hist_ok[i])
+      return fail(SEIS_ESTATE, "model " + std::to_string(i) +
+                                   " has no saved forward: run seis_gradient, seis_forward_save or "
+                                   "seis_gn_product on it first");
+  return SEIS_OK;
 }
 
 template <typename F>
@@ -1295,6 +1494,7 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
   p->n_waves = (int)L.waves.size();
   p->nt = time->nt;
   p->caps = caps;
+  p->hist_ok.assign(n_models, 0);
   for (int k = 0; k < SEIS_KIND_COUNT; ++k) p->cfg[k] = kDefaultCfg[k];
   const bool tan = caps & SEIS_CAP_TANGENT, adj = caps & SEIS_CAP_ADJOINT;
   auto bail = [&](int code) { plan_free(p); return code; };
@@ -1407,6 +1607,7 @@ int seis_forward_save(seis_plan_t* p, const float* vec, float* seis, seis_stream
   TRY(run_prep(p, s, vec, nullptr));
   TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
   HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
+  mark_saved(p);
   return copy_out(p, s, p->d_seis, seis);
 }
 
@@ -1445,6 +1646,7 @@ int seis_gradient(seis_plan_t* p, const float* vec, const float* dd, int32_t on_
   HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
   TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
   HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
+  mark_saved(p);
   if (on_residual && misfit) launch_misfit(p, s, misfit);
   const int kind = on_residual ? 0 : 1;
   TRY(get_graph(p, &p->g_adj[kind], [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
@@ -1462,6 +1664,32 @@ int seis_gn_product(seis_plan_t* p, const float* vec, const float* dvec, float* 
   TRY(run_prep(p, s, vec, dvec));
   TRY(get_graph(p, &p->g_fwd[1][1], [&](hipStream_t c) { enqueue_forward(p, c, true, true); }));
   HIP_TRY(hipGraphLaunch(p->g_fwd[1][1], s));
+  mark_saved(p);
+  TRY(get_graph(p, &p->g_adj[2], [&](hipStream_t c) { enqueue_adjoint(p, c, 2); }));
+  HIP_TRY(hipGraphLaunch(p->g_adj[2], s));
+  TRY(finish_grad(p, s, vec, grad));
+  return copy_out(p, s, p->d_dseis, dseis);
+}
+
+int seis_tangent_saved(seis_plan_t* p, const float* vec, const float* dvec, float* dseis,
+                       seis_stream_t stream) {
+  if (!p || !vec || !dvec) return fail(SEIS_EINVAL, "NULL argument");
+  TRY(check_saved(p));
+  hipStream_t s = (hipStream_t)stream;
+  TRY(run_prep(p, s, vec, dvec));
+  TRY(get_graph(p, &p->g_born, [&](hipStream_t c) { enqueue_born(p, c); }));
+  HIP_TRY(hipGraphLaunch(p->g_born, s));
+  return copy_out(p, s, p->d_dseis, dseis);
+}
+
+int seis_gn_product_saved(seis_plan_t* p, const float* vec, const float* dvec, float* dseis, double* grad,
+                          seis_stream_t stream) {
+  if (!p || !vec || !dvec || !grad) return fail(SEIS_EINVAL, "NULL argument");
+  TRY(check_saved(p));
+  hipStream_t s = (hipStream_t)stream;
+  TRY(run_prep(p, s, vec, dvec));
+  TRY(get_graph(p, &p->g_born, [&](hipStream_t c) { enqueue_born(p, c); }));
+  HIP_TRY(hipGraphLaunch(p->g_born, s));
   TRY(get_graph(p, &p->g_adj[2], [&](hipStream_t c) { enqueue_adjoint(p, c, 2); }));
   HIP_TRY(hipGraphLaunch(p->g_adj[2], s));
   TRY(finish_grad(p, s, vec, grad));

@@ -341,16 +341,19 @@ class Executor:
     ``slots`` lists one geometry per slot; request i of a batch occupies slot
     i.  ``max_batch`` > 1 runs each op kind as one launch over every slot that
     has one pending (inactive slots are skipped by the kernels); max_batch = 1
-    is the plain path: every op runs alone on a single-model plan.
+    is the plain path: every op runs alone on a single-model plan.  With
+    ``reuse_forward`` (full-history plans), J^T J products at the point of the
+    models' last saving forward skip the forward recomputation.
     """
 
     def __init__(self, slots: Sequence[Geometry], ckpt_every: int = 0, max_batch: int | None = None,
-                 fwd_fanout: int = 50, device="cuda"):
+                 fwd_fanout: int = 50, reuse_forward: bool = True, device="cuda"):
         self.slots = list(slots)
         self.n = len(self.slots)
         self.max_batch = self.n if max_batch is None else max_batch
         self.ckpt_every = ckpt_every
         self.fwd_fanout = fwd_fanout
+        self.reuse_forward = reuse_forward
         self.device = torch.device(device)
         self.targets: list = [None] * self.n
         self._plans: dict = {}
@@ -431,7 +434,12 @@ class Executor:
         P, loc = self._local(ids)
         v = self._pack(P, loc, [pending[i].vec for i in ids])
         dv = self._pack(P, loc, [pending[i].dvec for i in ids])
-        _, grad = P.gn_product(v, dv)
+        if self.reuse_forward and P.saved_matches(v):
+            # every model's last saving forward (its fev or previous product)
+            # ran at this point: tangent from the saved history
+            _, grad = P.gn_product_saved(v, dv, check=False)
+        else:
+            _, grad = P.gn_product(v, dv)
         return {i: grad[int(P.vec_off[l]):int(P.vec_off[l + 1])].clone() for i, l in zip(ids, loc)}
 
     def _fwd(self, ids, pending):
@@ -527,10 +535,10 @@ def bfgs(cost_and_gradient_func, x0, max_iter, tolerance_grad, tolerance_change,
 
 
 def invert(requests: Sequence[Request], method: str = "lbfgs", batched: bool = True,
-           ckpt_every: int = 0, **kw) -> list:
+           ckpt_every: int = 0, reuse_forward: bool = True, **kw) -> list:
     """Invert a list of requests; batched=False runs the plain one-at-a-time path."""
     ex = Executor([r.geom for r in requests], ckpt_every=ckpt_every,
-                  max_batch=None if batched else 1)
+                  max_batch=None if batched else 1, reuse_forward=reuse_forward)
     machines = {}
     for i, r in enumerate(requests):
         ex.set_target(i, r.target)

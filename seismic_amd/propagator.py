@@ -101,6 +101,13 @@ class Propagator:
         self.device_bytes = int(L.seis_plan_device_bytes(self._h))
         self._gen = 0          # bumped by every call that rewrites wavefield state
         self._saved_gen = -1   # generation of the last forward_save
+        # Full-history plans keep every step of a saving forward (gradient,
+        # forward_save, gn_product); the *_saved passes reuse it.  _lin holds
+        # the parameter vector each model's saved forward ran at.
+        full = ckpt_every == 0 or ckpt_every >= self.geoms[0].nt
+        self.full_history = bool(full and (caps & CAP_ADJOINT))
+        self._lin = torch.full((self.n_vec,), float("nan"), device=self.device) if self.full_history else None
+        self._act_idx = None   # parameter indices of the active models (None = all)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -126,7 +133,8 @@ class Propagator:
             raise ValueError(f"data vector has {d.numel()} entries, plan expects {self.n_seis}")
         return d.to(device=self.device, dtype=torch.float32).contiguous()
 
-    KINDS = {"forward": _lib.KIND_FORWARD, "tangent": _lib.KIND_TANGENT, "adjoint": _lib.KIND_ADJOINT}
+    KINDS = {"forward": _lib.KIND_FORWARD, "tangent": _lib.KIND_TANGENT, "adjoint": _lib.KIND_ADJOINT,
+             "born": _lib.KIND_BORN}
 
     def set_config(self, kind: str, rx: int, rz: int, k: int):
         """Launch config of one kernel kind: rx x rz LDS window, k steps per launch."""
@@ -139,8 +147,33 @@ class Propagator:
         return rx.value, rz.value, k.value
 
     def set_active(self, mask: Sequence[bool]):
+        mask = [bool(a) for a in mask]
         m = (ctypes.c_int32 * self.n_models)(*[1 if a else 0 for a in mask])
         _lib.check(_lib.lib().seis_set_active(self._h, m), "seis_set_active")
+        if all(mask):
+            self._act_idx = None
+        else:
+            idx = [np.arange(self.vec_off[i], self.vec_off[i + 1]) for i, a in enumerate(mask) if a]
+            self._act_idx = torch.from_numpy(np.concatenate(idx) if idx else np.zeros(0, np.int64)).to(self.device)
+
+    def _note_saved(self, v: torch.Tensor):
+        """The active models' saved forward now belongs to v."""
+        if self._lin is None:
+            return
+        if self._act_idx is None:
+            self._lin.copy_(v)
+        else:
+            self._lin.index_copy_(0, self._act_idx, v.index_select(0, self._act_idx))
+
+    def saved_matches(self, vec) -> bool:
+        """True when every active model's saved forward ran at vec (one
+        device-to-host sync), i.e. the *_saved passes apply."""
+        if self._lin is None or not (self.caps & CAP_TANGENT):
+            return False
+        v = self._vec(vec)
+        if self._act_idx is None:
+            return bool(torch.equal(v, self._lin))
+        return bool(torch.equal(v.index_select(0, self._act_idx), self._lin.index_select(0, self._act_idx)))
 
     def split_vec(self, v: torch.Tensor):
         return [v[self.vec_off[i]:self.vec_off[i + 1]] for i in range(self.n_models)]
@@ -183,6 +216,7 @@ class Propagator:
         misfit = torch.empty(self.n_models, device=self.device, dtype=torch.float64) if on_residual else None
         _lib.check(_lib.lib().seis_gradient(self._h, _ptr(v), _ptr(d), int(bool(on_residual)), _ptr(seis),
                                             _ptr(misfit), _ptr(grad), _stream()), "seis_gradient")
+        self._note_saved(v)
         return seis, misfit, grad
 
     def forward_save(self, vec) -> tuple:
@@ -192,6 +226,7 @@ class Propagator:
         v = self._vec(vec)
         seis = torch.empty(self.n_seis, device=self.device)
         _lib.check(_lib.lib().seis_forward_save(self._h, _ptr(v), _ptr(seis), _stream()), "seis_forward_save")
+        self._note_saved(v)
         self._saved_gen = self._gen
         return seis, self._gen
 
@@ -218,6 +253,34 @@ class Propagator:
         grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)
         _lib.check(_lib.lib().seis_gn_product(self._h, _ptr(v), _ptr(dv), _ptr(dseis), _ptr(grad),
                                               _stream()), "seis_gn_product")
+        self._note_saved(v)
+        return dseis, grad
+
+    def _check_lin(self, v, check: bool):
+        if check and not self.saved_matches(v):
+            raise RuntimeError("vec is not the point of the saved forward of every active model "
+                               "(run gradient, forward_save or gn_product at it first)")
+
+    def tangent_saved(self, vec, dvec, check: bool = True) -> torch.Tensor:
+        """J dvec from the saved forward at vec (no forward recomputation);
+        bit-identical to ``tangent(vec, dvec)[1]``.  ``check`` verifies that
+        vec is the saved point."""
+        v, dv = self._vec(vec), self._vec(dvec)
+        self._check_lin(v, check)
+        dseis = torch.empty(self.n_seis, device=self.device)
+        _lib.check(_lib.lib().seis_tangent_saved(self._h, _ptr(v), _ptr(dv), _ptr(dseis), _stream()),
+                   "seis_tangent_saved")
+        return dseis
+
+    def gn_product_saved(self, vec, dvec, check: bool = True):
+        """(J dvec, J^T J dvec) from the saved forward at vec; bit-identical
+        to ``gn_product``."""
+        v, dv = self._vec(vec), self._vec(dvec)
+        self._check_lin(v, check)
+        dseis = torch.empty(self.n_seis, device=self.device)
+        grad = torch.empty(self.n_vec, device=self.device, dtype=torch.float64)
+        _lib.check(_lib.lib().seis_gn_product_saved(self._h, _ptr(v), _ptr(dv), _ptr(dseis), _ptr(grad),
+                                                    _stream()), "seis_gn_product_saved")
         return dseis, grad
 
 
@@ -261,7 +324,8 @@ def vel_to_seis(vec, vec_diff=None, vec_adjoint=None, adjoint_on_residual=False,
         seis, _, grad = P.gradient(vec, vec_adjoint, on_residual=adjoint_on_residual)
         result_adjoint = grad.reshape(-1, 1)
         if vec_diff is not None:
-            _, dseis = P.tangent(vec, vec_diff)
+            # the gradient's forward was saved at vec: no second forward
+            dseis = P.tangent_saved(vec, vec_diff, check=False)
             result_diff = dseis.double().reshape(-1, 1)
     elif vec_diff is not None:
         seis, dseis = P.tangent(vec, vec_diff)

@@ -103,6 +103,23 @@ def test_batched_equals_plain(method):
         assert torch.equal(a, b)
 
 
+def test_gn_reuse_forward_identical():
+    """GN products from the saved forward (full history) give bit-identical
+    inversions to recomputing the forward per product, and to checkpointing."""
+    reqs = {k: [] for k in ("reuse", "recompute", "ckpt")}
+    for s in range(2):
+        g, og, true, target, start = _problem(20 + s)
+        for k in reqs:
+            reqs[k].append(_req(g, target, start, priors.L2Smoothness(g.nz, g.nx, lam=1e-9), s))
+    kw = dict(n_outer=2, n_cg=3, n_scales=7)
+    va = invert.invert(reqs["reuse"], method="gn", reuse_forward=True, **kw)
+    vb = invert.invert(reqs["recompute"], method="gn", reuse_forward=False, **kw)
+    vc = invert.invert(reqs["ckpt"], method="gn", ckpt_every=13, **kw)
+    for a, b, c in zip(va, vb, vc):
+        assert torch.equal(a, b)
+        assert torch.equal(a, c)
+
+
 def test_cost_and_gradient_dropin_fd():
     """core/seis_test.py:71-89 recipe on the GPU path: relative FD error < 1e-4
     (float32 propagator, float64 cost)."""

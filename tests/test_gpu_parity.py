@@ -117,6 +117,9 @@ def test_tile_configs_bit_identical(small_geoms):
             s, mis, gr = P.gradient(torch.tensor(vec), torch.tensor(target))
             s2, ds = P.tangent(torch.tensor(vec), torch.tensor(dv))
             out = (_np(s), _np(gr), _np(ds), float(mis[0]))
+            if ckpt == 0:   # Born pass on the gradient's saved forward
+                P.set_config("born", *cfg)
+                assert np.array_equal(_np(P.tangent_saved(torch.tensor(vec), torch.tensor(dv))), out[2]), cfg
             if ref is None:
                 ref = out
                 continue
@@ -138,6 +141,68 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
     ref = _np(P.forward(torch.tensor(vec)))
     P.set_config("forward", *cfg)
     assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref)
+
+
+@pytest.mark.parametrize("cfg", [(64, 32, 1), (128, 64, 4), (256, 32, 3), (64, 64, 8)])
+def test_born_saved_matches_fused(small_geoms, cfg):
+    """Tangent and GN product from the saved forward are bit-identical to the
+    passes that recompute the forward, and match the oracle."""
+    g, og = small_geoms
+    v, vec = _case(g, "curved", 4)
+    rng = np.random.default_rng(5)
+    dm = np.concatenate([rng.normal(0, 20, g.nz * g.nx), [3.0]])
+    P = Propagator([g])
+    P.set_config("born", *cfg)
+    _, ds_fused = P.tangent(torch.tensor(vec), torch.tensor(dm))
+    dd = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 0.99), og)[0])
+    P.gradient(torch.tensor(vec), dd)
+    ds_saved = P.tangent_saved(torch.tensor(vec), torch.tensor(dm))
+    assert np.array_equal(_np(ds_saved), _np(ds_fused))
+    d1, g1 = P.gn_product(torch.tensor(vec), torch.tensor(dm))
+    d2, g2 = P.gn_product_saved(torch.tensor(vec), torch.tensor(dm))
+    assert np.array_equal(_np(d1), _np(d2))
+    assert np.array_equal(_np(g1), _np(g2))
+    _, jdm, _ = fd2d.vel_to_seis(vec, og, vec_diff=dm)
+    _, _, jtjdm = fd2d.vel_to_seis(vec, og, vec_adjoint=jdm)
+    assert rel(_np(d2), jdm) < SEIS_TOL
+    assert rel(_np(g2), jtjdm) < GRAD_TOL
+
+
+def test_saved_passes_need_a_saved_forward(small_geoms):
+    g, og = small_geoms
+    _, vec = _case(g, "flat", 1)
+    v = torch.tensor(vec)
+    P = Propagator([g])
+    with pytest.raises(RuntimeError):      # no saving forward yet (library check)
+        P.tangent_saved(v, v, check=False)
+    P.forward(v)                           # a plain forward saves nothing
+    with pytest.raises(RuntimeError):
+        P.gn_product_saved(v, v, check=False)
+    P.forward_save(v)
+    assert P.saved_matches(v)
+    assert not P.saved_matches(v * 1.01)
+    with pytest.raises(RuntimeError):      # a different point (host check)
+        P.tangent_saved(v * 1.01, v)
+    P.tangent_saved(v, v)
+    with pytest.raises(RuntimeError):      # checkpointed plans keep no full history
+        Propagator([g], ckpt_every=11).tangent_saved(v, v, check=False)
+    # batch: a saving forward on model 0 alone leaves model 1 without history
+    Pb = Propagator([g, g])
+    vb = torch.cat([v, v])
+    Pb.set_active([True, False])
+    Pb.forward_save(vb)
+    assert Pb.saved_matches(vb)
+    Pb.set_active([True, True])
+    assert not Pb.saved_matches(vb)
+    with pytest.raises(RuntimeError):
+        Pb.tangent_saved(vb, vb, check=False)
+    Pb.set_active([False, True])
+    Pb.gradient(vb, torch.zeros(Pb.n_seis))
+    Pb.set_active([True, True])
+    assert Pb.saved_matches(vb)
+    ds = Pb.tangent_saved(vb, vb)
+    _, ds_ref = Pb.tangent(vb, vb)
+    assert np.array_equal(_np(ds), _np(ds_ref))
 
 
 def test_dot_product(small_geoms):
