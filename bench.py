@@ -3,8 +3,9 @@
 Workload (one MI355X): synthetic 70x70 models, curved layers with a dipping
 fault (seeded), depth-linear start 1500 -> 4000 m/s, L2 smoothness prior,
 5 shots, 70 receivers, nt = 1000 steps of 1 ms.  One inversion = 10
-Gauss-Newton passes, each with 20 CG steps on J^T J (one tangent + one
-adjoint pass per step) and the reference's 50-point line search.
+Gauss-Newton passes, each with 20 CG steps on J^T J (one Born pass on the
+pass's saved forward + one adjoint pass per step) and the reference's
+50-point line search.
 
 A *step* is one Gauss-Newton pass over the whole batch.  ``value`` is the
 batch's inversions per second (steps/10 inversions per model); the same
@@ -33,7 +34,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12                     # MI355X HBM3E bytes/s
-ALG_BYTES = {"forward": 20, "tangent": 40, "adjoint": 28}   # per cell-update (float32)
+# Algorithmic HBM bytes per cell-update (float32, one padded cell advanced one
+# step for one shot): forward reads p[t-1], p[t], alpha, kappa and writes
+# p[t+1]; the fused tangent also reads dp[t-1], dp[t], dalpha, dkappa and
+# writes dp[t+1]; the Born pass reads dp[t-1], dp[t], alpha, kappa, dalpha,
+# dkappa and the saved pair (Lf(p[t]), q[t]) and writes dp[t+1]; the adjoint
+# reads lam[t+1], lam[t+2], alpha, kappa and the pair of p[t] and writes lam[t].
+ALG_BYTES = {"forward": 20, "tangent": 40, "born": 36, "adjoint": 28}
 N_OUTER, N_CG, N_SCALES = 10, 20, 50
 
 
@@ -102,15 +109,18 @@ def kernel_roofline(P, B, g, reps=3):
 
     t_fwd = timed(lambda: P.forward(vec))
     t_tan = timed(lambda: P.tangent(vec, vec))
-    t_grad = timed(lambda: P.gradient(vec, dd))
+    _, tok = P.forward_save(vec)
+    t_adj = timed(lambda: P.adjoint(vec, dd, tok, on_residual=True))
+    t_born = timed(lambda: P.tangent_saved(vec, vec, check=False))
     cells = B * g.n_shots * g.padded_cells
     out = {}
     for kind, t_pass, steps in [("forward", t_fwd, g.nt + 1), ("tangent", t_tan, g.nt + 1),
-                                ("adjoint", t_grad - t_fwd, g.nt)]:
+                                ("born", t_born, g.nt + 1), ("adjoint", t_adj, g.nt)]:
         k = P.get_config(kind)[2]
         launches = math.ceil(steps / k)
         out[kind] = {"pass_s": t_pass, "launches": launches, "launch_s": t_pass / launches,
-                     "cell_updates_per_launch": cells * k, "k": k, "window": P.get_config(kind)[:2]}
+                     "cell_updates_per_launch": cells * k, "k": k, "window": P.get_config(kind)[:2],
+                     "strips": P.get_config(kind)[3]}
     return out
 
 
@@ -264,7 +274,9 @@ def main():
     P.set_active([True] * B)
     P._mask = tuple([True] * B)
     kr = kernel_roofline(P, B, g)
-    per_outer = {"forward": 1 + N_SCALES, "tangent": N_CG, "adjoint": 1 + N_CG}
+    # passes per GN iteration: 1 saving forward + 50 line-search forwards,
+    # 20 Born passes (J^T J products on the saved forward), 21 adjoints
+    per_outer = {"forward": 1 + N_SCALES, "tangent": 0, "born": N_CG, "adjoint": 1 + N_CG}
     share = {k: per_outer[k] * kr[k]["pass_s"] for k in kr}
     dom = max(share, key=share.get)
     kd = kr[dom]
@@ -313,13 +325,13 @@ def main():
                    "model0_plain": preq[0].meta["stats"]["misfit_hist"],
                    "equal": hist_b0[:args.plain_steps] == preq[0].meta["stats"]["misfit_hist"]},
         "serving": serving,
-        "roofline": {"bound": "hbm", "kernel": f"k_{'fwd' if dom != 'adjoint' else 'adj'}"
-                     f"{'(tangent)' if dom == 'tangent' else ''}",
+        "roofline": {"bound": "hbm", "kernel": {"forward": "k_fwd", "tangent": "k_fwd(tangent)",
+                                                "born": "k_born", "adjoint": "k_adj"}[dom],
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "launch_us": kd["launch_s"] * 1e6, "alg_bytes_per_cell_update": ALG_BYTES[dom],
                      "cell_updates_per_launch": kd["cell_updates_per_launch"], "k": kd["k"],
-                     "window": kd["window"]},
+                     "window": kd["window"], "strips": kd["strips"]},
         "kernels": {k: {"launch_us": v["launch_s"] * 1e6, "k": v["k"],
                         "gcell_updates_per_s": v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
                         "alg_GBps": ALG_BYTES[k] * v["cell_updates_per_launch"] / v["launch_s"] / 1e9}

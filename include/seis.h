@@ -178,6 +178,9 @@ enum {
 /* Launch configuration of one kernel kind: an rx x rz cell window (halo
  * included) is staged in LDS per workgroup and advanced k time steps per
  * launch; the window's interior, (rz-4k) x (rx-4k) cells, is written back.
+ * strips = 1 gives each wavefront consecutive window rows (vertical stencil
+ * neighbours stay in registers), strips = 0 deals rows round-robin over the
+ * wavefronts (even work per wavefront as the valid band shrinks).
  * Windows (rx x rz): forward and Born 64x32, 128x32, 64x64, 128x64, 256x32;
  * tangent and adjoint 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of
  * at least 4 rows and 8 columns.  Results do not depend on the config
@@ -185,9 +188,9 @@ enum {
  * one 32x32-thread launch per step (core/seis_forward2.py:191-193).
  * Setting a config synchronizes the device and drops captured graphs. */
 int seis_plan_set_config(seis_plan_t* plan, int32_t kind, int32_t rx, int32_t rz,
-                         int32_t k);
+                         int32_t k, int32_t strips);
 int seis_plan_get_config(const seis_plan_t* plan, int32_t kind, int32_t* rx,
-                         int32_t* rz, int32_t* k /* host */);
+                         int32_t* rz, int32_t* k, int32_t* strips /* host */);
 
 /* Device bytes held by the plan. */
 int64_t seis_plan_device_bytes(const seis_plan_t* plan);

@@ -45,8 +45,9 @@ EXPORTS = {
     "seis_tangent_saved": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_gn_product_saved": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_plan_device_bytes": (c_int64, [c_void_p]),
-    "seis_plan_set_config": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32]),
-    "seis_plan_get_config": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+    "seis_plan_set_config": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32]),
+    "seis_plan_get_config": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                       POINTER(c_int32)]),
     "seis_last_error": (c_char_p, []),
 }
 

@@ -1,9 +1,9 @@
 """Autotuner for the step kernels' launch configuration.
 
 Each kernel kind (forward, fused tangent, adjoint, Born on the saved
-forward) runs an rx x rz cell
-window per workgroup and advances k time steps per launch (libseis
-``seis_plan_set_config``).  The best choice depends on how many cells one
+forward) runs an rx x rz cell window per workgroup, advances k time steps
+per launch and deals window rows to wavefronts either as consecutive strips
+or round-robin (libseis ``seis_plan_set_config``).  The best choice depends on how many cells one
 time step covers (batch size x model size: a single 70x70 model leaves most
 of the GPU idle with large windows) and on whether the forward writes the full
 history.  ``tune`` times every candidate on a live plan; ``apply`` looks a
@@ -27,14 +27,14 @@ from ._lib import CAP_ADJOINT, CAP_TANGENT
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_configs.json")
 
-CANDIDATES = {
-    "forward": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3),
-                (64, 64, 4), (64, 64, 6), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3)],
-    "tangent": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
-    "adjoint": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
-    "born": [(64, 32, 1), (64, 32, 2), (64, 32, 3), (128, 32, 2), (128, 32, 3), (64, 64, 2), (64, 64, 3),
-             (128, 64, 2), (128, 64, 3), (128, 64, 4), (256, 32, 2), (256, 32, 3)],
+_WINDOWS = {
+    "forward": [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3)],
+    "tangent": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (64, 64, 3), (64, 64, 4)],
+    "adjoint": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (64, 64, 3), (64, 64, 4)],
+    "born": [(64, 32, 2), (64, 32, 3), (128, 32, 3), (64, 64, 3), (128, 64, 3), (128, 64, 4), (256, 32, 3)],
 }
+# (rx, rz, k, strips): every window with both row ownerships
+CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WINDOWS.items()}
 KINDS = ("forward", "tangent", "adjoint", "born")
 
 

@@ -35,6 +35,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "seis.h"
@@ -130,21 +131,61 @@ __device__ __forceinline__ int damp_j(int z, int x, int nz, int nx, int nbc) {
   return 0;
 }
 
-// Lf(p) at (r, c) of an LDS window with row length W: lapg(p) + 2*c1*p written
-// as neighbour-minus-centre differences (exact cancellation of the centre
-// weight, 4*c2 + 4*c3 = -2*c1).  Fixed evaluation order: every kernel that
-// advances a wavefield shares it, so results do not depend on tiling.
+// Lf(p) at window row r (LDS row r + 2), column cc (LDS column cc + 2):
+// lapg(p) + 2*c1*p written as neighbour-minus-centre differences (exact
+// cancellation of the centre weight, 4*c2 + 4*c3 = -2*c1).  The vertical
+// neighbours come from `col` (col[t] = window row r0 - 2 + t of this column,
+// held in registers for the thread's own rows), the horizontal ones from the
+// LDS window T.  Fixed evaluation order: every kernel that advances a
+// wavefield shares it, so results do not depend on tiling.
 template <int W>
-__device__ __forceinline__ float lap_full(const float (*T)[W], int r, int c) {
-  const float pc = T[r][c];
-  float t1 = (T[r][c + 1] - pc) + (T[r][c - 1] - pc);
-  t1 = t1 + (T[r + 1][c] - pc);
-  t1 = t1 + (T[r - 1][c] - pc);
-  float t2 = (T[r][c + 2] - pc) + (T[r][c - 2] - pc);
-  t2 = t2 + (T[r + 2][c] - pc);
-  t2 = t2 + (T[r - 2][c] - pc);
+__device__ __forceinline__ float lap_col(const float (*T)[W], const float* col, int j, int r, int cc) {
+  const float pc = col[j + 2];
+  float t1 = (T[r + 2][cc + 3] - pc) + (T[r + 2][cc + 1] - pc);
+  t1 = t1 + (col[j + 3] - pc);
+  t1 = t1 + (col[j + 1] - pc);
+  float t2 = (T[r + 2][cc + 4] - pc) + (T[r + 2][cc] - pc);
+  t2 = t2 + (col[j + 4] - pc);
+  t2 = t2 + (col[j] - pc);
   return fmaf(kC2, t1, kC3 * t2);
 }
+
+// Same stencil with every neighbour read from the LDS window and the centre
+// value pc (the thread's own cell) from a register.
+template <int W>
+__device__ __forceinline__ float lap_lds(const float (*T)[W], float pc, int r, int cc) {
+  float t1 = (T[r + 2][cc + 3] - pc) + (T[r + 2][cc + 1] - pc);
+  t1 = t1 + (T[r + 3][cc + 2] - pc);
+  t1 = t1 + (T[r + 1][cc + 2] - pc);
+  float t2 = (T[r + 2][cc + 4] - pc) + (T[r + 2][cc] - pc);
+  t2 = t2 + (T[r + 4][cc + 2] - pc);
+  t2 = t2 + (T[r][cc + 2] - pc);
+  return fmaf(kC2, t1, kC3 * t2);
+}
+
+// Window row of a thread's j-th owned row.  STRIP: each wavefront owns RPW
+// consecutive rows (vertical neighbours of inner rows stay in registers);
+// otherwise rows are dealt round-robin over the NW wavefronts, which keeps
+// the shrinking valid band and the interior evenly spread over them.
+template <bool STRIP, int NW, int RPW>
+__device__ __forceinline__ int own_row(int wv, int j) {
+  return STRIP ? wv * RPW + j : wv + NW * j;
+}
+
+// Vertical strip of column cc for a thread owning window rows r0 .. r0+RPW-1:
+// two halo rows above and below from the LDS window, the own rows from `own`
+// (stride CPR, starting at chunk c).
+template <int W, int RPW, int CPR>
+__device__ __forceinline__ void load_col(const float (*T)[W], const float* own, int c, int r0, int cc,
+                                         float* col) {
+  col[0] = T[r0][cc + 2];
+  col[1] = T[r0 + 1][cc + 2];
+#pragma unroll
+  for (int j = 0; j < RPW; ++j) col[j + 2] = own[j * CPR + c];
+  col[RPW + 2] = T[r0 + RPW + 2][cc + 2];
+  col[RPW + 3] = T[r0 + RPW + 3][cc + 2];
+}
+
 
 // Tile of the compacted work list: one (wavefield, interior origin) pair.
 // Interior origins are unwrapped padded-grid coordinates; the window staged
@@ -191,25 +232,34 @@ __device__ __forceinline__ int rcv_index(int gx, const ModelDev& m) {
 
 // K forward time steps (update_p :428-462 with lapg :381-421 and the source
 // injection fused), and with TAN the Born recursion (update_p_diff :470-509),
-// for every tile of the work list.  Thread-owned cells keep q, alpha and
-// (1-kappa) in registers across the K steps; p (and dp) live in LDS.  No
-// global load is issued inside the step loop: the wavelet samples of the
-// launch are staged in LDS with the window.  A saving forward also stores,
-// per interior cell and step, the pair (Lf(p[n]), q[n]) that the tangent and
-// adjoint recursions consume.
-template <int RX, int RZ, int NT, bool TAN>
+// for every tile of the work list.  Each thread owns RPW window rows (see
+// own_row) and one column per 64-column chunk, and keeps p, q, alpha and
+// (1-kappa) of its cells in registers; with STRIP only the horizontal
+// neighbours and two halo rows above and below the strip are read from LDS.
+// The LDS window is double-buffered: step s reads buffer s&1 and writes the
+// other, one barrier per step.  Every owned cell is advanced each step (cells
+// outside the valid band, which shrinks by 2 per step, compute values no
+// committed cell depends on), so the row loop has no branches and the LDS
+// reads of different rows can be in flight together; seismogram samples are
+// stored after it.  TAN (two stencils per cell) skips rows outside the band
+// instead, which keeps its register count down.  No global load is issued inside the step loop: the
+// launch's wavelet samples are staged in LDS with the window.  With HIST (a
+// saving forward) the kernel also stores, per interior cell and step, the pair
+// (Lf(p[n]), q[n]) that the tangent and adjoint recursions consume.
+template <int RX, int RZ, int NT, bool TAN, bool STRIP, bool HIST>
 __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
   static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
-  // LDS windows carry a 2-cell apron on every side so that every owned
-  // cell's stencil stays inside the array; the apron is never read for a
-  // committed cell.  Logical window cell (r, c) lives at [r + 2][c + 2].
+  // Window cell (r, c) lives at [r + 2][c + 2]: a 2-cell apron keeps every
+  // stencil inside the array.  The apron and cells outside the valid band
+  // hold stale values that no committed cell depends on.
   constexpr int PW = RX + 4, PH = RZ + 4;
-  __shared__ float P[PH][PW];
-  __shared__ float D[TAN ? PH : 1][TAN ? PW : 1];
+  __shared__ float P0[PH][PW], P1[PH][PW];
+  __shared__ float D0[TAN ? PH : 1][TAN ? PW : 1], D1[TAN ? PH : 1][TAN ? PW : 1];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r0 = wv * RPW;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
   for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
@@ -221,16 +271,14 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
     const int64_t Pn = (int64_t)NZ * NX;
     const float* pin = a.arena + w.off[R_PP] + a.p_in_slot * Pn;
     const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
-    float2* hist = a.hist_slot0 >= 0
-                       ? reinterpret_cast<float2*>(a.arena + w.off[R_HIST]) + a.hist_slot0 * Pn
-                       : nullptr;
+    float2* hist = HIST ? reinterpret_cast<float2*>(a.arena + w.off[R_HIST]) + a.hist_slot0 * Pn : nullptr;
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float amp = a.wparam[te.w].amp;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;  // unwrapped index of window cell (0, 0)
-    float q[NC], al[NC], k1[NC], pn[NC];
-    float dq[TAN ? NC : 1], dal[TAN ? NC : 1], dk[TAN ? NC : 1], dpn[TAN ? NC : 1];
+    float pv[NC], q[NC], al[NC], k1[NC];
+    float dpv[TAN ? NC : 1], dq[TAN ? NC : 1], dal[TAN ? NC : 1], dk[TAN ? NC : 1];
     float dkscale = 0.f, amp_d = 0.f;
     const float* dpin = nullptr;
     const float* dqin = nullptr;
@@ -249,100 +297,114 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
 #pragma unroll
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
+    for (int j = 0; j < RPW; ++j) {
+      const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
-      if (gz == m.rcv_row) rcvmask |= 1u << k;
+      if (gz == m.rcv_row) rcvmask |= 1u << j;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
-        const int i = k * CPR + c;
+        const int i = j * CPR + c;
         const int cc = lane + 64 * c;
         const int gx = wrapi(ox + cc, NX);
         const int g = gz * NX + gx;
-        P[r + 2][cc + 2] = pin[g];
+        pv[i] = pin[g];
+        P0[r + 2][cc + 2] = pv[i];
         q[i] = qin[g];
         al[i] = alpha[g];
-        const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
-        const float jj = (float)(j * j);
+        const int jd = damp_j(gz, gx, m.nz, m.nx, m.nbc);
+        const float jj = (float)(jd * jd);
         k1[i] = 1.0f - kscale * jj;
         if (g == w.src_cell) srcmask |= 1u << i;
         if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
           intmask |= 1u << i;
         if constexpr (TAN) {
-          D[r + 2][cc + 2] = dpin[g];
+          dpv[i] = dpin[g];
+          D0[r + 2][cc + 2] = dpv[i];
           dq[i] = dqin[g];
           dal[i] = a.dalpha[m.cell_off + g];
           dk[i] = dkscale * jj;
         }
-        pn[i] = 0.f;
-        if constexpr (TAN) dpn[i] = 0.f;
       }
     }
     __syncthreads();
-    for (int s = 0; s < a.K; ++s) {
+    // Step s reads the window buffer s&1 and writes the other; the buffer
+    // is a compile-time choice so that reads and writes never alias.
+    auto step = [&](auto cb, const int s) {
+      constexpr int CUR = decltype(cb)::value;
+      float (*const Prd)[PW] = CUR ? P1 : P0;
+      float (*const Pwr)[PW] = CUR ? P0 : P1;
+      float (*const Drd)[TAN ? PW : 1] = CUR ? D1 : D0;
+      float (*const Dwr)[TAN ? PW : 1] = CUR ? D0 : D1;
       const int it = a.n0 + s - 1;
       const bool live = it >= 0 && it < a.nt;
       const float sv = SW[s];
       const float src = amp * sv;
       const float srcd = amp_d * sv;
       const int lo = 2 * (s + 1);
-      float2* hs = hist ? hist + s * Pn : nullptr;
-      // Forward: every owned cell is computed (its stencil stays inside the
-      // padded window) and only the valid band [lo, RZ-lo) x [lo, RX-lo) is
-      // committed, which keeps the cell loop branch-free.  Tangent: rows
-      // outside the band are skipped (two stencils per cell would otherwise
-      // exceed the register budget).  Interior rows are always in the band.
-#pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-        const bool rok = r >= lo && r < RZ - lo;
-        if constexpr (TAN) {
-          if (!rok) continue;
-        }
+      float2* hs = HIST ? hist + s * Pn : nullptr;
+      if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {  // the strip meets the valid band (wave-uniform)
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
-          const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          const bool ok = rok && cc >= lo && cc < RX - lo;
-          const float L = lap_full<PW>(P, r + 2, cc + 2);
-          if (hs && ((intmask >> i) & 1u)) hs[gint0 + r * NX + cc] = make_float2(L, q[i]);
-          float qn = fmaf(k1[i], q[i], al[i] * L);
-          if ((srcmask >> i) & 1u) qn = qn + src;
-          if constexpr (TAN) {
-            // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
-            const float Ld = lap_full<PW>(D, r + 2, cc + 2);
-            float dqn = fmaf(k1[i], dq[i], al[i] * Ld);
-            dqn = fmaf(dal[i], L, dqn);
-            dqn = fmaf(-dk[i], q[i], dqn);
-            if ((srcmask >> i) & 1u) dqn = dqn + srcd;
-            dpn[i] = ok ? D[r + 2][cc + 2] + dqn : dpn[i];
-            dq[i] = ok ? dqn : dq[i];
+          float col[STRIP ? RPW + 4 : 1];
+          if constexpr (STRIP) load_col<PW, RPW, CPR>(Prd, pv, c, r0, cc, col);
+          float dcol[TAN && STRIP ? RPW + 4 : 1];
+          if constexpr (TAN && STRIP) load_col<PW, RPW, CPR>(Drd, dpv, c, r0, cc, dcol);
+#pragma unroll
+          for (int j = 0; j < RPW; ++j) {
+            const int r = own_row<STRIP, NW, RPW>(wv, j);
+            if constexpr (TAN) {  // two stencils: skip rows outside the band to save registers
+              if (r < lo || r >= RZ - lo) continue;
+            }
+            const int i = j * CPR + c;
+            float L;
+            if constexpr (STRIP) L = lap_col<PW>(Prd, col, j, r, cc);
+            else L = lap_lds<PW>(Prd, pv[i], r, cc);
+            if constexpr (HIST) {
+              if ((intmask >> i) & 1u) hs[gint0 + r * NX + cc] = make_float2(L, q[i]);
+            }
+            float qn = fmaf(k1[i], q[i], al[i] * L);
+            qn = ((srcmask >> i) & 1u) ? qn + src : qn;
+            if constexpr (TAN) {
+              // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
+              float Ld;
+              if constexpr (STRIP) Ld = lap_col<PW>(Drd, dcol, j, r, cc);
+              else Ld = lap_lds<PW>(Drd, dpv[i], r, cc);
+              float dqn = fmaf(k1[i], dq[i], al[i] * Ld);
+              dqn = fmaf(dal[i], L, dqn);
+              dqn = fmaf(-dk[i], q[i], dqn);
+              dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
+              const float dp1 = dpv[i] + dqn;
+              Dwr[r + 2][cc + 2] = dp1;
+              dpv[i] = dp1;
+              dq[i] = dqn;
+            }
+            const float p1 = pv[i] + qn;
+            Pwr[r + 2][cc + 2] = p1;
+            pv[i] = p1;
+            q[i] = qn;
           }
-          pn[i] = ok ? P[r + 2][cc + 2] + qn : pn[i];
-          q[i] = ok ? qn : q[i];
         }
       }
-      __syncthreads();
+      if (a.write_seis && live && rcvmask) {  // receiver samples of the interior
 #pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-        if (r < lo || r >= RZ - lo) continue;
+        for (int j = 0; j < RPW; ++j) {
+          if (!((rcvmask >> j) & 1u)) continue;
 #pragma unroll
-        for (int c = 0; c < CPR; ++c) {
-          const int i = k * CPR + c;
-          const int cc = lane + 64 * c;
-          if (cc < lo || cc >= RX - lo) continue;
-          P[r + 2][cc + 2] = pn[i];
-          if constexpr (TAN) D[r + 2][cc + 2] = dpn[i];
-          if (!((intmask >> i) & 1u)) continue;
-          if (a.write_seis && live && ((rcvmask >> k) & 1u) && rcol[c] >= 0) {
+          for (int c = 0; c < CPR; ++c) {
+            const int i = j * CPR + c;
+            if (!((intmask >> i) & 1u) || rcol[c] < 0) continue;
             const int64_t si = w.seis_off + (int64_t)it * m.n_rcv + rcol[c];
-            a.seis[si] = pn[i];
-            if constexpr (TAN) a.dseis[si] = dpn[i];
+            a.seis[si] = pv[i];
+            if constexpr (TAN) a.dseis[si] = dpv[i];
           }
         }
       }
       __syncthreads();
+    };
+    for (int s = 0; s < a.K; s += 2) {
+      step(std::integral_constant<int, 0>(), s);
+      if (s + 1 < a.K) step(std::integral_constant<int, 1>(), s + 1);
     }
     // state (p, q[, dp, dq]) at n0+K on the interior
     float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
@@ -350,21 +412,20 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
     float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
     float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
+    for (int j = 0; j < RPW; ++j) {
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
-        const int i = k * CPR + c;
+        const int i = j * CPR + c;
         if (!((intmask >> i) & 1u)) continue;
-        const int g = gint0 + r * NX + lane + 64 * c;
+        const int g = gint0 + own_row<STRIP, NW, RPW>(wv, j) * NX + lane + 64 * c;
         qout[g] = q[i];
-        pout[g] = pn[i];
+        pout[g] = pv[i];
         if (ckp) {
-          ckp[g] = pn[i];
+          ckp[g] = pv[i];
           ckq[g] = q[i];
         }
         if constexpr (TAN) {
-          a.arena[w.off[R_DP] + a.dp_out_slot * Pn + g] = dpn[i];
+          a.arena[w.off[R_DP] + a.dp_out_slot * Pn + g] = dpv[i];
           a.arena[w.off[R_DQ] + a.dq_out_slot * Pn + g] = dq[i];
         }
       }
@@ -395,19 +456,20 @@ struct BornArgs {
 // wavefield taken from the saved history: (Lf(p[n]), q[n]) of every band cell
 // is read from HBM instead of advancing p, which leaves one stencil per step.
 // The next step's pairs are requested as soon as the current step's last use
-// of the registers they land in is issued, so the commit phase and barriers
-// cover their latency.  The
-// fused multiply-add sequence is k_fwd<TAN>'s, so dseis and (dp, dq) are
-// bit-identical to the fused forward + tangent pass.
-template <int RX, int RZ, int NT>
+// of the registers they land in is issued, so the barrier covers part of
+// their latency.  Same window layout and double-buffering as k_fwd, and its
+// fused multiply-add sequence, so dseis and (dp, dq) are bit-identical to the
+// fused forward + tangent pass (whatever the row ownership of either).
+template <int RX, int RZ, int NT, bool STRIP>
 __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
   static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
   constexpr int PW = RX + 4, PH = RZ + 4;
-  __shared__ float D[PH][PW];
+  __shared__ float D0[PH][PW], D1[PH][PW];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r0 = wv * RPW;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;
   for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
@@ -427,7 +489,7 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
     const float amp_d = a.wparam[te.w].amp_d;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;
-    float dq[NC], al[NC], dal[NC], jj[NC], dpn[NC];
+    float dpv[NC], dq[NC], al[NC], dal[NC], jj[NC];
     float2 hc[NC];
     int gw[NC];
     if (threadIdx.x < KMAX) {
@@ -440,99 +502,100 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
     const float2* h0 = hist + a.n0 * Pn;
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
+    for (int j = 0; j < RPW; ++j) {
+      const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
-      if (gz == m.rcv_row) rcvmask |= 1u << k;
+      if (gz == m.rcv_row) rcvmask |= 1u << j;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
-        const int i = k * CPR + c;
+        const int i = j * CPR + c;
         const int cc = lane + 64 * c;
         const int gx = wrapi(ox + cc, NX);
         const int g = gz * NX + gx;
         gw[i] = g;
-        D[r + 2][cc + 2] = dpin[g];
+        dpv[i] = dpin[g];
+        D0[r + 2][cc + 2] = dpv[i];
         dq[i] = dqin[g];
         al[i] = alpha[g];
         dal[i] = dalpha[g];
-        const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
-        jj[i] = (float)(j * j);
+        const int jd = damp_j(gz, gx, m.nz, m.nx, m.nbc);
+        jj[i] = (float)(jd * jd);
         if (g == w.src_cell) srcmask |= 1u << i;
         if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
           intmask |= 1u << i;
-        dpn[i] = 0.f;
-        hc[i] = (r >= 2 && r < RZ - 2 && cc >= 2 && cc < RX - 2) ? h0[g] : make_float2(0.f, 0.f);
+        hc[i] = h0[g];
       }
     }
     __syncthreads();
-    for (int s = 0; s < a.K; ++s) {
+    // Step s reads the window buffer s&1 and writes the other; the buffer
+    // is a compile-time choice so that reads and writes never alias.
+    auto step = [&](auto cb, const int s) {
+      constexpr int CUR = decltype(cb)::value;
+      float (*const Drd)[PW] = CUR ? D1 : D0;
+      float (*const Dwr)[PW] = CUR ? D0 : D1;
       const int it = a.n0 + s - 1;
       const bool live = it >= 0 && it < a.nt;
       const float srcd = amp_d * SW[s];
       const int lo = 2 * (s + 1);
-#pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-        if (r < lo || r >= RZ - lo) continue;  // wave-uniform
+      if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
-          const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          const bool ok = cc >= lo && cc < RX - lo;
-          const float Ld = lap_full<PW>(D, r + 2, cc + 2);
-          const float k1 = 1.0f - kscale * jj[i];
-          const float dk = dkscale * jj[i];
-          float dqn = fmaf(k1, dq[i], al[i] * Ld);
-          dqn = fmaf(dal[i], hc[i].x, dqn);
-          dqn = fmaf(-dk, hc[i].y, dqn);
-          if ((srcmask >> i) & 1u) dqn = dqn + srcd;
-          dpn[i] = ok ? D[r + 2][cc + 2] + dqn : dpn[i];
-          dq[i] = ok ? dqn : dq[i];
-        }
-      }
-      if (s + 1 < a.K) {  // pairs of the next step's band
-        const float2* hx = hist + (int64_t)(a.n0 + s + 1) * Pn;
-        const int l1 = lo + 2;
+          float dcol[STRIP ? RPW + 4 : 1];
+          if constexpr (STRIP) load_col<PW, RPW, CPR>(Drd, dpv, c, r0, cc, dcol);
 #pragma unroll
-        for (int k = 0; k < RPW; ++k) {
-          const int r = wv + NW * k;
-          if (r < l1 || r >= RZ - l1) continue;
-#pragma unroll
-          for (int c = 0; c < CPR; ++c) {
-            const int cc = lane + 64 * c;
-            if (cc >= l1 && cc < RX - l1) hc[k * CPR + c] = hx[gw[k * CPR + c]];
+          for (int j = 0; j < RPW; ++j) {
+            const int r = own_row<STRIP, NW, RPW>(wv, j);
+            const int i = j * CPR + c;
+            float Ld;
+            if constexpr (STRIP) Ld = lap_col<PW>(Drd, dcol, j, r, cc);
+            else Ld = lap_lds<PW>(Drd, dpv[i], r, cc);
+            const float k1 = 1.0f - kscale * jj[i];
+            const float dk = dkscale * jj[i];
+            float dqn = fmaf(k1, dq[i], al[i] * Ld);
+            dqn = fmaf(dal[i], hc[i].x, dqn);
+            dqn = fmaf(-dk, hc[i].y, dqn);
+            dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
+            const float dp1 = dpv[i] + dqn;
+            Dwr[r + 2][cc + 2] = dp1;
+            dpv[i] = dp1;
+            dq[i] = dqn;
           }
         }
       }
-      __syncthreads();
+      if (live && rcvmask) {  // receiver samples of the interior
 #pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-        if (r < lo || r >= RZ - lo) continue;
+        for (int j = 0; j < RPW; ++j) {
+          if (!((rcvmask >> j) & 1u)) continue;
 #pragma unroll
-        for (int c = 0; c < CPR; ++c) {
-          const int i = k * CPR + c;
-          const int cc = lane + 64 * c;
-          if (cc < lo || cc >= RX - lo) continue;
-          D[r + 2][cc + 2] = dpn[i];
-          if (!((intmask >> i) & 1u)) continue;
-          if (live && ((rcvmask >> k) & 1u) && rcol[c] >= 0)
-            a.dseis[w.seis_off + (int64_t)it * m.n_rcv + rcol[c]] = dpn[i];
+          for (int c = 0; c < CPR; ++c) {
+            const int i = j * CPR + c;
+            if (!((intmask >> i) & 1u) || rcol[c] < 0) continue;
+            a.dseis[w.seis_off + (int64_t)it * m.n_rcv + rcol[c]] = dpv[i];
+          }
         }
       }
+      if (s + 1 < a.K) {  // pairs of the next step (every owned cell, back to back)
+        const float2* hx = hist + (int64_t)(a.n0 + s + 1) * Pn;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) hc[i] = hx[gw[i]];
+      }
       __syncthreads();
+    };
+    for (int s = 0; s < a.K; s += 2) {
+      step(std::integral_constant<int, 0>(), s);
+      if (s + 1 < a.K) step(std::integral_constant<int, 1>(), s + 1);
     }
     float* dpout = a.arena + w.off[R_DP] + a.dp_out_slot * Pn;
     float* dqout = a.arena + w.off[R_DQ] + a.dq_out_slot * Pn;
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
+    for (int j = 0; j < RPW; ++j) {
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
-        const int i = k * CPR + c;
+        const int i = j * CPR + c;
         if (!((intmask >> i) & 1u)) continue;
-        const int g = gint0 + r * NX + lane + 64 * c;
-        dpout[g] = dpn[i];
+        const int g = gint0 + own_row<STRIP, NW, RPW>(wv, j) * NX + lane + 64 * c;
+        dpout[g] = dpv[i];
         dqout[g] = dq[i];
       }
     }
@@ -563,21 +626,23 @@ struct AdjArgs {
 // tile, with the imaging accumulators (alpha / kappa adjoints) and the
 // source-wavelet correlation (:314) fused on the tile interior.  lam, mu,
 // alpha, (1-kappa) and the imaging accumulators of owned cells stay in
-// registers; alpha*lam is staged in LDS for the stencil.  The forward enters
-// only through the saved pairs (Lf(p[m-1]), q[m-1]) of interior cells; the
-// next step's pairs are requested right after the current step's last use.  The accumulators see the same fused
-// multiply-add sequence as a per-step read-modify-write, so results do not
-// depend on K.
-template <int RX, int RZ, int NT>
+// registers; alpha*lam is staged in a double-buffered LDS window for the
+// stencil (row ownership as k_fwd, one barrier per step).  The forward
+// enters only through the saved pairs (Lf(p[m-1]), q[m-1]) of interior cells;
+// the next step's pairs are requested right after the current step's last
+// use.  The accumulators see the same fused multiply-add sequence as a
+// per-step read-modify-write, so results do not depend on K.
+template <int RX, int RZ, int NT, bool STRIP>
 __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
   static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
-  constexpr int PW = RX + 4, PH = RZ + 4;     // alpha*lam window with a 2-cell apron
-  __shared__ float A[PH][PW];
+  constexpr int PW = RX + 4, PH = RZ + 4;     // alpha*lam windows with a 2-cell apron
+  __shared__ float A0[PH][PW], A1[PH][PW];
   __shared__ float INJ[KMAX][RX];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r0 = wv * RPW;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
   for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
@@ -596,7 +661,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     const float kscale = a.params[w.model].kscale;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;
-    float lam[NC], mu[NC], al[NC], k1[NC], ln[NC], ga[NC], gk[NC];
+    float lam[NC], mu[NC], al[NC], k1[NC], ga[NC], gk[NC];
     float2 hc[NC];
     unsigned srcmask = 0, rcvmask = 0, intmask = 0;
     int rcol[CPR];
@@ -619,23 +684,23 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     }
     const float2* h0 = hist + (int64_t)(a.M0 - 2 - a.h_base) * Pn;  // pairs of n = M0-2
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
+    for (int j = 0; j < RPW; ++j) {
+      const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
-      if (gz == m.rcv_row) rcvmask |= 1u << k;
+      if (gz == m.rcv_row) rcvmask |= 1u << j;
       const int zz = oz + r;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
-        const int i = k * CPR + c;
+        const int i = j * CPR + c;
         const int cc = lane + 64 * c;
         const int gx = wrapi(ox + cc, NX);
         const int g = gz * NX + gx;
         lam[i] = lin[g];
         mu[i] = min_[g];
         al[i] = alpha[g];
-        A[r + 2][cc + 2] = al[i] * lam[i];
-        const int j = damp_j(gz, gx, m.nz, m.nx, m.nbc);
-        k1[i] = 1.0f - kscale * (float)(j * j);
+        A0[r + 2][cc + 2] = al[i] * lam[i];
+        const int jd = damp_j(gz, gx, m.nz, m.nx, m.nbc);
+        k1[i] = 1.0f - kscale * (float)(jd * jd);
         if (g == w.src_cell) srcmask |= 1u << i;
         const int xx = ox + cc;
         const bool inner = r >= H && r < RZ - H && zz < NZ && cc >= H && cc < RX - H && xx < NX;
@@ -649,75 +714,93 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
           gk[i] = GK[gi];
           hc[i] = h0[gi];
         }
-        ln[i] = 0.f;
       }
     }
     double ssum = 0.0;
     const bool src_here = (srcmask & intmask) != 0;
     if (src_here) ssum = a.wparam[te.w].ssum;
     __syncthreads();
-    for (int s = 0; s < a.K; ++s) {
+    // Step s reads the window buffer s&1 and writes the other; the buffer
+    // is a compile-time choice so that reads and writes never alias.
+    auto step = [&](auto cb, const int s) {
+      constexpr int CUR = decltype(cb)::value;
+      float (*const Ard)[PW] = CUR ? A1 : A0;
+      float (*const Awr)[PW] = CUR ? A0 : A1;
       const int mm = a.M0 - 1 - s;
       const int lo = 2 * (s + 1);
-#pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-        const bool rok = r >= lo && r < RZ - lo;
-        if (!rok) continue;  // wave-uniform
-        const bool rrow = (rcvmask >> k) & 1u;
+      const float sw = SW[s];
+      if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
-          const int i = k * CPR + c;
           const int cc = lane + 64 * c;
-          const bool ok = rok && cc >= lo && cc < RX - lo;
-          const float La = lap_full<PW>(A, r + 2, cc + 2);
-          float mun = fmaf(k1[i], mu[i], La);
-          if (rrow && rcol[c] >= 0) mun = mun + INJ[s][cc];
-          ln[i] = ok ? lam[i] + mun : ln[i];
-          mu[i] = ok ? mun : mu[i];
+          float acol[STRIP ? RPW + 4 : 1];
+          if constexpr (STRIP) {
+            acol[0] = Ard[r0][cc + 2];
+            acol[1] = Ard[r0 + 1][cc + 2];
+#pragma unroll
+            for (int j = 0; j < RPW; ++j) acol[j + 2] = al[j * CPR + c] * lam[j * CPR + c];
+            acol[RPW + 2] = Ard[r0 + RPW + 2][cc + 2];
+            acol[RPW + 3] = Ard[r0 + RPW + 3][cc + 2];
+          }
+          const float inj = INJ[s][cc];
+          const bool rc = rcol[c] >= 0;
+#pragma unroll
+          for (int j = 0; j < RPW; ++j) {
+            const int r = own_row<STRIP, NW, RPW>(wv, j);
+            const int i = j * CPR + c;
+            float La;
+            if constexpr (STRIP) La = lap_col<PW>(Ard, acol, j, r, cc);
+            else La = lap_lds<PW>(Ard, al[i] * lam[i], r, cc);
+            float mun = fmaf(k1[i], mu[i], La);
+            mun = (((rcvmask >> j) & 1u) && rc) ? mun + inj : mun;
+            const float ln = lam[i] + mun;
+            Awr[r + 2][cc + 2] = al[i] * ln;
+            lam[i] = ln;
+            mu[i] = mun;
+          }
         }
       }
-      __syncthreads();
-      const float sw = SW[s];
+      // imaging on the interior (always inside the band): lam now holds lam[m]
+      // GA += lam[m] Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
 #pragma unroll
-      for (int k = 0; k < RPW; ++k) {
-        const int r = wv + NW * k;
-        if (r < lo || r >= RZ - lo) continue;
+      for (int i = 0; i < NC; ++i) {
+        const bool inner = (intmask >> i) & 1u;
+        const float gan = fmaf(lam[i], hc[i].x, ga[i]);
+        const float gkn = fmaf(lam[i], -hc[i].y, gk[i]);
+        ga[i] = inner ? gan : ga[i];
+        gk[i] = inner ? gkn : gk[i];
+      }
+      if (src_here) {  // only the thread owning the source cell
 #pragma unroll
-        for (int c = 0; c < CPR; ++c) {
-          const int i = k * CPR + c;
-          const int cc = lane + 64 * c;
-          if (cc < lo || cc >= RX - lo) continue;
-          A[r + 2][cc + 2] = al[i] * ln[i];
-          lam[i] = ln[i];
-          if (!((intmask >> i) & 1u)) continue;
-          // GA += lam[m] Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
-          ga[i] = fmaf(ln[i], hc[i].x, ga[i]);
-          gk[i] = fmaf(ln[i], -hc[i].y, gk[i]);
-          if ((srcmask >> i) & 1u) ssum += (double)ln[i] * (double)sw;
-        }
+        for (int i = 0; i < NC; ++i)
+          if ((srcmask & intmask) >> i & 1u) ssum += (double)lam[i] * (double)sw;
       }
       if (s + 1 < a.K) {  // pairs of n = mm - 2 for the next step
+        // issued unconditionally (cells outside the interior all read one
+        // dummy address) so that the loads go out back to back
         const float2* hx = hist + (int64_t)(mm - 2 - a.h_base) * Pn;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-          const int r = wv + NW * (i / CPR);
+          const int r = own_row<STRIP, NW, RPW>(wv, i / CPR);
           const int cc = lane + 64 * (i % CPR);
-          if ((intmask >> i) & 1u) hc[i] = hx[gint0 + r * NX + cc];
+          hc[i] = hx[((intmask >> i) & 1u) ? gint0 + r * NX + cc : 0];
         }
       }
       __syncthreads();
+    };
+    for (int s = 0; s < a.K; s += 2) {
+      step(std::integral_constant<int, 0>(), s);
+      if (s + 1 < a.K) step(std::integral_constant<int, 1>(), s + 1);
     }
     float* lout = a.arena + w.off[R_LAM] + a.lam_out_slot * Pn;
     float* mout = a.arena + w.off[R_MU] + a.mu_out_slot * Pn;
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int r = wv + NW * k;
+    for (int j = 0; j < RPW; ++j) {
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
-        const int i = k * CPR + c;
+        const int i = j * CPR + c;
         if (!((intmask >> i) & 1u)) continue;
-        const int g = gint0 + r * NX + lane + 64 * c;
+        const int g = gint0 + own_row<STRIP, NW, RPW>(wv, j) * NX + lane + 64 * c;
         lout[g] = lam[i];
         mout[g] = mu[i];
         GA[g] = ga[i];
@@ -942,10 +1025,12 @@ __global__ __launch_bounds__(NTHR) void k_copy_active(const ModelDev* models, co
 
 int64_t align64(int64_t n) { return (n + 63) & ~int64_t(63); }
 
-// Kernel launch configuration: LDS window (rx x rz cells, halo included) and
-// time steps per launch k.  The interior of a window is (rz-4k) x (rx-4k).
+// Kernel launch configuration: LDS window (rx x rz cells, halo included),
+// time steps per launch k and row ownership (strips: 1 = consecutive rows per
+// wavefront, 0 = round-robin rows).  The interior of a window is
+// (rz-4k) x (rx-4k).
 struct KCfg {
-  int rx, rz, k;
+  int rx, rz, k, strips;
 };
 
 // Instantiated windows per kernel kind: rx x rz cells on `nt` threads, 8
@@ -969,10 +1054,10 @@ int shape_threads(int kind, int rx, int rz) {
 
 bool cfg_ok(int kind, const KCfg& c) {
   return shape_threads(kind, c.rx, c.rz) && c.k >= 1 && c.k <= 8 && c.rz - 4 * c.k >= 4 &&
-         c.rx - 4 * c.k >= 8;
+         c.rx - 4 * c.k >= 8 && (c.strips == 0 || c.strips == 1);
 }
 
-const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3}, {64, 32, 3}, {64, 32, 3}, {64, 32, 2}};
+const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3, 1}, {64, 32, 3, 0}, {64, 32, 3, 0}, {64, 32, 2, 1}};
 
 struct Layout {
   std::vector<ModelDev> models;
@@ -1147,11 +1232,20 @@ int dev_alloc(T** ptr, int64_t count) {
 #define SEIS_ADJ_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
 #define SEIS_BORN_SHAPES(X) SEIS_FWD_SHAPES(X)
 
+// Instantiation of a step kernel for config c (row ownership, and for the
+// forward whether it stores the history pairs).
+template <int RX, int RZ, int NT, bool TAN, bool HIST>
+const void* fwd_ptr(const KCfg& c) {
+  return c.strips ? (const void*)k_fwd<RX, RZ, NT, TAN, true, HIST> : (const void*)k_fwd<RX, RZ, NT, TAN, false, HIST>;
+}
+
 const void* kernel_ptr(int kind, const KCfg& c) {
-#define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_fwd<RX, RZ, NT, false>;
-#define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_fwd<RX, RZ, NT, true>;
-#define PTR_A(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_adj<RX, RZ, NT>;
-#define PTR_B(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return (const void*)k_born<RX, RZ, NT>;
+#define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
+#define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
+#define PTR_A(RX, RZ, NT) \
+  if (c.rx == RX && c.rz == RZ) return c.strips ? (const void*)k_adj<RX, RZ, NT, true> : (const void*)k_adj<RX, RZ, NT, false>;
+#define PTR_B(RX, RZ, NT) \
+  if (c.rx == RX && c.rz == RZ) return c.strips ? (const void*)k_born<RX, RZ, NT, true> : (const void*)k_born<RX, RZ, NT, false>;
   if (kind == SEIS_KIND_FORWARD) { SEIS_FWD_SHAPES(PTR_F) }
   else if (kind == SEIS_KIND_TANGENT) { SEIS_TAN_SHAPES(PTR_T) }
   else if (kind == SEIS_KIND_ADJOINT) { SEIS_ADJ_SHAPES(PTR_A) }
@@ -1163,27 +1257,47 @@ const void* kernel_ptr(int kind, const KCfg& c) {
   return nullptr;
 }
 
+template <int RX, int RZ, int NT, bool TAN>
+void launch_fwd_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
+  const bool hist = a.hist_slot0 >= 0;
+  if (c.strips) {
+    if (hist) hipLaunchKernelGGL((k_fwd<RX, RZ, NT, TAN, true, true>), dim3(grid), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_fwd<RX, RZ, NT, TAN, true, false>), dim3(grid), dim3(NT), 0, s, a);
+  } else {
+    if (hist) hipLaunchKernelGGL((k_fwd<RX, RZ, NT, TAN, false, true>), dim3(grid), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_fwd<RX, RZ, NT, TAN, false, false>), dim3(grid), dim3(NT), 0, s, a);
+  }
+}
+
 void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
-#define LF(RX, RZ, NT) \
-  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_fwd<RX, RZ, NT, false>), dim3(grid), dim3(NT), 0, s, a); return; }
-#define LT(RX, RZ, NT) \
-  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_fwd<RX, RZ, NT, true>), dim3(grid), dim3(NT), 0, s, a); return; }
+#define LF(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_fwd_t<RX, RZ, NT, false>(c, grid, s, a); return; }
+#define LT(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_fwd_t<RX, RZ, NT, true>(c, grid, s, a); return; }
   if (tan) { SEIS_TAN_SHAPES(LT) }
   else { SEIS_FWD_SHAPES(LF) }
 #undef LF
 #undef LT
 }
 
+template <int RX, int RZ, int NT>
+void launch_adj_t(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
+  if (c.strips) hipLaunchKernelGGL((k_adj<RX, RZ, NT, true>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_adj<RX, RZ, NT, false>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 void launch_adj(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
-#define LA(RX, RZ, NT) \
-  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_adj<RX, RZ, NT>), dim3(grid), dim3(NT), 0, s, a); return; }
+#define LA(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_adj_t<RX, RZ, NT>(c, grid, s, a); return; }
   SEIS_ADJ_SHAPES(LA)
 #undef LA
 }
 
+template <int RX, int RZ, int NT>
+void launch_born_t(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
+  if (c.strips) hipLaunchKernelGGL((k_born<RX, RZ, NT, true>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_born<RX, RZ, NT, false>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 void launch_born(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
-#define LB(RX, RZ, NT) \
-  if (c.rx == RX && c.rz == RZ) { hipLaunchKernelGGL((k_born<RX, RZ, NT>), dim3(grid), dim3(NT), 0, s, a); return; }
+#define LB(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_born_t<RX, RZ, NT>(c, grid, s, a); return; }
   SEIS_BORN_SHAPES(LB)
 #undef LB
 }
@@ -1548,23 +1662,26 @@ int seis_plan_layout(const seis_plan_t* plan, int64_t* vec_off, int64_t* seis_of
   return SEIS_OK;
 }
 
-int seis_plan_set_config(seis_plan_t* p, int32_t kind, int32_t rx, int32_t rz, int32_t k) {
+int seis_plan_set_config(seis_plan_t* p, int32_t kind, int32_t rx, int32_t rz, int32_t k, int32_t strips) {
   if (!p || kind < 0 || kind >= SEIS_KIND_COUNT) return fail(SEIS_EINVAL, "bad plan or kind");
-  const KCfg c{rx, rz, k};
+  const KCfg c{rx, rz, k, strips};
   if (!cfg_ok(kind, c))
     return fail(SEIS_EINVAL, "unsupported config: window " + std::to_string(rx) + "x" + std::to_string(rz) +
-                                 ", k=" + std::to_string(k));
+                                 ", k=" + std::to_string(k) + ", strips=" + std::to_string(strips));
   HIP_TRY(hipDeviceSynchronize());
   drop_graphs(p);
   p->cfg[kind] = c;
   return setup_kind(p, kind);
 }
 
-int seis_plan_get_config(const seis_plan_t* p, int32_t kind, int32_t* rx, int32_t* rz, int32_t* k) {
-  if (!p || kind < 0 || kind >= SEIS_KIND_COUNT || !rx || !rz || !k) return fail(SEIS_EINVAL, "bad argument");
+int seis_plan_get_config(const seis_plan_t* p, int32_t kind, int32_t* rx, int32_t* rz, int32_t* k,
+                         int32_t* strips) {
+  if (!p || kind < 0 || kind >= SEIS_KIND_COUNT || !rx || !rz || !k || !strips)
+    return fail(SEIS_EINVAL, "bad argument");
   *rx = p->cfg[kind].rx;
   *rz = p->cfg[kind].rz;
   *k = p->cfg[kind].k;
+  *strips = p->cfg[kind].strips;
   return SEIS_OK;
 }
 

@@ -1,7 +1,7 @@
 """Run each propagator pass once on a batch plan, for counter collection.
 
 Usage (under rocprofv3):
-    python -m seismic_amd.kernel_probe [--batch 16] [--kind forward|tangent|adjoint|all]
+    python -m seismic_amd.kernel_probe [--batch 16] [--kind forward|tangent|born|adjoint|all]
 
 Launches: one warm call (graph capture), then the measured call of each
 requested pass, then a calibration copy of exactly 1 GiB (torch copy_) whose
@@ -36,9 +36,10 @@ def main():
     vec = torch.tensor(np.concatenate([synth.to_vec(synth.curve_fault_model(70, 70, i)) for i in range(B)]),
                        dtype=torch.float32, device="cuda")
     dd = P.forward(vec).clone()
-    kinds = ["forward", "tangent", "adjoint"] if args.kind == "all" else [args.kind]
+    kinds = ["forward", "tangent", "born", "adjoint"] if args.kind == "all" else [args.kind]
+    P.gradient(vec, dd)               # saved forward for the Born pass
     run = {"forward": lambda: P.forward(vec), "tangent": lambda: P.tangent(vec, vec),
-           "adjoint": lambda: P.gradient(vec, dd)}
+           "born": lambda: P.tangent_saved(vec, vec, check=False), "adjoint": lambda: P.gradient(vec, dd)}
     for k in kinds:
         run[k]()                      # capture
     torch.cuda.synchronize()
@@ -54,9 +55,9 @@ def main():
     torch.cuda.synchronize()
     del flush
     info = {"batch": B, "cells_per_step": B * g.n_shots * g.padded_cells, "calibration_copy_bytes": 4 * n}
-    for k in ["forward", "tangent", "adjoint"]:
-        rx, rz, kk = P.get_config(k)
-        info[k] = {"window": [rx, rz], "k": kk,
+    for k in ["forward", "tangent", "born", "adjoint"]:
+        rx, rz, kk, st = P.get_config(k)
+        info[k] = {"window": [rx, rz], "k": kk, "strips": st,
                    "launches_per_pass": math.ceil((g.nt + (1 if k != "adjoint" else 0)) / kk)}
     print(json.dumps(info))
 

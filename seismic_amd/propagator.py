@@ -136,15 +136,21 @@ class Propagator:
     KINDS = {"forward": _lib.KIND_FORWARD, "tangent": _lib.KIND_TANGENT, "adjoint": _lib.KIND_ADJOINT,
              "born": _lib.KIND_BORN}
 
-    def set_config(self, kind: str, rx: int, rz: int, k: int):
-        """Launch config of one kernel kind: rx x rz LDS window, k steps per launch."""
-        _lib.check(_lib.lib().seis_plan_set_config(self._h, self.KINDS[kind], rx, rz, k), "seis_plan_set_config")
+    def set_config(self, kind: str, rx: int, rz: int, k: int, strips: int | None = None):
+        """Launch config of one kernel kind: rx x rz LDS window, k steps per
+        launch, row ownership (1 = consecutive rows per wavefront, 0 =
+        round-robin; None keeps the current one)."""
+        if strips is None:
+            strips = self.get_config(kind)[3]
+        _lib.check(_lib.lib().seis_plan_set_config(self._h, self.KINDS[kind], rx, rz, k, int(strips)),
+                   "seis_plan_set_config")
 
     def get_config(self, kind: str):
-        rx, rz, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _lib.check(_lib.lib().seis_plan_get_config(self._h, self.KINDS[kind], ctypes.byref(rx), ctypes.byref(rz),
-                                                   ctypes.byref(k)), "seis_plan_get_config")
-        return rx.value, rz.value, k.value
+        """(rx, rz, k, strips)."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        _lib.check(_lib.lib().seis_plan_get_config(self._h, self.KINDS[kind], *[ctypes.byref(x) for x in v]),
+                   "seis_plan_get_config")
+        return tuple(x.value for x in v)
 
     def set_active(self, mask: Sequence[bool]):
         mask = [bool(a) for a in mask]

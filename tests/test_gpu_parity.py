@@ -96,12 +96,14 @@ def test_checkpoint_matches_full_history(small_geoms):
         assert np.array_equal(gr, outs[0][1])
 
 
-CONFIGS = [(64, 32, 1), (64, 32, 3), (64, 64, 2), (64, 64, 5), (128, 32, 2), (128, 32, 6), (128, 32, 7)]
+CONFIGS = [(64, 32, 1, 1), (64, 32, 3, 0), (64, 32, 3, 1), (64, 64, 2, 0), (64, 64, 5, 1), (128, 32, 2, 1),
+           (128, 32, 6, 0), (128, 32, 7, 1)]
 
 
 def test_tile_configs_bit_identical(small_geoms):
-    """Every window shape and temporal-blocking depth gives bit-identical
-    seismograms, tangents and gradients (checkpointed and full history)."""
+    """Every window shape, temporal-blocking depth and row ownership gives
+    bit-identical seismograms, tangents and gradients (checkpointed and full
+    history)."""
     g, og = small_geoms
     v, vec = _case(g, "grf", 21)
     rng = np.random.default_rng(3)
@@ -131,9 +133,11 @@ def test_tile_configs_bit_identical(small_geoms):
         P.set_config("forward", 64, 32, 8)   # interior would be empty
     with pytest.raises(ValueError):
         P.set_config("adjoint", 128, 64, 2)  # forward-only window
+    with pytest.raises(ValueError):
+        P.set_config("born", 64, 32, 2, 2)   # no such row ownership
 
 
-@pytest.mark.parametrize("cfg", [(128, 64, 3), (128, 64, 8), (256, 32, 2), (256, 32, 5)])
+@pytest.mark.parametrize("cfg", [(128, 64, 3, 1), (128, 64, 8, 0), (256, 32, 2, 0), (256, 32, 5, 1)])
 def test_large_forward_windows_bit_identical(small_geoms, cfg):
     g, og = small_geoms
     _, vec = _case(g, "curved", 22)
@@ -143,7 +147,7 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
     assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref)
 
 
-@pytest.mark.parametrize("cfg", [(64, 32, 1), (128, 64, 4), (256, 32, 3), (64, 64, 8)])
+@pytest.mark.parametrize("cfg", [(64, 32, 1, 1), (128, 64, 4, 0), (256, 32, 3, 1), (64, 64, 8, 0)])
 def test_born_saved_matches_fused(small_geoms, cfg):
     """Tangent and GN product from the saved forward are bit-identical to the
     passes that recompute the forward, and match the oracle."""
