@@ -28,10 +28,11 @@ from ._lib import CAP_ADJOINT, CAP_TANGENT
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_configs.json")
 
 _WINDOWS = {
-    "forward": [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3)],
-    "tangent": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (64, 64, 3), (64, 64, 4)],
-    "adjoint": [(64, 32, 2), (64, 32, 3), (64, 32, 4), (128, 32, 3), (64, 64, 3), (64, 64, 4)],
-    "born": [(64, 32, 2), (64, 32, 3), (128, 32, 3), (64, 64, 3), (128, 64, 3), (128, 64, 4), (256, 32, 3)],
+    "forward": [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3),
+                (128, 96, 6), (128, 96, 8), (128, 128, 6), (128, 128, 8)],
+    "tangent": [(64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
+    "adjoint": [(64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4), (64, 64, 5)],
+    "born": [(64, 32, 3), (128, 32, 3), (64, 64, 3), (64, 64, 4), (128, 64, 3), (128, 64, 4), (256, 32, 3)],
 }
 # (rx, rz, k, strips): every window with both row ownerships
 CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WINDOWS.items()}
@@ -100,7 +101,9 @@ def tune(P, kinds=KINDS, reps: int = 2, verbose: bool = False) -> dict:
     vec = torch.tensor(np.concatenate([synth.to_vec(synth.make_model("curved", g.nz, g.nx, i % 7))
                                        for i, g in enumerate(P.geoms)]), dtype=torch.float32, device="cuda")
     dd = P.forward(vec).clone()
-    run = {"forward": lambda: P.forward(vec), "tangent": lambda: P.tangent(vec, vec),
+    # plans with an adjoint run the saving forward (history pairs or checkpoints)
+    fwd = (lambda: P.forward_save(vec)) if P.caps & CAP_ADJOINT else (lambda: P.forward(vec))
+    run = {"forward": fwd, "tangent": lambda: P.tangent(vec, vec),
            "adjoint": lambda: P.gradient(vec, dd), "born": lambda: P.tangent_saved(vec, vec, check=False)}
     out = {}
     for kind in kinds:

@@ -1033,18 +1033,19 @@ struct KCfg {
   int rx, rz, k, strips;
 };
 
-// Instantiated windows per kernel kind: rx x rz cells on `nt` threads, 8
-// cells per thread.  The fused tangent and the adjoint carry about twice the
-// per-cell state of the forward and Born kernels, which caps them at 512
-// threads.
+// Instantiated windows per kernel kind: rx x rz cells on `nt` threads.  The
+// forward keeps 8 cells per thread (about 100 VGPRs); the fused tangent and the
+// adjoint carry about twice the per-cell state and run 4 cells per thread, so
+// that 5-6 wavefronts per SIMD stay resident; the Born kernel has both.
 struct Shape {
   int rx, rz, nt;
 };
-const Shape kShapes[SEIS_KIND_COUNT][5] = {
-    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}},
-    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}},
-    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {0, 0, 0}, {0, 0, 0}},
-    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}}};
+const Shape kShapes[SEIS_KIND_COUNT][7] = {
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}, {128, 96, 1024},
+     {128, 128, 1024}},
+    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {0, 0, 0}, {0, 0, 0}},
+    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {0, 0, 0}, {0, 0, 0}},
+    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {128, 64, 1024}, {256, 32, 1024}}};
 
 int shape_threads(int kind, int rx, int rz) {
   for (const Shape& sh : kShapes[kind])
@@ -1227,10 +1228,11 @@ int dev_alloc(T** ptr, int64_t count) {
 }
 
 // ---- kernel dispatch over the instantiated window shapes
-#define SEIS_FWD_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512) X(128, 64, 1024) X(256, 32, 1024)
-#define SEIS_TAN_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
-#define SEIS_ADJ_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512)
-#define SEIS_BORN_SHAPES(X) SEIS_FWD_SHAPES(X)
+#define SEIS_FWD_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512) X(128, 64, 1024) X(256, 32, 1024) \
+  X(128, 96, 1024) X(128, 128, 1024)
+#define SEIS_TAN_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024)
+#define SEIS_ADJ_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024)
+#define SEIS_BORN_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024) X(128, 64, 1024) X(256, 32, 1024)
 
 // Instantiation of a step kernel for config c (row ownership, and for the
 // forward whether it stores the history pairs).

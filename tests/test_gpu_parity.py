@@ -137,14 +137,23 @@ def test_tile_configs_bit_identical(small_geoms):
         P.set_config("born", 64, 32, 2, 2)   # no such row ownership
 
 
-@pytest.mark.parametrize("cfg", [(128, 64, 3, 1), (128, 64, 8, 0), (256, 32, 2, 0), (256, 32, 5, 1)])
+@pytest.mark.parametrize("cfg", [(128, 64, 3, 1), (128, 64, 8, 0), (256, 32, 2, 0), (256, 32, 5, 1),
+                                 (128, 96, 6, 1), (128, 128, 8, 0), (128, 128, 5, 1)])
 def test_large_forward_windows_bit_identical(small_geoms, cfg):
+    """Forward-only windows (larger than the 52 x 56 padded grid, so tiles wrap
+    several times) give the same seismograms, and the same gradients when they
+    run the saving forward and the checkpoint recompute."""
     g, og = small_geoms
-    _, vec = _case(g, "curved", 22)
-    P = Propagator([g], ckpt_every=0)
-    ref = _np(P.forward(torch.tensor(vec)))
-    P.set_config("forward", *cfg)
-    assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref)
+    v, vec = _case(g, "curved", 22)
+    target = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 1.01), og)[0])
+    for ckpt in (0, 17):
+        P = Propagator([g], ckpt_every=ckpt)
+        ref = _np(P.forward(torch.tensor(vec)))
+        _, _, gref = P.gradient(torch.tensor(vec), target)
+        P.set_config("forward", *cfg)
+        assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref)
+        _, _, gr = P.gradient(torch.tensor(vec), target)
+        assert np.array_equal(_np(gr), _np(gref)), ckpt
 
 
 @pytest.mark.parametrize("cfg", [(64, 32, 1, 1), (128, 64, 4, 0), (256, 32, 3, 1), (64, 64, 8, 0)])
