@@ -40,7 +40,10 @@ HBM_PEAK = 8.0e12                     # MI355X HBM3E bytes/s
 # writes dp[t+1]; the Born pass reads dp[t-1], dp[t], alpha, kappa, dalpha,
 # dkappa and the saved pair (Lf(p[t]), q[t]) and writes dp[t+1]; the adjoint
 # reads lam[t+1], lam[t+2], alpha, kappa and the pair of p[t] and writes lam[t].
-ALG_BYTES = {"forward": 20, "tangent": 40, "born": 36, "adjoint": 28}
+# The saving forward also writes the pair (Lf(p[t]), q[t]) of every cell.
+ALG_BYTES = {"linesearch_forward": 20, "saving_forward": 28, "tangent": 40, "born": 36, "adjoint": 28}
+KERNEL = {"linesearch_forward": "k_fwd", "saving_forward": "k_fwd (history)", "tangent": "k_fwd (tangent)",
+          "born": "k_born", "adjoint": "k_adj"}
 N_OUTER, N_CG, N_SCALES = 10, 20, 50
 
 
@@ -86,13 +89,21 @@ def run_gn(executor, reqs, n_outer):
     return time.perf_counter() - t0, [res[i] for i in range(len(reqs))]
 
 
-def kernel_roofline(P, B, g, reps=3):
-    """Average launch time of each step kernel on a batch plan, from HIP events
-    around its captured time loop (launch stream = torch's current stream)."""
+def kernel_roofline(ex, B, g, reps=3):
+    """Average launch time of each step kernel as the GN pass runs it: the
+    line-search forward on the executor's fan-out plan (B x 50 models), the
+    saving forward, Born and adjoint passes on the batch plan.  HIP events on
+    torch's current stream, which is the stream the plans launch on."""
     import torch
     from seismic_amd import synth
-    vec = torch.tensor(np.concatenate([synth.to_vec(synth.curve_fault_model(g.nz, g.nx, i)) for i in range(B)]),
-                       dtype=torch.float32, device="cuda")
+    P = ex._grad_plan(list(range(B)))
+    P.set_active([True] * B)
+    P._mask = tuple([True] * B)
+    Pf = max((v for k, v in ex._plans.items() if k[0] == "fB"), key=lambda q: q.n_models)
+    models = [synth.to_vec(synth.curve_fault_model(g.nz, g.nx, i)) for i in range(B)]
+    vec = torch.tensor(np.concatenate(models), dtype=torch.float32, device="cuda")
+    vec_f = torch.tensor(np.concatenate([models[i % B] for i in range(Pf.n_models)]), dtype=torch.float32,
+                         device="cuda")
     dd = P.forward(vec).clone()
     s = torch.cuda.current_stream()
 
@@ -107,21 +118,41 @@ def kernel_roofline(P, B, g, reps=3):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3 / reps
 
-    t_fwd = timed(lambda: P.forward(vec))
+    t_ls = timed(lambda: Pf.forward(vec_f))
+    t_save = timed(lambda: P.forward_save(vec))
     t_tan = timed(lambda: P.tangent(vec, vec))
     _, tok = P.forward_save(vec)
     t_adj = timed(lambda: P.adjoint(vec, dd, tok, on_residual=True))
     t_born = timed(lambda: P.tangent_saved(vec, vec, check=False))
-    cells = B * g.n_shots * g.padded_cells
     out = {}
-    for kind, t_pass, steps in [("forward", t_fwd, g.nt + 1), ("tangent", t_tan, g.nt + 1),
-                                ("born", t_born, g.nt + 1), ("adjoint", t_adj, g.nt)]:
-        k = P.get_config(kind)[2]
+    for name, plan, kind, t_pass, steps in [("linesearch_forward", Pf, "forward", t_ls, g.nt + 1),
+                                            ("saving_forward", P, "forward", t_save, g.nt + 1),
+                                            ("tangent", P, "tangent", t_tan, g.nt + 1),
+                                            ("born", P, "born", t_born, g.nt + 1),
+                                            ("adjoint", P, "adjoint", t_adj, g.nt)]:
+        rx, rz, k, strips = plan.get_config(kind)
         launches = math.ceil(steps / k)
-        out[kind] = {"pass_s": t_pass, "launches": launches, "launch_s": t_pass / launches,
-                     "cell_updates_per_launch": cells * k, "k": k, "window": P.get_config(kind)[:2],
-                     "strips": P.get_config(kind)[3]}
+        cells = plan.n_models * g.n_shots * g.padded_cells
+        out[name] = {"kernel": KERNEL[name], "pass_s": t_pass, "launches": launches, "launch_s": t_pass / launches,
+                     "cell_updates_per_launch": cells * k, "k": k, "window": [rx, rz], "strips": strips,
+                     "models": plan.n_models}
     return out
+
+
+def measured_traffic(name, kd):
+    """HBM bytes per launch of this kernel and config from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary in profiles/, or None when the committed
+    measurement was taken on another config."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tab = json.load(f)
+    e = tab.get(name)
+    if not e or e.get("window") != kd["window"] or e.get("k") != kd["k"] or e.get("strips") != kd["strips"] \
+            or e.get("models") != kd["models"]:
+        return None
+    return e["bytes_per_launch"]
 
 
 def cpu_baseline(g, seconds_hint=25.0):
@@ -269,18 +300,17 @@ def main():
     hist_b0 = list(reqs[0].meta["stats"]["misfit_hist"])
     log(f"[bench] batched: {t_b:.2f} s, {value:.4f} inversions/s")
 
-    # ---- roofline of the step kernels on the batched plan
-    P = ex_b._grad_plan(list(range(B)))
-    P.set_active([True] * B)
-    P._mask = tuple([True] * B)
-    kr = kernel_roofline(P, B, g)
-    # passes per GN iteration: 1 saving forward + 50 line-search forwards,
-    # 20 Born passes (J^T J products on the saved forward), 21 adjoints
-    per_outer = {"forward": 1 + N_SCALES, "tangent": 0, "born": N_CG, "adjoint": 1 + N_CG}
+    # ---- roofline of the step kernels as the GN pass runs them
+    kr = kernel_roofline(ex_b, B, g)
+    # passes per GN iteration: the 50-point line search (one pass of the
+    # fan-out plan), 1 saving forward, 20 Born passes (J^T J products on the
+    # saved forward) and 21 adjoints
+    per_outer = {"linesearch_forward": 1, "saving_forward": 1, "tangent": 0, "born": N_CG, "adjoint": 1 + N_CG}
     share = {k: per_outer[k] * kr[k]["pass_s"] for k in kr}
     dom = max(share, key=share.get)
     kd = kr[dom]
     achieved = ALG_BYTES[dom] * kd["cell_updates_per_launch"] / kd["launch_s"]
+    traffic = measured_traffic(dom, kd)
     log(f"[bench] kernels: " + ", ".join(f"{k} {v['launch_s']*1e6:.1f} us/launch" for k, v in kr.items()))
 
     # ---- plain path: same kernels, one model at a time
@@ -325,16 +355,17 @@ def main():
                    "model0_plain": preq[0].meta["stats"]["misfit_hist"],
                    "equal": hist_b0[:args.plain_steps] == preq[0].meta["stats"]["misfit_hist"]},
         "serving": serving,
-        "roofline": {"bound": "hbm", "kernel": {"forward": "k_fwd", "tangent": "k_fwd(tangent)",
-                                                "born": "k_born", "adjoint": "k_adj"}[dom],
+        "roofline": {"bound": "hbm", "kernel": kd["kernel"], "pass": dom,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "launch_us": kd["launch_s"] * 1e6, "alg_bytes_per_cell_update": ALG_BYTES[dom],
                      "cell_updates_per_launch": kd["cell_updates_per_launch"], "k": kd["k"],
-                     "window": kd["window"], "strips": kd["strips"]},
-        "kernels": {k: {"launch_us": v["launch_s"] * 1e6, "k": v["k"],
+                     "window": kd["window"], "strips": kd["strips"], "models": kd["models"]},
+        "kernels": {k: {"kernel": v["kernel"], "launch_us": v["launch_s"] * 1e6, "k": v["k"], "window": v["window"],
+                        "strips": v["strips"], "models": v["models"],
                         "gcell_updates_per_s": v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
-                        "alg_GBps": ALG_BYTES[k] * v["cell_updates_per_launch"] / v["launch_s"] / 1e9}
+                        "alg_GBps": ALG_BYTES[k] * v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
+                        "share_of_gn_pass_s": share[k]}
                     for k, v in kr.items()},
         "cpu_baseline": cpu,
     }
