@@ -196,20 +196,27 @@ def _cpu_fev(a):
     return fd2d.vel_to_seis(vec, og, vec_adjoint=tgt, adjoint_on_residual=True, shots=[s])[2]
 
 
-def serving_run(n_req, rate, device, seed=100):
+def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep_requests=36):
     """Served LBFGS inversions (30 iterations, memory 10, smoothness prior) of
-    mixed grid sizes arriving as a Poisson process."""
+    mixed grid sizes arriving as a Poisson process: `n_req` requests at
+    `rate` on as many slots, then a rate sweep with `sweep_requests` requests
+    on the same slots (requests queue for a free slot of their size)."""
     import torch
     from seismic_amd import invert, priors, serve, synth
     from seismic_amd.geometry import Geometry
     from seismic_amd.propagator import Propagator
     sizes = [(70, 70), (100, 100), (140, 140)]
-    mix = [0] * (n_req // 2) + [1] * (n_req // 3) + [2] * (n_req - n_req // 2 - n_req // 3)
-    geoms = {s: Geometry(nz=s[0], nx=s[1], nt=1000) for s in sizes}
-    trues = [synth.curve_fault_model(*sizes[si], seed + i) for i, si in enumerate(mix)]
-    targets = [None] * n_req
+
+    def size_mix(n):
+        return [0] * (n // 2) + [1] * (n // 3) + [2] * (n - n // 2 - n // 3)
+
+    n_max = max(n_req, sweep_requests)
+    mix_all = size_mix(n_max)
+    geoms = {sz: Geometry(nz=sz[0], nx=sz[1], nt=1000) for sz in sizes}
+    trues = [synth.curve_fault_model(*sizes[si], seed + i) for i, si in enumerate(mix_all)]
+    targets = [None] * n_max
     for si, sz in enumerate(sizes):
-        ids = [i for i, k in enumerate(mix) if k == si]
+        ids = [i for i, k in enumerate(mix_all) if k == si]
         if not ids:
             continue
         Pf = Propagator([geoms[sz]] * len(ids), caps=1)
@@ -217,34 +224,47 @@ def serving_run(n_req, rate, device, seed=100):
         for j, i in enumerate(ids):
             targets[i] = d[int(Pf.seis_off[j]):int(Pf.seis_off[j + 1])].clone()
         Pf.close()
+    by_size = {si: [i for i, k in enumerate(mix_all) if k == si] for si in range(3)}
 
-    def make_reqs():
+    def make_reqs(n):
         out = []
-        for i, si in enumerate(mix):
+        taken = {si: 0 for si in range(3)}
+        for rid, si in enumerate(size_mix(n)):
+            i = by_size[si][taken[si]]
+            taken[si] += 1
             g = geoms[sizes[si]]
             start = synth.to_vec(synth.depth_linear_start(g.nz, g.nx), min_vel=1500.0)
             out.append(invert.Request(geom=g, target=targets[i], prior=priors.L2Smoothness(g.nz, g.nx, lam=1e-8),
-                                      start=torch.tensor(start, device=device), rid=i))
+                                      start=torch.tensor(start, device=device), rid=rid))
         return out
 
-    reqs = make_reqs()
-    slots = [geoms[sizes[si]] for si in mix]
+    slots = [geoms[sizes[si]] for si in size_mix(n_req)]
     srv = serve.InversionServer(slots, hbm_budget=200 * 2 ** 30, ckpt_every=50, max_iter=30, history_size=10)
     srv.kw["max_iter"] = 1                      # capture the server's graphs before timing
-    srv.run(make_reqs(), [0.0] * n_req)
+    srv.run(make_reqs(n_req), [0.0] * n_req)
     srv.kw["max_iter"] = 30
-    srv.log.clear()
-    arrivals = serve.poisson_arrivals(n_req, rate, seed)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    recs = srv.run(reqs, arrivals)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lat = serve.percentiles([r.latency for r in recs])
+
+    def timed_run(n, r, sd):
+        srv.log.clear()
+        reqs = make_reqs(n)
+        arrivals = serve.poisson_arrivals(n, r, sd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        recs = srv.run(reqs, arrivals)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return {"requests": n, "rate_req_per_s": r, "inversions_per_s": n / wall, "wall_s": wall,
+                "latency_s": serve.percentiles([x.latency for x in recs]),
+                "queue_wait_s": serve.percentiles([x.admitted - x.arrival for x in recs]),
+                "max_in_flight": max(k for _, k in srv.log) if srv.log else 0,
+                "failed": int(sum(x.failed for x in recs))}, reqs
+
+    main_run, reqs = timed_run(n_req, rate, seed)
+    sweep_runs = [timed_run(sweep_requests, r, seed + 7)[0] for r in sweep]
     # plain path: the same LBFGS inversion, one 70x70 request alone
     g = geoms[(70, 70)]
     ex = invert.Executor([g], ckpt_every=0, max_batch=1)
-    r0 = reqs[mix.index(0)]
+    r0 = reqs[0]
     ex.set_target(0, r0.target)
     r0.meta.clear()
     torch.cuda.synchronize()
@@ -252,11 +272,89 @@ def serving_run(n_req, rate, device, seed=100):
     invert.run_machines({0: invert.lbfgs_machine(r0, max_iter=30, history_size=10)}, ex)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    return {"requests": n_req, "rate_req_per_s": rate, "sizes": [f"{a}x{b}" for a, b in sizes],
-            "inversions_per_s": n_req / wall, "wall_s": wall, "latency_s": lat,
-            "max_in_flight": max(n for _, n in srv.log) if srv.log else 0,
-            "plain_70x70_inversions_per_s": 1.0 / t_plain,
-            "failed": int(sum(r.failed for r in recs))}
+    ex.close()
+    srv.ex.close()
+    out = dict(main_run)
+    out.update({"slots": n_req, "sizes": [f"{a}x{b}" for a, b in sizes],
+                "plain_70x70_inversions_per_s": 1.0 / t_plain, "rate_sweep": sweep_runs})
+    return out
+
+
+def _timed_s(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def mixed_batch_run(device, n_models=512, ckpt=32, seed=300):
+    """Many models of uneven size in one plan: 512 models of 70x70, 100x100 and
+    70(z)x140(x) (flat, curved, faulted, GRF-textured), 5 shots, n_rcv = nx,
+    nt = 1000.  Every launch advances all 2,560 wavefields.  Times one forward
+    pass and one fev (misfit + gradient, checkpoints every `ckpt` steps)."""
+    import torch
+    from seismic_amd import autotune, synth
+    from seismic_amd._lib import CAP_ADJOINT, CAP_FORWARD
+    from seismic_amd.geometry import Geometry
+    from seismic_amd.propagator import Propagator
+    sizes = [(70, 70), (100, 100), (70, 140)]
+    styles = ["flat", "curved", "faulted", "grf"]
+    geoms = {sz: Geometry(nz=sz[0], nx=sz[1], nt=1000) for sz in sizes}
+    gl = [geoms[sizes[i % 3]] for i in range(n_models)]
+    vec = np.concatenate([synth.to_vec(synth.make_model(styles[(i // 3) % 4], g.nz, g.nx, seed + i))
+                          for i, g in enumerate(gl)])
+    cells = sum(g.n_shots * g.padded_cells for g in gl)
+    v = torch.tensor(vec, dtype=torch.float32, device=device)
+    Pf = Propagator(gl, caps=CAP_FORWARD, device=device)
+    autotune.apply(Pf)
+    dd = Pf.forward(v * 1.01).clone()
+    t_fwd = _timed_s(lambda: Pf.forward(v), 2)
+    Pf.close()
+    Pg = Propagator(gl, ckpt_every=ckpt, caps=CAP_FORWARD | CAP_ADJOINT, device=device)
+    autotune.apply(Pg)
+    t_fev = _timed_s(lambda: Pg.gradient(v, dd), 1)
+    gbytes = Pg.device_bytes
+    Pg.close()
+    torch.cuda.empty_cache()
+    steps = 1001
+    return {"models": n_models, "sizes": [f"{a}x{b}" for a, b in sizes], "wavefields": n_models * 5,
+            "cells_per_step": cells, "forward_s": t_fwd,
+            "forward_gcell_updates_per_s": cells * steps / t_fwd / 1e9,
+            "fev_s": t_fev, "fevs_per_s": n_models / t_fev, "fev_ckpt_every": ckpt,
+            "fev_plan_gb": gbytes / 1e9}
+
+
+def autograd_run(device, n_models=64, ckpt=50, reps=3):
+    """torch.autograd through the propagator: 64 70x70 models, checkpoints
+    every 50 steps, loss = data misfit + small-CNN prior; one step = forward,
+    backward (segment recompute + adjoint) and the prior."""
+    import torch
+    from seismic_amd import autotune, synth
+    from seismic_amd.autograd import SmallCNNPrior, propagate
+    from seismic_amd.geometry import Geometry
+    from seismic_amd.propagator import Propagator
+    g = Geometry(nt=1000)
+    vel = np.stack([synth.to_vec(synth.curve_fault_model(70, 70, 500 + i)) for i in range(n_models)])
+    P = Propagator([g] * n_models, ckpt_every=ckpt, device=device)
+    autotune.apply(P)
+    obs = P.forward(torch.tensor(vel * 1.01).reshape(-1)).reshape(n_models, -1).clone()
+    prior = SmallCNNPrior(g.nz, g.nx).to(device)
+    base = torch.tensor(vel, dtype=torch.float32, device=device)
+
+    def step():
+        v = base.clone().requires_grad_(True)
+        loss = ((propagate(v, P) - obs) ** 2).mean(dim=1).sum() + 0.5 * prior(v).sum()
+        loss.backward()
+
+    t = _timed_s(step, reps)
+    gbytes = P.device_bytes
+    P.close()
+    return {"models": n_models, "ckpt_every": ckpt, "step_s": t, "model_gradients_per_s": n_models / t,
+            "plan_gb": gbytes / 1e9}
 
 
 def main():
@@ -271,6 +369,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the 512-model and autograd legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); inversions are independent, "
@@ -322,6 +421,19 @@ def main():
     plain = 1 * (args.plain_steps / N_OUTER) / t_p
     log(f"[bench] plain: {t_p:.2f} s, {plain:.4f} inversions/s")
 
+    ex_b.close()
+    ex_p.close()
+    torch.cuda.empty_cache()
+    extras = {}
+    if not args.no_extras:
+        log("[bench] 512 mixed-size models in one plan")
+        extras["mixed_batch_512"] = mixed_batch_run(dev)
+        log(f"[bench] mixed 512: forward {extras['mixed_batch_512']['forward_s']:.2f} s, "
+            f"fev {extras['mixed_batch_512']['fev_s']:.2f} s")
+        log("[bench] autograd, 64 models")
+        extras["autograd_64"] = autograd_run(dev)
+        log(f"[bench] autograd: {extras['autograd_64']['step_s']:.2f} s per step")
+
     serving = None
     if not args.no_serve:
         log(f"[bench] serving {args.serve_requests} LBFGS requests at {args.serve_rate}/s")
@@ -355,6 +467,7 @@ def main():
                    "model0_plain": preq[0].meta["stats"]["misfit_hist"],
                    "equal": hist_b0[:args.plain_steps] == preq[0].meta["stats"]["misfit_hist"]},
         "serving": serving,
+        "extras": extras,
         "roofline": {"bound": "hbm", "kernel": kd["kernel"], "pass": dom,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,

@@ -358,6 +358,12 @@ class Executor:
         self.targets: list = [None] * self.n
         self._plans: dict = {}
 
+    def close(self):
+        """Free every plan's device memory (the executor can be used again)."""
+        for P in self._plans.values():
+            P.close()
+        self._plans.clear()
+
     # plan cache --------------------------------------------------------
     def _plan(self, key, geoms, caps):
         if key not in self._plans:

@@ -4,7 +4,8 @@ Requests (``invert.Request``) arrive over time.  The server owns one plan
 whose slots cover a fixed mix of grid sizes; a request is admitted into a
 free slot of its size when its HBM footprint (its share of the plan:
 wavefields, checkpoints, coefficient fields, data; plus its LBFGS history)
-fits the budget.  Every loop iteration collects the pending propagator op
+fits the budget.  Waiting requests are admitted first-fit in arrival order,
+so one waiting for a busy size does not hold back the others.  Every loop iteration collects the pending propagator op
 of every in-flight request and runs them as one batched launch per op kind;
 requests that arrive meanwhile join at the next iteration (continuous
 batching, not a queue of whole inversions).  Per-request state (model,
@@ -110,12 +111,16 @@ class InversionServer:
             t = now()
             while queue and queue[0][1] <= t:
                 waiting.append(queue.popleft())
+            # first fit in arrival order: a request that cannot be admitted
+            # (no free slot of its size, or footprint over budget) does not
+            # hold back later requests that can
+            blocked: deque = deque()
             while waiting:
-                req, arr, idx = waiting[0]
+                req, arr, idx = waiting.popleft()
                 slot = self._try_admit(req)
                 if slot is None:
-                    break
-                waiting.popleft()
+                    blocked.append((req, arr, idx))
+                    continue
                 self.ex.set_target(slot, req.target)
                 m = lbfgs_machine(req, **self.kw)
                 out[idx].admitted = now()
@@ -126,6 +131,7 @@ class InversionServer:
                     self._release(slot, req)
                     if on_event:
                         on_event(out[idx])
+            waiting = blocked
             if not active:
                 if queue:
                     time.sleep(max(0.0, queue[0][1] - now()))

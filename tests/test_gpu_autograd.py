@@ -114,3 +114,43 @@ def test_backward_after_intervening_pass_recomputes():
         loss.backward()
         grads.append(v.grad.clone())
     assert torch.equal(grads[0], grads[1])
+
+
+def test_autograd_full_size_batch64_golden():
+    """64 reference-size models (70x70, nbc 120, nt 999, 5 shots) through
+    autograd with checkpoints every 50 steps, loss = data misfit + small-CNN
+    prior.  Model 0 is the golden model with its float32 target: its gradient
+    is (2/N) J^T (F(v) - target) from the committed FP64 fixture plus the
+    prior's float64 autograd gradient (1e-4 relative); other models match
+    their single-model gradients bit for bit."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref70_grad.npz"))
+    g = Geometry()
+    nb = 64
+    vels = [synth.to_vec(synth.curve_fault_model(70, 70, seed=7))]
+    vels += [synth.to_vec(synth.make_model(["flat", "curved", "faulted", "grf"][i % 4], 70, 70, 100 + i))
+             for i in range(1, nb)]
+    vel = np.stack(vels)
+    P = Propagator([g] * nb, ckpt_every=50)
+    o = P.forward(torch.tensor(vel * 1.01).reshape(-1)).reshape(nb, -1).clone()
+    o[0] = torch.tensor(gold["target"], device="cuda")
+    prior = SmallCNNPrior(g.nz, g.nx).cuda().double()
+    lam = 0.5
+    v = torch.tensor(vel, dtype=torch.float32, device="cuda", requires_grad=True)
+    loss = ((propagate(v, P) - o) ** 2).mean(dim=1).sum() + lam * prior(v.double()).sum()
+    loss.backward()
+    got = v.grad.double().cpu().numpy()
+    v64 = torch.tensor(vel[:1], dtype=torch.float64, device="cuda", requires_grad=True)
+    prior(v64).sum().backward()
+    ref0 = 2.0 / g.n_data * gold["grad"] + lam * v64.grad[0].cpu().numpy()
+    r = rel(got[0], ref0)
+    print("batch-64 autograd, model 0 vs golden gradient + prior:", r, flush=True)
+    assert r < 1e-4
+    # batch invariance of the data gradient: models 5 and 63 alone
+    vd = torch.tensor(vel, dtype=torch.float32, device="cuda", requires_grad=True)
+    ((propagate(vd, P) - o) ** 2).mean(dim=1).sum().backward()
+    P1 = Propagator([g], ckpt_every=50)
+    for b in (5, nb - 1):
+        v1 = torch.tensor(vel[b:b + 1], dtype=torch.float32, device="cuda", requires_grad=True)
+        ((propagate(v1, P1) - o[b:b + 1]) ** 2).mean(dim=1).sum().backward()
+        assert torch.equal(v1.grad[0], vd.grad[b])

@@ -121,6 +121,18 @@ def test_continuous_batching_and_results_match_alone():
         assert r.latency >= 0 and r.finished >= r.admitted >= r.arrival - 1e-9
 
 
+def test_first_fit_admission_does_not_block_other_sizes():
+    """A request waiting for a busy 4x4 slot does not hold back a 5x5 request
+    behind it in the queue."""
+    g1, g2 = Geometry(nz=4, nx=4, nbc=2, nt=150), Geometry(nz=5, nx=5, nbc=2, nt=150)
+    srv = _server([g1, g2], budget=10 ** 9, fake=FakeExecutor())
+    reqs = [_req(g1, 0), _req(g1, 1), _req(g2, 2, n=7)]
+    recs = srv.run(reqs, [0.0, 0.0, 0.0])
+    assert all(not r.failed for r in recs)
+    assert recs[2].admitted < recs[0].finished      # admitted while request 0 held the 4x4 slot
+    assert recs[1].admitted >= recs[0].finished     # request 1 waited for that slot
+
+
 def test_nan_request_falls_back_to_start():
     g = Geometry(nz=4, nx=4, nbc=2, nt=150)
     fake = FakeExecutor(nan_rids=[1])
