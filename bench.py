@@ -126,7 +126,7 @@ def kernel_roofline(ex, B, g, reps=3):
     t_born = timed(lambda: P.tangent_saved(vec, vec, check=False))
     out = {}
     for name, plan, kind, t_pass, steps in [("linesearch_forward", Pf, "forward", t_ls, g.nt + 1),
-                                            ("saving_forward", P, "forward", t_save, g.nt + 1),
+                                            ("saving_forward", P, "record", t_save, g.nt + 1),
                                             ("tangent", P, "tangent", t_tan, g.nt + 1),
                                             ("born", P, "born", t_born, g.nt + 1),
                                             ("adjoint", P, "adjoint", t_adj, g.nt)]:

@@ -168,11 +168,13 @@ int seis_gn_product_saved(seis_plan_t* plan, const float* vec, const float* dvec
 
 /* Kernel kinds with a tunable launch configuration. */
 enum {
-  SEIS_KIND_FORWARD = 0, /* forward step kernel (also segment recompute)     */
+  SEIS_KIND_FORWARD = 0, /* forward step kernel (no history pairs)           */
   SEIS_KIND_TANGENT = 1, /* fused forward + tangent step kernel               */
   SEIS_KIND_ADJOINT = 2, /* adjoint step kernel with fused imaging            */
   SEIS_KIND_BORN = 3,    /* tangent step kernel on the saved forward history  */
-  SEIS_KIND_COUNT = 4
+  SEIS_KIND_RECORD = 4,  /* forward that stores the history pairs: saving     */
+                         /* forward of full-history plans, segment recompute */
+  SEIS_KIND_COUNT = 5
 };
 
 /* Launch configuration of one kernel kind: an rx x rz cell window (halo
@@ -181,9 +183,9 @@ enum {
  * strips = 1 gives each wavefront consecutive window rows (vertical stencil
  * neighbours stay in registers), strips = 0 deals rows round-robin over the
  * wavefronts (even work per wavefront as the valid band shrinks).
- * Windows (rx x rz): forward 64x32, 128x32, 64x64, 128x64, 256x32, 128x96,
- * 128x128; Born 64x32, 128x32, 64x64, 128x64, 256x32; tangent and adjoint
- * 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4 rows and
+ * Windows (rx x rz): forward and record 64x32, 128x32, 64x64, 128x64, 256x32,
+ * 128x96, 128x128; Born 64x32, 128x32, 64x64, 128x64, 256x32; tangent and
+ * adjoint 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4 rows and
  * 8 columns.  Results do not depend on the config
  * (bit-identical wavefields).  No reference counterpart: the reference runs
  * one 32x32-thread launch per step (core/seis_forward2.py:191-193).

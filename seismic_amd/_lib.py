@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libseis.so"
 CAP_FORWARD = 1
 CAP_TANGENT = 2
 CAP_ADJOINT = 4
-KIND_FORWARD, KIND_TANGENT, KIND_ADJOINT, KIND_BORN = 0, 1, 2, 3
+KIND_FORWARD, KIND_TANGENT, KIND_ADJOINT, KIND_BORN, KIND_RECORD = 0, 1, 2, 3, 4
 MAX_SHOTS = 16
 
 ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "HIP error",

@@ -1,7 +1,7 @@
 """Autotuner for the step kernels' launch configuration.
 
-Each kernel kind (forward, fused tangent, adjoint, Born on the saved
-forward) runs an rx x rz cell window per workgroup, advances k time steps
+Each kernel kind (forward, forward recording the history, fused tangent,
+adjoint, Born on the saved forward) runs an rx x rz cell window per workgroup, advances k time steps
 per launch and deals window rows to wavefronts either as consecutive strips
 or round-robin (libseis ``seis_plan_set_config``).  The best choice depends on how many cells one
 time step covers (batch size x model size: a single 70x70 model leaves most
@@ -31,12 +31,18 @@ _WINDOWS = {
     "forward": [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3),
                 (128, 96, 6), (128, 96, 8), (128, 128, 6), (128, 128, 8)],
     "tangent": [(64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
-    "adjoint": [(64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4), (64, 64, 5)],
-    "born": [(64, 32, 3), (128, 32, 3), (64, 64, 3), (64, 64, 4), (128, 64, 3), (128, 64, 4), (256, 32, 3)],
+    "adjoint": [(64, 32, 3), (64, 32, 4), (64, 32, 5), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4),
+                (64, 64, 5)],
+    "born": [(64, 32, 3), (128, 32, 3), (64, 64, 3), (64, 64, 4), (128, 64, 3), (128, 64, 4), (128, 64, 5),
+             (256, 32, 3)],
 }
+_WINDOWS["record"] = [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8),
+                      (256, 32, 3), (128, 96, 6), (128, 128, 6)]
 # (rx, rz, k, strips): every window with both row ownerships
 CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WINDOWS.items()}
-KINDS = ("forward", "tangent", "adjoint", "born")
+# tuning order: the adjoint and Born passes run after the forward that
+# records their history
+KINDS = ("forward", "record", "tangent", "adjoint", "born")
 
 
 def _has(P, kind: str) -> bool:
@@ -46,6 +52,8 @@ def _has(P, kind: str) -> bool:
         return bool(P.caps & CAP_ADJOINT)
     if kind == "born":
         return bool(P.caps & CAP_TANGENT) and P.full_history
+    if kind == "record":
+        return bool(P.caps & CAP_ADJOINT)
     return True
 
 
@@ -101,9 +109,14 @@ def tune(P, kinds=KINDS, reps: int = 2, verbose: bool = False) -> dict:
     vec = torch.tensor(np.concatenate([synth.to_vec(synth.make_model("curved", g.nz, g.nx, i % 7))
                                        for i, g in enumerate(P.geoms)]), dtype=torch.float32, device="cuda")
     dd = P.forward(vec).clone()
-    # plans with an adjoint run the saving forward (history pairs or checkpoints)
-    fwd = (lambda: P.forward_save(vec)) if P.caps & CAP_ADJOINT else (lambda: P.forward(vec))
-    run = {"forward": fwd, "tangent": lambda: P.tangent(vec, vec),
+    # the forward kind runs plain forwards and, on checkpointed adjoint plans,
+    # the saving forward (checkpoints only); the record kind runs the saving
+    # forward of full-history plans and the segment recompute inside the
+    # adjoint pass of checkpointed ones
+    ckpt_adj = bool(P.caps & CAP_ADJOINT) and not P.full_history
+    run = {"forward": (lambda: P.forward_save(vec)) if ckpt_adj else (lambda: P.forward(vec)),
+           "record": (lambda: P.gradient(vec, dd)) if ckpt_adj else (lambda: P.forward_save(vec)),
+           "tangent": lambda: P.tangent(vec, vec),
            "adjoint": lambda: P.gradient(vec, dd), "born": lambda: P.tangent_saved(vec, vec, check=False)}
     out = {}
     for kind in kinds:

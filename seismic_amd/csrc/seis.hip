@@ -258,7 +258,8 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
   __shared__ float P0[PH][PW], P1[PH][PW];
   __shared__ float D0[TAN ? PH : 1][TAN ? PW : 1], D1[TAN ? PH : 1][TAN ? PW : 1];
   __shared__ float SW[KMAX];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform rows
   const int r0 = wv * RPW;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
@@ -292,7 +293,9 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       const int it = a.n0 + (int)threadIdx.x - 1;
       SW[threadIdx.x] = ((int)threadIdx.x < a.K && it >= 0 && it < a.nt) ? a.wavelet[it] : 0.f;
     }
-    unsigned srcmask = 0, rcvmask = 0, intmask = 0;
+    // row bits (srcrow, introw, rcvmask) are wave-uniform; cell bits per lane
+    unsigned srcmask = 0, rcvmask = 0, intmask = 0, srcrow = 0, introw = 0;
+    const int srow = w.src_cell / NX;
     int rcol[CPR];
 #pragma unroll
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
@@ -301,6 +304,8 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << j;
+      if (gz == srow) srcrow |= 1u << j;
+      if (r >= H && r < RZ - H && oz + r < NZ) introw |= 1u << j;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = j * CPR + c;
@@ -361,10 +366,12 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
             if constexpr (STRIP) L = lap_col<PW>(Prd, col, j, r, cc);
             else L = lap_lds<PW>(Prd, pv[i], r, cc);
             if constexpr (HIST) {
-              if ((intmask >> i) & 1u) hs[gint0 + r * NX + cc] = make_float2(L, q[i]);
+              if ((introw >> j) & 1u) {
+                if ((intmask >> i) & 1u) hs[gint0 + r * NX + cc] = make_float2(L, q[i]);
+              }
             }
             float qn = fmaf(k1[i], q[i], al[i] * L);
-            qn = ((srcmask >> i) & 1u) ? qn + src : qn;
+            if ((srcrow >> j) & 1u) qn = ((srcmask >> i) & 1u) ? qn + src : qn;
             if constexpr (TAN) {
               // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
               float Ld;
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
               float dqn = fmaf(k1[i], dq[i], al[i] * Ld);
               dqn = fmaf(dal[i], L, dqn);
               dqn = fmaf(-dk[i], q[i], dqn);
-              dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
+              if ((srcrow >> j) & 1u) dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
               const float dp1 = dpv[i] + dqn;
               Dwr[r + 2][cc + 2] = dp1;
               dpv[i] = dp1;
@@ -468,7 +475,8 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
   constexpr int PW = RX + 4, PH = RZ + 4;
   __shared__ float D0[PH][PW], D1[PH][PW];
   __shared__ float SW[KMAX];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform rows
   const int r0 = wv * RPW;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;
@@ -496,7 +504,8 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
       const int it = a.n0 + (int)threadIdx.x - 1;
       SW[threadIdx.x] = ((int)threadIdx.x < a.K && it >= 0 && it < a.nt) ? a.wavelet[it] : 0.f;
     }
-    unsigned srcmask = 0, rcvmask = 0, intmask = 0;
+    unsigned srcmask = 0, rcvmask = 0, intmask = 0, srcrow = 0;
+    const int srow = w.src_cell / NX;
     int rcol[CPR];
 #pragma unroll
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
@@ -506,6 +515,7 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
       const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << j;
+      if (gz == srow) srcrow |= 1u << j;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = j * CPR + c;
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
             float dqn = fmaf(k1, dq[i], al[i] * Ld);
             dqn = fmaf(dal[i], hc[i].x, dqn);
             dqn = fmaf(-dk, hc[i].y, dqn);
-            dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
+            if ((srcrow >> j) & 1u) dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
             const float dp1 = dpv[i] + dqn;
             Dwr[r + 2][cc + 2] = dp1;
             dpv[i] = dp1;
@@ -641,7 +651,8 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   __shared__ float A0[PH][PW], A1[PH][PW];
   __shared__ float INJ[KMAX][RX];
   __shared__ float SW[KMAX];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform rows
   const int r0 = wv * RPW;
   const int ntiles = *a.n_tiles;
   const int H = 2 * a.Kt;  // window halo of the tile list; a.K <= a.Kt steps run
@@ -663,6 +674,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     const int gint0 = oz * NX + ox;
     float lam[NC], mu[NC], al[NC], k1[NC], ga[NC], gk[NC];
     float2 hc[NC];
+    int hidx[NC];  // pair index of an interior cell; 0 (a dummy read) elsewhere
     unsigned srcmask = 0, rcvmask = 0, intmask = 0;
     int rcol[CPR];
 #pragma unroll
@@ -707,12 +719,14 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         ga[i] = 0.f;
         gk[i] = 0.f;
         hc[i] = make_float2(0.f, 0.f);
+        hidx[i] = 0;
         if (inner) {
           intmask |= 1u << i;
           const int gi = zz * NX + xx;
           ga[i] = GA[gi];
           gk[i] = GK[gi];
           hc[i] = h0[gi];
+          hidx[i] = gi;
         }
       }
     }
@@ -752,7 +766,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
             if constexpr (STRIP) La = lap_col<PW>(Ard, acol, j, r, cc);
             else La = lap_lds<PW>(Ard, al[i] * lam[i], r, cc);
             float mun = fmaf(k1[i], mu[i], La);
-            mun = (((rcvmask >> j) & 1u) && rc) ? mun + inj : mun;
+            if ((rcvmask >> j) & 1u) mun = rc ? mun + inj : mun;
             const float ln = lam[i] + mun;
             Awr[r + 2][cc + 2] = al[i] * ln;
             lam[i] = ln;
@@ -762,13 +776,11 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       }
       // imaging on the interior (always inside the band): lam now holds lam[m]
       // GA += lam[m] Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
+      // every cell accumulates; only interior cells' sums are stored
 #pragma unroll
       for (int i = 0; i < NC; ++i) {
-        const bool inner = (intmask >> i) & 1u;
-        const float gan = fmaf(lam[i], hc[i].x, ga[i]);
-        const float gkn = fmaf(lam[i], -hc[i].y, gk[i]);
-        ga[i] = inner ? gan : ga[i];
-        gk[i] = inner ? gkn : gk[i];
+        ga[i] = fmaf(lam[i], hc[i].x, ga[i]);
+        gk[i] = fmaf(lam[i], -hc[i].y, gk[i]);
       }
       if (src_here) {  // only the thread owning the source cell
 #pragma unroll
@@ -780,11 +792,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         // dummy address) so that the loads go out back to back
         const float2* hx = hist + (int64_t)(mm - 2 - a.h_base) * Pn;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) {
-          const int r = own_row<STRIP, NW, RPW>(wv, i / CPR);
-          const int cc = lane + 64 * (i % CPR);
-          hc[i] = hx[((intmask >> i) & 1u) ? gint0 + r * NX + cc : 0];
-        }
+        for (int i = 0; i < NC; ++i) hc[i] = hx[hidx[i]];
       }
       __syncthreads();
     };
@@ -1045,7 +1053,9 @@ const Shape kShapes[SEIS_KIND_COUNT][7] = {
      {128, 128, 1024}},
     {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {0, 0, 0}, {0, 0, 0}},
     {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {0, 0, 0}, {0, 0, 0}},
-    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {128, 64, 1024}, {256, 32, 1024}}};
+    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {128, 64, 1024}, {256, 32, 1024}},
+    {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}, {128, 96, 1024},
+     {128, 128, 1024}}};
 
 int shape_threads(int kind, int rx, int rz) {
   for (const Shape& sh : kShapes[kind])
@@ -1058,7 +1068,8 @@ bool cfg_ok(int kind, const KCfg& c) {
          c.rx - 4 * c.k >= 8 && (c.strips == 0 || c.strips == 1);
 }
 
-const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {{64, 32, 3, 1}, {64, 32, 3, 0}, {64, 32, 3, 0}, {64, 32, 2, 1}};
+const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {
+    {64, 32, 3, 1}, {64, 32, 3, 0}, {64, 32, 3, 0}, {64, 32, 2, 1}, {64, 32, 3, 1}};
 
 struct Layout {
   std::vector<ModelDev> models;
@@ -1243,6 +1254,7 @@ const void* fwd_ptr(const KCfg& c) {
 
 const void* kernel_ptr(int kind, const KCfg& c) {
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
+#define PTR_R(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, true>(c);
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
 #define PTR_A(RX, RZ, NT) \
   if (c.rx == RX && c.rz == RZ) return c.strips ? (const void*)k_adj<RX, RZ, NT, true> : (const void*)k_adj<RX, RZ, NT, false>;
@@ -1251,8 +1263,10 @@ const void* kernel_ptr(int kind, const KCfg& c) {
   if (kind == SEIS_KIND_FORWARD) { SEIS_FWD_SHAPES(PTR_F) }
   else if (kind == SEIS_KIND_TANGENT) { SEIS_TAN_SHAPES(PTR_T) }
   else if (kind == SEIS_KIND_ADJOINT) { SEIS_ADJ_SHAPES(PTR_A) }
-  else { SEIS_BORN_SHAPES(PTR_B) }
+  else if (kind == SEIS_KIND_BORN) { SEIS_BORN_SHAPES(PTR_B) }
+  else { SEIS_FWD_SHAPES(PTR_R) }
 #undef PTR_F
+#undef PTR_R
 #undef PTR_T
 #undef PTR_A
 #undef PTR_B
@@ -1383,13 +1397,14 @@ FwdArgs fwd_args(seis_plan* p, int kind) {
 // Forward time loop from the zero state at n = 0 through n = nt (p[1] .. p[nt+1],
 // core/seis_forward2.py:211-247), fused with the tangent when `tan`
 // (:255-283).  A saving forward (`save`) on a full-history plan stores the
-// pairs (Lf(p[n]), q[n]) of n = 0..nt; in checkpoint mode its launches stop at
-// every checkpoint boundary n = j*Kc and store the state there.
+// pairs (Lf(p[n]), q[n]) of n = 0..nt (the RECORD kind's config); in
+// checkpoint mode its launches stop at every checkpoint boundary n = j*Kc and
+// store the state there.
 void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   const Layout& L = p->L;
-  const int kind = tan ? SEIS_KIND_TANGENT : SEIS_KIND_FORWARD;
-  const KCfg& c = p->cfg[kind];
   const bool hist = save && L.fwd_hist;
+  const int kind = tan ? SEIS_KIND_TANGENT : (hist ? SEIS_KIND_RECORD : SEIS_KIND_FORWARD);
+  const KCfg& c = p->cfg[kind];
   const bool ck = save && !L.full && (p->caps & SEIS_CAP_ADJOINT);
   fill(p, s, R_Q, 0, 1);
   fill(p, s, R_PP, 0, 1);
@@ -1483,8 +1498,8 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
     run_range(p->nt + 1, 2, 0);
     return;
   }
-  const KCfg& cf = p->cfg[SEIS_KIND_FORWARD];
-  FwdArgs f = fwd_args(p, SEIS_KIND_FORWARD);
+  const KCfg& cf = p->cfg[SEIS_KIND_RECORD];
+  FwdArgs f = fwd_args(p, SEIS_KIND_RECORD);
   f.write_seis = 0;
   for (int j = L.J - 1; j >= 0; --j) {
     const int lo = j * L.Kc;
@@ -1499,7 +1514,7 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
       f.p_in_slot = f.q_in_slot = fi & 1;
       f.p_out_slot = f.q_out_slot = (fi + 1) & 1;
       f.hist_slot0 = n - lo;
-      launch_fwd(cf, false, p->tl[SEIS_KIND_FORWARD].grid, s, f);
+      launch_fwd(cf, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
       n += K;
       ++fi;
     }

@@ -78,7 +78,7 @@ def main():
     torch.cuda.synchronize()
     del flush
     info = {"batch": B, "fanout_models": Pf.n_models, "calibration_copy_bytes": 4 * n}
-    kind_of = {"linesearch_forward": (Pf, "forward"), "saving_forward": (P, "forward"), "born": (P, "born"),
+    kind_of = {"linesearch_forward": (Pf, "forward"), "saving_forward": (P, "record"), "born": (P, "born"),
                "adjoint": (P, "adjoint"), "tangent": (P, "tangent")}
     for k in PASSES:
         plan, kind = kind_of[k]

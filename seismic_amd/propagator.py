@@ -134,7 +134,7 @@ class Propagator:
         return d.to(device=self.device, dtype=torch.float32).contiguous()
 
     KINDS = {"forward": _lib.KIND_FORWARD, "tangent": _lib.KIND_TANGENT, "adjoint": _lib.KIND_ADJOINT,
-             "born": _lib.KIND_BORN}
+             "born": _lib.KIND_BORN, "record": _lib.KIND_RECORD}
 
     def set_config(self, kind: str, rx: int, rz: int, k: int, strips: int | None = None):
         """Launch config of one kernel kind: rx x rz LDS window, k steps per

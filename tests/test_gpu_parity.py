@@ -113,7 +113,7 @@ def test_tile_configs_bit_identical(small_geoms):
     for ckpt in (0, 17):
         P = Propagator([g], ckpt_every=ckpt)
         for cfg in CONFIGS:
-            for kind in ("forward", "tangent", "adjoint"):
+            for kind in ("forward", "record", "tangent", "adjoint"):
                 P.set_config(kind, *cfg)
                 assert P.get_config(kind) == cfg
             s, mis, gr = P.gradient(torch.tensor(vec), torch.tensor(target))
@@ -151,6 +151,7 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
         ref = _np(P.forward(torch.tensor(vec)))
         _, _, gref = P.gradient(torch.tensor(vec), target)
         P.set_config("forward", *cfg)
+        P.set_config("record", *cfg)
         assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref)
         _, _, gr = P.gradient(torch.tensor(vec), target)
         assert np.array_equal(_np(gr), _np(gref)), ckpt
