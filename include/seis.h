@@ -184,8 +184,8 @@ enum {
  * neighbours stay in registers), strips = 0 deals rows round-robin over the
  * wavefronts (even work per wavefront as the valid band shrinks).
  * Windows (rx x rz): forward and record 64x32, 128x32, 64x64, 128x64, 256x32,
- * 128x96, 128x128; Born 64x32, 128x32, 64x64, 128x64, 256x32; tangent and
- * adjoint 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4 rows and
+ * 128x96, 128x128; Born 64x32, 128x32, 64x64, 128x64, 256x32; adjoint 64x32,
+ * 128x32, 64x64, 128x64; tangent 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4 rows and
  * 8 columns.  Results do not depend on the config
  * (bit-identical wavefields).  No reference counterpart: the reference runs
  * one 32x32-thread launch per step (core/seis_forward2.py:191-193).

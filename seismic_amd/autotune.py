@@ -31,9 +31,9 @@ _WINDOWS = {
     "forward": [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8), (256, 32, 3),
                 (128, 96, 6), (128, 96, 8), (128, 128, 6), (128, 128, 8)],
     "tangent": [(64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4)],
-    "adjoint": [(64, 32, 3), (64, 32, 4), (64, 32, 5), (128, 32, 3), (128, 32, 4), (64, 64, 3), (64, 64, 4),
-                (64, 64, 5)],
-    "born": [(64, 32, 3), (128, 32, 3), (64, 64, 3), (64, 64, 4), (128, 64, 3), (128, 64, 4), (128, 64, 5),
+    "adjoint": [(64, 32, 3), (64, 32, 4), (128, 32, 3), (128, 32, 4), (64, 64, 4), (64, 64, 5), (128, 64, 4),
+                (128, 64, 5), (128, 64, 6), (128, 64, 7)],
+    "born": [(64, 32, 3), (128, 32, 3), (64, 64, 3), (64, 64, 4), (128, 64, 4), (128, 64, 5), (128, 64, 6),
              (256, 32, 3)],
 }
 _WINDOWS["record"] = [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128, 64, 6), (128, 64, 8),

@@ -1042,9 +1042,10 @@ struct KCfg {
 };
 
 // Instantiated windows per kernel kind: rx x rz cells on `nt` threads.  The
-// forward keeps 8 cells per thread (about 100 VGPRs); the fused tangent and the
-// adjoint carry about twice the per-cell state and run 4 cells per thread, so
-// that 5-6 wavefronts per SIMD stay resident; the Born kernel has both.
+// forward keeps 8-16 cells per thread; the fused tangent and the adjoint carry
+// about twice the per-cell state and run 4 cells per thread (5-6 wavefronts
+// per SIMD), plus one 8-cell adjoint window (128x64) whose larger interior
+// outweighs its lower occupancy; the Born kernel has both.
 struct Shape {
   int rx, rz, nt;
 };
@@ -1052,7 +1053,7 @@ const Shape kShapes[SEIS_KIND_COUNT][7] = {
     {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}, {128, 96, 1024},
      {128, 128, 1024}},
     {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {0, 0, 0}, {0, 0, 0}},
-    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {0, 0, 0}, {0, 0, 0}},
+    {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {128, 64, 1024}, {0, 0, 0}},
     {{64, 32, 512}, {128, 32, 1024}, {64, 64, 1024}, {128, 64, 1024}, {256, 32, 1024}},
     {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}, {128, 96, 1024},
      {128, 128, 1024}}};
@@ -1242,7 +1243,7 @@ int dev_alloc(T** ptr, int64_t count) {
 #define SEIS_FWD_SHAPES(X) X(64, 32, 256) X(128, 32, 512) X(64, 64, 512) X(128, 64, 1024) X(256, 32, 1024) \
   X(128, 96, 1024) X(128, 128, 1024)
 #define SEIS_TAN_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024)
-#define SEIS_ADJ_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024)
+#define SEIS_ADJ_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024) X(128, 64, 1024)
 #define SEIS_BORN_SHAPES(X) X(64, 32, 512) X(128, 32, 1024) X(64, 64, 1024) X(128, 64, 1024) X(256, 32, 1024)
 
 // Instantiation of a step kernel for config c (row ownership, and for the

@@ -132,7 +132,13 @@ def test_tile_configs_bit_identical(small_geoms):
     with pytest.raises(ValueError):
         P.set_config("forward", 64, 32, 8)   # interior would be empty
     with pytest.raises(ValueError):
-        P.set_config("adjoint", 128, 64, 2)  # forward-only window
+        P.set_config("adjoint", 128, 128, 2)  # forward-only window
+    # the 8-cell adjoint window
+    for ckpt in (0, 17):
+        P = Propagator([g], ckpt_every=ckpt)
+        P.set_config("adjoint", 128, 64, 6, 1)
+        s, mis, gr = P.gradient(torch.tensor(vec), torch.tensor(target))
+        assert np.array_equal(_np(gr), ref[1]) and float(mis[0]) == ref[3]
     with pytest.raises(ValueError):
         P.set_config("born", 64, 32, 2, 2)   # no such row ownership
 
