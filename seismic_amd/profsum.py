@@ -1,7 +1,9 @@
 """Summarise rocprofv3 CSV output into short markdown tables.
 
-    python -m seismic_amd.profsum trace  <kernel_trace.csv>            per (kernel, grid) durations
-    python -m seismic_amd.profsum pmc    <counter_collection.csv> NAME per (kernel, grid) counter means
+    python -m seismic_amd.profsum trace   <kernel_trace.csv>            per (kernel, grid) durations
+    python -m seismic_amd.profsum pmc     <counter_collection.csv> NAME per (kernel, grid) counter means
+    python -m seismic_amd.profsum traffic <probe.log> <FETCH_SIZE csv> <WRITE_SIZE csv>
+                                          HBM bytes per launch of each kernel_probe pass (JSON)
 """
 from __future__ import annotations
 
@@ -37,6 +39,52 @@ def trace(path):
               f"{sum(v)/1e6:.2f} | {sum(v)/total:.1%} |")
 
 
+def traffic(probe_log, fetch_csv, write_csv):
+    """Per pass of seismic_amd.kernel_probe: mean FETCH_SIZE / WRITE_SIZE per
+    dispatch of the pass's kernel (matched by name from the probe's configs),
+    scaled to bytes by the probe's 1 GiB calibration copy (the largest
+    copyBuffer dispatch of each run)."""
+    import json
+    import re
+    info = json.loads([line for line in open(probe_log) if line.startswith("{")][-1])
+    alg = {"linesearch_forward": 20, "saving_forward": 28, "tangent": 40, "born": 36, "adjoint": 28}
+
+    def pattern(name, e):
+        rx, rz = e["window"]
+        st = "true" if e["strips"] else "false"
+        return {"linesearch_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, false>",
+                "saving_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, true>",
+                "tangent": rf"k_fwd<{rx}, {rz}, \d+, true, {st}, false>",
+                "born": rf"k_born<{rx}, {rz}, \d+, {st}>",
+                "adjoint": rf"k_adj<{rx}, {rz}, \d+, {st}>"}[name]
+
+    def load(path):
+        rows = list(csv.DictReader(open(path)))
+        copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+        return rows, copy
+
+    fr, fcopy = load(fetch_csv)
+    wr, wcopy = load(write_csv)
+    gib_kib = info["calibration_copy_bytes"] / 1024.0
+    out = {"read_factor": round(gib_kib / fcopy, 4), "write_factor": round(gib_kib / wcopy, 4)}
+    for name in alg:
+        e = dict(info[name])
+        pat = re.compile(pattern(name, e))
+        fv = [float(r["Counter_Value"]) for r in fr if pat.search(r["Kernel_Name"])]
+        wv = [float(r["Counter_Value"]) for r in wr if pat.search(r["Kernel_Name"])]
+        if not fv or not wv:
+            continue
+        e["fetch_kib"] = sum(fv) / len(fv)
+        e["write_kib"] = sum(wv) / len(wv)
+        e["dispatches"] = len(fv)
+        e["bytes_per_launch"] = int(round((e["fetch_kib"] * out["read_factor"] + e["write_kib"] * out["write_factor"])
+                                          * 1024))
+        e["alg_bytes_per_launch"] = alg[name] * e["cell_updates_per_launch"]
+        e["traffic_over_alg"] = round(e["bytes_per_launch"] / e["alg_bytes_per_launch"], 3)
+        out[name] = e
+    print(json.dumps(out, indent=1))
+
+
 def pmc(path, counter):
     rows = list(csv.DictReader(open(path)))
     agg = defaultdict(list)
@@ -54,5 +102,7 @@ def pmc(path, counter):
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         trace(sys.argv[2])
+    elif sys.argv[1] == "traffic":
+        traffic(sys.argv[2], sys.argv[3], sys.argv[4])
     else:
         pmc(sys.argv[2], sys.argv[3])
