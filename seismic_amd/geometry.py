@@ -32,9 +32,11 @@ def ricker(freq: float, dt: float, nt: int) -> np.ndarray:
 class Geometry:
     """Grid, time axis and acquisition of one velocity model.
 
-    Rows are depth (z), columns lateral (x).  ``src_z``/``rcv_z`` are physical
-    rows: the reference places shots and receivers at z = dx, which
-    adjust_sr (core/seis_forward2.py:60-68) maps to physical row 1.
+    Rows are depth (z), columns lateral (x).  ``src_z``/``rcv_z`` are the
+    depths of shots and receivers in grid cells (z / dx).  As adjust_sr
+    (core/seis_forward2.py:60-68) resolves them, a depth of 0 is moved down to
+    physical row 1 (``src_row``/``rcv_row``), so the reference's z = dx and
+    z = 0 both inject and record on row 1.
     """
 
     nz: int = 70
@@ -62,6 +64,16 @@ class Geometry:
         r = np.asarray(self.rcv_x)
         if len(r) > 1 and not (np.all(np.diff(r) == r[1] - r[0]) and r[1] > r[0]):
             raise ValueError("receivers must sit at evenly spaced, increasing columns")
+
+    @property
+    def src_row(self) -> int:
+        """Physical row of the shots (adjust_sr, core/seis_forward2.py:62,65-66)."""
+        return self.src_z + (1 if self.src_z == 0 else 0)
+
+    @property
+    def rcv_row(self) -> int:
+        """Physical row of the receivers (adjust_sr, core/seis_forward2.py:64,67)."""
+        return self.rcv_z + (1 if self.rcv_z == 0 else 0)
 
     @property
     def NZ(self) -> int:

@@ -37,10 +37,10 @@ def model_desc(g: Geometry) -> _lib.ModelDesc:
     d = _lib.ModelDesc()
     d.nz, d.nx, d.dx = g.nz, g.nx, g.dx
     d.n_shots = g.n_shots
-    d.src_z = g.src_z
+    d.src_z = g.src_row
     for i, x in enumerate(g.src_x):
         d.src_x[i] = x
-    d.rcv_z = g.rcv_z
+    d.rcv_z = g.rcv_row
     d.rcv_x0 = g.rcv_x[0]
     d.rcv_dx = g.rcv_x[1] - g.rcv_x[0] if g.n_rcv > 1 else 1
     d.n_rcv = g.n_rcv

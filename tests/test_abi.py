@@ -68,3 +68,27 @@ def test_last_error_message():
     g = Geometry(nz=10, nx=10, nbc=5, nt=200)
     with pytest.raises(ValueError, match="ckpt_every"):
         plan_bytes([g], ckpt_every=1)
+
+
+@pytest.mark.parametrize("src_z,rcv_z", [(1, 1), (0, 0), (0, 3), (4, 0), (2, 5)])
+def test_geometry_rows_follow_adjust_sr(src_z, rcv_z):
+    """The rows handed to the library are the oracle's adjust_sr rows
+    (core/seis_forward2.py:60-68: a depth of 0 moves to row 1)."""
+    from oracle import fd2d
+    from seismic_amd.propagator import model_desc
+    kw = dict(nz=12, nx=16, nbc=8, nt=50, src_x=(0, 7, 15), rcv_x=(2, 5, 8, 11, 14), src_z=src_z, rcv_z=rcv_z)
+    d = model_desc(Geometry(**kw))
+    og = fd2d.Geometry(**kw)
+    igz, igx = og.rcv_cells()
+    assert all(isz - og.nbc == d.src_z for isz, _ in og.src_cells())
+    assert [isx - og.nbc for _, isx in og.src_cells()] == list(d.src_x[:d.n_shots])
+    assert set(igz - og.nbc) == {d.rcv_z}
+    assert list(igx - og.nbc) == [d.rcv_x0 + r * d.rcv_dx for r in range(d.n_rcv)]
+
+
+def test_nt_below_wavelet_length_is_rejected():
+    """ricker (core/seis_forward2.py:20-30) raises when nt is shorter than the
+    wavelet (147 samples at 15 Hz, dt 1 ms); so does the geometry here."""
+    assert len(Geometry(nt=147).wavelet()) == 147
+    with pytest.raises(ValueError):
+        Geometry(nt=146).wavelet()
