@@ -177,7 +177,10 @@ def cpu_baseline(g, seconds_hint=25.0):
         t_fev = time.perf_counter() - t0
     t_gnp = t_fev + t_fev - t_fwd           # tangent ~ forward + adjoint cost, then adjoint
     t_inv = N_OUTER * (t_fev + N_CG * t_gnp + N_SCALES * t_fwd)
+    cells = n * (g.nt + 1) * og.NZ * og.NX
     return {"value": 1.0 / t_inv, "unit": "inversions/s", "cores": workers, "kind": "port",
+            "s_per_forward": t_fwd, "s_per_fev": t_fev, "fevs_per_s": 1.0 / t_fev,
+            "forward_cell_updates_per_s": cells / t_fwd,
             "sample": f"NumPy float64 oracle, {g.nz}x{g.nx} nt={g.nt}, {n} shots in {workers} processes: "
                       f"forward {t_fwd:.2f} s, forward+adjoint gradient {t_fev:.2f} s; "
                       f"scaled to {N_OUTER} GN passes x ({N_CG} CG + {N_SCALES} line-search forwards)"}
@@ -362,7 +365,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="GN passes timed (10 = one full inversion)")
     ap.add_argument("--warmup", type=int, default=1, help="GN passes run untimed first")
-    ap.add_argument("--batch", type=int, default=16, help="models per batched launch")
+    ap.add_argument("--batch", type=int, default=32, help="models per batched launch")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--lam", type=float, default=1e-8)
     ap.add_argument("--serve-requests", type=int, default=24)

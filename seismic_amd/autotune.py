@@ -162,6 +162,7 @@ def main():
     work = [
         ("plain", [g70], 0, 7),
         ("batch16", [g70] * 16, 0, 7),
+        ("batch32", [g70] * 32, 0, 7),
         ("batch16-ckpt", [g70] * 16, 50, 7),
         ("batch64-ckpt", [g70] * 64, 50, 7),
         ("serve-mix", [sizes[0]] * 12 + [sizes[1]] * 8 + [sizes[2]] * 4, 50, 7),
