@@ -35,7 +35,7 @@ PASSES = ("linesearch_forward", "saving_forward", "born", "adjoint", "tangent")
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--fanout", type=int, default=50)
     ap.add_argument("--kind", default="all")
     ap.add_argument("--nt", type=int, default=1000)
