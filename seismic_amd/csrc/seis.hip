@@ -48,6 +48,7 @@ constexpr float kC3 = -1.0f / 12.0f;  // core/seis_forward2.py:81
 
 constexpr int NTHR = 256;        // 4 wavefronts per workgroup
 constexpr int NXCD = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kKzPad = 64;       // rows of periodic apron on each side of a model's 1 - kappa row table
 
 // Per-wavefield state regions in the float arena.
 enum Region {
@@ -71,6 +72,7 @@ struct ModelDev {
   int nbc, n_shots, first_wave, active;
   int64_t vec_off;    // into the concatenated parameter vectors
   int64_t cell_off;   // into per-model padded coefficient fields
+  int64_t kz_off;     // into the alpha array: 1 - kappa of the padded rows (kKzPad apron each side)
   int64_t seis_off;   // into the concatenated seismograms
   int64_t seis_len;   // n_shots * nt * n_rcv
   int64_t fold_off;   // into the float64 fold scratch (NZ*nx + NZ per model)
@@ -435,6 +437,199 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
           a.arena[w.off[R_DP] + a.dp_out_slot * Pn + g] = dpv[i];
           a.arena[w.off[R_DQ] + a.dq_out_slot * Pn + g] = dq[i];
         }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- forward, streamed down the rows
+constexpr int SUN = 12;  // pipeline steps per unrolled block: a multiple of every ring length
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {  // f(integral_constant<I>) .. f(<N-1>)
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>());
+    static_for<I + 1, N>(f);
+  }
+}
+
+constexpr int SPD = 3;   // pipeline steps between a row's load and its intake
+
+
+// 1 - kappa of padded row gz in a column outside the left/right damping
+// strips (the same expression as the per-cell value of k_fwd).
+__device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float kscale) {
+  const int j = damp_j(gz, m.nbc, m.nz, m.nx, m.nbc);
+  return 1.0f - kscale * (float)(j * j);
+}
+
+// KL forward time steps (update_p :428-462, lapg :381-421, source injection
+// fused) over a column strip of NT padded columns, one per thread, marched
+// down the rows of a segment in a software pipeline.  Pipeline step t takes
+// in window row t of p[n0], q[n0] and alpha (requested SPD steps earlier) as
+// level 0, and level l = 1..KL computes window row t - 2l of p[n0+l], q[n0+l]
+// from level l-1; levels run in increasing order, so the lower neighbour two
+// rows down is the value level l-1 produced in the same step.  Each thread
+// keeps the last rows of its own column of every level in registers (centre,
+// vertical neighbours, q) and alpha of the rows in flight in a ring, and
+// reads the horizontal neighbours from a three-row LDS ring per level, with
+// one barrier per pipeline step; 1 - kappa outside the left/right damping
+// strips, one value per padded row, comes from the model's row table with
+// the row's other inputs and rides in a ring like alpha.
+// The window is the segment's rows (TileEnt.pad of them) plus 2*KL rows above
+// and below and 2*KL columns either side, wrapped periodically like k_fwd's
+// tiles; cells outside the valid band compute values no committed cell
+// depends on.  Pipeline steps that meet the source row or the receiver row
+// run a separate copy of the step body, so the others carry no per-cell
+// source or receiver work; receiver samples are kept in registers and stored
+// after the march.  Same stencil order and fused multiply-adds as k_fwd, so
+// wavefields and seismograms are bit-identical to it.  Launches always
+// advance KL steps (a shorter last launch runs k_fwd).
+template <int NT, int KL>
+__global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
+  static_assert(2 * KL + SPD < SUN && SUN % 6 == 0 && SUN % SPD == 0, "pipeline rings too short");
+  constexpr int LW = NT + 4;   // two apron cells either side
+  constexpr int H = 2 * KL;
+  __shared__ float RG[KL][3][LW];
+  const unsigned c = threadIdx.x;
+  const int ntiles = *a.n_tiles;
+  const int n0 = a.n0, nt = a.nt;
+  for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    const TileEnt te = a.tiles[e];
+    if (te.w < 0) continue;
+    const WaveDev& w = a.waves[te.w];
+    const ModelDev& m = a.models[w.model];
+    const int NZ = w.NZ, NX = w.NX, nz = m.nz, nx = m.nx, nbc = m.nbc;
+    const int64_t Pn = (int64_t)NZ * NX;
+    const float* pin = a.arena + w.off[R_PP] + a.p_in_slot * Pn;
+    const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
+    float* pout = a.arena + w.off[R_PP] + a.p_out_slot * Pn;
+    float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
+    float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
+    float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
+    float* seis = a.write_seis ? a.seis + w.seis_off : nullptr;
+    const float* alpha = a.alpha + m.cell_off;
+    const float kscale = a.params[w.model].kscale;
+    const float amp = a.wparam[te.w].amp;
+    const int n_rcv = m.n_rcv, rcv_row = m.rcv_row;
+    const float* kzt = a.alpha + m.kz_off + kKzPad;   // kzt[z]: padded row z, -kKzPad <= z < NZ + kKzPad
+    const int IZ = te.pad;              // rows of the segment (all inside the grid)
+    const int U = IZ + 2 * H;           // window rows
+    const int oz = te.z0 - H, ox = te.x0 - H;
+    const unsigned gx = wrapi(ox + (int)c, NX);
+    const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
+    const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
+    const int jx = damp_j(0, gx, nz, nx, nbc);
+    const float kx1 = 1.0f - kscale * (float)(jx * jx);
+    const int rc = rcv_index(gx, m);
+    const int srow = w.src_cell / NX;
+    const bool srcc = (int)gx == w.src_cell - srow * NX;
+    // window rows holding the source row (twice when the window wraps), and
+    // the segment row holding the receiver row (-1 if none)
+    const int su0 = wrapi(srow - oz, NZ), su1 = su0 + NZ;
+    const int ruz = rcv_row - te.z0;
+    const int rui = ruz >= 0 && ruz < IZ ? ruz + H : -(1 << 20);
+    float srcv[KL], rec[KL];
+#pragma unroll
+    for (int l = 1; l <= KL; ++l) {
+      const int it = n0 + l - 2;
+      srcv[l - 1] = amp * ((it >= 0 && it < nt) ? a.wavelet[it] : 0.f);
+      rec[l - 1] = 0.f;
+    }
+    float pr[KL][6], qr[KL][3], ar[SUN], kr[SUN], pfp[SPD], pfq[SPD];
+#pragma unroll
+    for (int l = 0; l < KL; ++l) {
+#pragma unroll
+      for (int s = 0; s < 6; ++s) pr[l][s] = 0.f;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) qr[l][s] = 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < SUN; ++s) ar[s] = kr[s] = 0.f;
+#pragma unroll
+    for (int d = 0; d < SPD; ++d) {
+      const int64_t ro = (int64_t)wrapi(oz + d, NZ) * NX;
+      pfp[d] = (pin + ro)[gx];
+      pfq[d] = (qin + ro)[gx];
+      ar[d] = (alpha + ro)[gx];
+      kr[d] = kzt[oz + d];
+    }
+    int gzn = wrapi(oz + SPD, NZ);      // padded row requested next
+    __syncthreads();
+    // one pipeline step; SPECIAL: some level meets the source or receiver row
+    auto body = [&](auto ic, auto sc, const int t) {
+      constexpr int i = decltype(ic)::value;
+      constexpr bool SPECIAL = decltype(sc)::value;
+      {  // level 0: intake of window row t
+        const float p0 = pfp[i % SPD];
+        RG[0][i % 3][c + 2] = p0;
+        pr[0][i % 6] = p0;
+        qr[0][i % 3] = pfq[i % SPD];
+      }
+      static_for<1, KL + 1>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        const int u = t - 2 * l;                      // window row of level l
+        constexpr int B = 4 * SUN;                    // keeps the ring indices non-negative
+        constexpr int s6 = (i - 2 * l + B) % 6, s3 = (i - 2 * l + B) % 3, sr = (i - 2 * l + B) % SUN;
+        const float pc = pr[l - 1][s6];
+        const float (*T)[LW] = RG[l - 1];
+        float t1 = (T[s3][c + 3] - pc) + (T[s3][c + 1] - pc);
+        t1 = t1 + (pr[l - 1][(i - 2 * l + 1 + B) % 6] - pc);
+        t1 = t1 + (pr[l - 1][(i - 2 * l - 1 + B) % 6] - pc);
+        float t2 = (T[s3][c + 4] - pc) + (T[s3][c] - pc);
+        t2 = t2 + (pr[l - 1][(i - 2 * l + 2 + B) % 6] - pc);
+        t2 = t2 + (pr[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
+        const float L = fmaf(kC2, t1, kC3 * t2);
+        const float k1 = isx ? kx1 : kr[sr];
+        float qn = fmaf(k1, qr[l - 1][s3], ar[sr] * L);
+        if constexpr (SPECIAL) {
+          if (u == su0 || u == su1) qn = srcc ? qn + srcv[l - 1] : qn;
+        }
+        const float p1 = pc + qn;
+        if constexpr (SPECIAL) {
+          if (u == rui) rec[l - 1] = p1;
+        }
+        if constexpr (l < KL) {
+          RG[l][s3][c + 2] = p1;
+          pr[l][s6] = p1;
+          qr[l][s3] = qn;
+        } else if (u >= H && u < H + IZ && colint) {  // committed cell of p[n0+KL], q[n0+KL]
+          const int64_t ro = (int64_t)(oz + u) * NX;
+          (pout + ro)[gx] = p1;
+          (qout + ro)[gx] = qn;
+          if (ckp) {
+            (ckp + ro)[gx] = p1;
+            (ckq + ro)[gx] = qn;
+          }
+        }
+      });
+      {  // request of window row t + SPD
+        const int64_t ro = (int64_t)gzn * NX;
+        pfp[i % SPD] = (pin + ro)[gx];
+        pfq[i % SPD] = (qin + ro)[gx];
+        ar[(i + SPD) % SUN] = (alpha + ro)[gx];
+        kr[(i + SPD) % SUN] = kzt[oz + t + SPD];        // oz + t + SPD in [-H, NZ + H + SUN)
+        gzn = gzn + 1 == NZ ? 0 : gzn + 1;
+      }
+      __syncthreads();
+    };
+    auto hits = [&](int t, int r) {  // does some level 1..KL compute window row r at step t
+      const int d = t - r;
+      return d >= 2 && d <= 2 * KL && !(d & 1);
+    };
+    for (int tb = 0; tb < U; tb += SUN) {
+      static_for<0, SUN>([&](auto ic) {
+        const int t = tb + decltype(ic)::value;
+        if (hits(t, su0) || hits(t, su1) || hits(t, rui))
+          body(ic, std::true_type(), t);
+        else
+          body(ic, std::false_type(), t);
+      });
+    }
+    if (seis && rui >= H && colint && rc >= 0) {
+#pragma unroll
+      for (int l = 1; l <= KL; ++l) {
+        const int it = n0 + l - 2;
+        if (it >= 0 && it < nt) seis[(int64_t)it * n_rcv + rc] = rec[l - 1];
       }
     }
   }
@@ -865,6 +1060,16 @@ __global__ __launch_bounds__(NTHR) void k_prep(const ModelDev* models, const flo
   const float* dv = dvec ? dvec + m.vec_off : nullptr;
   const int64_t P = (int64_t)m.NZ * m.NX;
   const double dtdx2 = m.dtdx * m.dtdx;
+  // 1 - kappa of every padded row outside the left/right damping strips, with
+  // a periodic apron (entry e is padded row e - kKzPad modulo NZ); kappa
+  // scale and expression as in k_params and k_fwd
+  const float kscale = (float)((double)v[m.nz * m.nx] * m.kunit);
+  for (int e = blockIdx.x * NTHR + threadIdx.x; e < m.NZ + 2 * kKzPad; e += gridDim.x * NTHR) {
+    int z = (e - kKzPad) % m.NZ;
+    if (z < 0) z += m.NZ;
+    const int j = damp_j(z, m.nbc, m.nz, m.nx, m.nbc);
+    alpha[m.kz_off + e] = 1.0f - kscale * (float)(j * j);
+  }
   for (int64_t i = (int64_t)blockIdx.x * NTHR + threadIdx.x; i < P; i += (int64_t)gridDim.x * NTHR) {
     const int z = (int)(i / m.NX);
     const int x = (int)(i - (int64_t)z * m.NX);
@@ -1058,6 +1263,18 @@ const Shape kShapes[SEIS_KIND_COUNT][7] = {
     {{64, 32, 256}, {128, 32, 512}, {64, 64, 512}, {128, 64, 1024}, {256, 32, 1024}, {128, 96, 1024},
      {128, 128, 1024}}};
 
+// Streamed forward (strips = 2): rx-column strips, one thread per column, k
+// levels per launch, rz rows per segment (0: the whole column).
+constexpr int kStreamNT[3] = {128, 192, 256};
+constexpr int kStreamKMin = 3, kStreamKMax = 4;
+
+bool stream_cfg(int kind, const KCfg& c) {
+  if (kind != SEIS_KIND_FORWARD || c.strips != 2) return false;
+  bool nt_ok = false;
+  for (int nt : kStreamNT) nt_ok = nt_ok || c.rx == nt;
+  return nt_ok && c.k >= kStreamKMin && c.k <= kStreamKMax && (c.rz == 0 || c.rz >= 8);
+}
+
 int shape_threads(int kind, int rx, int rz) {
   for (const Shape& sh : kShapes[kind])
     if (sh.rx == rx && sh.rz == rz && sh.nt) return sh.nt;
@@ -1065,6 +1282,7 @@ int shape_threads(int kind, int rx, int rz) {
 }
 
 bool cfg_ok(int kind, const KCfg& c) {
+  if (c.strips == 2) return stream_cfg(kind, c);
   return shape_threads(kind, c.rx, c.rz) && c.k >= 1 && c.k <= 8 && c.rz - 4 * c.k >= 4 &&
          c.rx - 4 * c.k >= 8 && (c.strips == 0 || c.strips == 1);
 }
@@ -1121,6 +1339,7 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
     m.kunit = (double)td->dt * m.kappa0 / ((double)(nbc - 1) * (nbc - 1));
     const int64_t P = (int64_t)m.NZ * m.NX;
     if (P > INT32_MAX / 4) return fail(SEIS_EINVAL, who + "model too large");
+    m.kz_off = L.cells + align64(P);
     L.max_P = std::max<int>(L.max_P, (int)P);
     L.max_NZ = std::max(L.max_NZ, m.NZ);
     L.max_cells_phys = std::max(L.max_cells_phys, d.nz * d.nx);
@@ -1129,7 +1348,7 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
     L.vec += (int64_t)d.nz * d.nx + 1;
     m.fold_off = L.fold;
     L.fold += align64((int64_t)m.NZ * m.nx + m.NZ);
-    L.cells += align64(P);
+    L.cells += align64(P) + align64(m.NZ + 2 * kKzPad);
     L.seis += m.seis_len;
     for (int s = 0; s < d.n_shots; ++s) {
       WaveDev w{};
@@ -1253,7 +1472,19 @@ const void* fwd_ptr(const KCfg& c) {
   return c.strips ? (const void*)k_fwd<RX, RZ, NT, TAN, true, HIST> : (const void*)k_fwd<RX, RZ, NT, TAN, false, HIST>;
 }
 
+template <int NT>
+const void* fws_ptr_nt(int k) {
+  return k == 3 ? (const void*)k_fws<NT, 3> : (const void*)k_fws<NT, 4>;
+}
+
+const void* fws_ptr(const KCfg& c) {
+  if (c.rx == 128) return fws_ptr_nt<128>(c.k);
+  if (c.rx == 192) return fws_ptr_nt<192>(c.k);
+  return fws_ptr_nt<256>(c.k);
+}
+
 const void* kernel_ptr(int kind, const KCfg& c) {
+  if (c.strips == 2) return fws_ptr(c);
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
 #define PTR_R(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, true>(c);
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
@@ -1286,7 +1517,19 @@ void launch_fwd_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
   }
 }
 
+template <int NT>
+void launch_fws_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
+  if (c.k == 3) hipLaunchKernelGGL((k_fws<NT, 3>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_fws<NT, 4>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
+  if (c.strips == 2) {
+    if (c.rx == 128) launch_fws_t<128>(c, grid, s, a);
+    else if (c.rx == 192) launch_fws_t<192>(c, grid, s, a);
+    else launch_fws_t<256>(c, grid, s, a);
+    return;
+  }
 #define LF(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_fwd_t<RX, RZ, NT, false>(c, grid, s, a); return; }
 #define LT(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_fwd_t<RX, RZ, NT, true>(c, grid, s, a); return; }
   if (tan) { SEIS_TAN_SHAPES(LT) }
@@ -1325,12 +1568,16 @@ void launch_born(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
 // halo rows and columns, stay in one XCD's L2.
 std::vector<TileEnt> make_tiles(const Layout& L, const KCfg& c, bool all_active) {
   std::vector<TileEnt> seq;
-  const int IZ = c.rz - 4 * c.k, IX = c.rx - 4 * c.k;
+  const bool stream = c.strips == 2;
+  const int IX = c.rx - 4 * c.k;
   for (int wi = 0; wi < (int)L.waves.size(); ++wi) {
     const WaveDev& w = L.waves[wi];
     if (!all_active && !L.models[w.model].active) continue;
+    // streamed strips carry their segment's row count in `pad`
+    const int IZ = stream ? (c.rz > 0 ? c.rz : w.NZ) : c.rz - 4 * c.k;
     for (int z0 = 0; z0 < w.NZ; z0 += IZ)
-      for (int x0 = 0; x0 < w.NX; x0 += IX) seq.push_back(TileEnt{wi, z0, x0, 0});
+      for (int x0 = 0; x0 < w.NX; x0 += IX)
+        seq.push_back(TileEnt{wi, z0, x0, stream ? std::min(IZ, w.NZ - z0) : 0});
   }
   const int n = (int)seq.size();
   const int per = (n + NXCD - 1) / NXCD;
@@ -1370,7 +1617,7 @@ int setup_kind(seis_plan* p, int kind) {
     if (rc) return rc;
   }
   int per_cu = 0;
-  const int nthr = shape_threads(kind, p->cfg[kind].rx, p->cfg[kind].rz);
+  const int nthr = p->cfg[kind].strips == 2 ? p->cfg[kind].rx : shape_threads(kind, p->cfg[kind].rx, p->cfg[kind].rz);
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kind, p->cfg[kind]), nthr, 0));
   int gmax = std::max(1, per_cu) * p->num_cu;
   gmax = std::max(NXCD, gmax / NXCD * NXCD);
@@ -1419,17 +1666,26 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   }
   FwdArgs a = fwd_args(p, kind);
   a.write_seis = 1;
+  // a streamed forward always advances c.k steps; shorter launches (the last
+  // one, or up to a checkpoint) run k_fwd with the record kind's window
+  const KCfg& cr = p->cfg[SEIS_KIND_RECORD];
+  FwdArgs ar = fwd_args(p, SEIS_KIND_RECORD);
   int n = 0, li = 0;
   while (n < p->nt + 1) {
     int K = std::min(c.k, p->nt + 1 - n);
     if (ck) K = std::min(K, (n / L.Kc + 1) * L.Kc - n);
-    a.n0 = n;
-    a.K = K;
-    a.p_in_slot = a.q_in_slot = a.dp_in_slot = a.dq_in_slot = li & 1;
-    a.p_out_slot = a.q_out_slot = a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
-    a.hist_slot0 = hist ? n : -1;
-    a.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
-    launch_fwd(c, tan, p->tl[kind].grid, s, a);
+    const bool tail = c.strips == 2 && K < c.k;
+    if (tail) K = std::min(K, cr.k);
+    FwdArgs& f = tail ? ar : a;
+    f.write_seis = 1;
+    f.n0 = n;
+    f.K = K;
+    f.p_in_slot = f.q_in_slot = f.dp_in_slot = f.dq_in_slot = li & 1;
+    f.p_out_slot = f.q_out_slot = f.dp_out_slot = f.dq_out_slot = (li + 1) & 1;
+    f.hist_slot0 = hist ? n : -1;
+    f.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
+    if (tail) launch_fwd(cr, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
+    else launch_fwd(c, tan, p->tl[kind].grid, s, f);
     n += K;
     ++li;
   }
@@ -1686,6 +1942,9 @@ int seis_plan_set_config(seis_plan_t* p, int32_t kind, int32_t rx, int32_t rz, i
   if (!cfg_ok(kind, c))
     return fail(SEIS_EINVAL, "unsupported config: window " + std::to_string(rx) + "x" + std::to_string(rz) +
                                  ", k=" + std::to_string(k) + ", strips=" + std::to_string(strips));
+  if (c.strips == 2)
+    for (const WaveDev& w : p->L.waves)
+      if (w.NZ < 4 * c.k) return fail(SEIS_EINVAL, "streamed forward needs padded grids of at least 4k rows");
   HIP_TRY(hipDeviceSynchronize());
   drop_graphs(p);
   p->cfg[kind] = c;

@@ -163,6 +163,62 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
         assert np.array_equal(_np(gr), _np(gref)), ckpt
 
 
+STREAM_CONFIGS = [(128, 0, 4, 2), (128, 0, 3, 2), (192, 0, 4, 2), (256, 0, 3, 2), (128, 8, 4, 2),
+                  (192, 23, 3, 2)]
+
+
+@pytest.mark.parametrize("cfg", STREAM_CONFIGS)
+def test_streamed_forward_bit_identical(small_geoms, cfg):
+    """The row-streamed forward (column strips, one thread per column, whole
+    columns or row segments) gives the seismograms, misfits and wavefields of
+    the tiled kernel: plain forward, forward + misfit, and the checkpointing
+    saving forward of a gradient (whose last launches and checkpoint-bounded
+    launches run the tiled kernel)."""
+    g, og = small_geoms
+    v, vec = _case(g, "faulted", 23)
+    target = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 1.01), og)[0])
+    for ckpt in (0, 17):
+        P = Propagator([g], ckpt_every=ckpt)
+        ref = _np(P.forward(torch.tensor(vec)))
+        mref = float(P.forward_misfit(torch.tensor(vec), target)[0])
+        _, _, gref = P.gradient(torch.tensor(vec), target)
+        P.set_config("forward", *cfg)
+        assert P.get_config("forward") == cfg
+        out = _np(P.forward(torch.tensor(vec)))
+        print(cfg, ckpt, "rel to tiled", rel(out, ref), flush=True)
+        assert np.array_equal(out, ref), (cfg, ckpt)
+        assert float(P.forward_misfit(torch.tensor(vec), target)[0]) == mref
+        _, _, gr = P.gradient(torch.tensor(vec), target)
+        assert np.array_equal(_np(gr), _np(gref)), (cfg, ckpt)
+    ref_o = fd2d.vel_to_seis(vec, og)[0]
+    assert rel(out, ref_o) < SEIS_TOL
+
+
+def test_streamed_forward_ragged_batch():
+    """Mixed grid shapes, shot counts and source / receiver rows (incl. the
+    surface row) in one streamed launch, against the tiled kernel; the
+    streamed config is refused for padded grids under 4k rows."""
+    kw = dict(nbc=8, nt=170)
+    geoms = [Geometry(nz=12, nx=16, src_x=(7,), src_z=0, rcv_z=3, rcv_x=(2, 5, 8, 11, 14), **kw),
+             Geometry(nz=20, nx=10, src_x=(0, 4, 9), src_z=2, rcv_z=0, rcv_x=(0, 3, 6, 9), **kw),
+             Geometry(nz=8, nx=30, n_shots=7, src_z=5, rcv_z=7, **kw),
+             Geometry(nz=40, nx=170, n_shots=3, **kw)]
+    rng = np.random.default_rng(4)
+    vec = np.concatenate([synth.to_vec(np.linspace(1800, 3600, gg.nz)[:, None] * np.ones((1, gg.nx))
+                                       + rng.uniform(-100, 100, (gg.nz, gg.nx))) for gg in geoms])
+    P = Propagator(geoms, ckpt_every=0)
+    ref = _np(P.forward(torch.tensor(vec)))
+    for cfg in STREAM_CONFIGS:
+        P.set_config("forward", *cfg)
+        assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref), cfg
+    Ps = Propagator([Geometry(nz=2, nx=5, nbc=6, nt=150)])
+    Ps.set_config("forward", 128, 0, 3, 2)      # 14 padded rows >= 12
+    with pytest.raises(ValueError):
+        Ps.set_config("forward", 128, 0, 4, 2)  # 14 < 16
+    with pytest.raises(ValueError):
+        P.set_config("record", 128, 0, 4, 2)    # streamed: plain forward kind only
+
+
 @pytest.mark.parametrize("cfg", [(64, 32, 1, 1), (128, 64, 4, 0), (256, 32, 3, 1), (64, 64, 8, 0)])
 def test_born_saved_matches_fused(small_geoms, cfg):
     """Tangent and GN product from the saved forward are bit-identical to the
