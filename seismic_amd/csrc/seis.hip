@@ -465,31 +465,52 @@ __device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float ksca
 // KL forward time steps (update_p :428-462, lapg :381-421, source injection
 // fused) over a column strip of NT padded columns, one per thread, marched
 // down the rows of a segment in a software pipeline.  Pipeline step t takes
-// in window row t of p[n0], q[n0] and alpha (requested SPD steps earlier) as
-// level 0, and level l = 1..KL computes window row t - 2l of p[n0+l], q[n0+l]
-// from level l-1; levels run in increasing order, so the lower neighbour two
-// rows down is the value level l-1 produced in the same step.  Each thread
-// keeps the last rows of its own column of every level in registers (centre,
-// vertical neighbours, q) and alpha of the rows in flight in a ring, and
-// reads the horizontal neighbours from a three-row LDS ring per level, with
-// one barrier per pipeline step; 1 - kappa outside the left/right damping
-// strips, one value per padded row, comes from the model's row table with
-// the row's other inputs and rides in a ring like alpha.
-// The window is the segment's rows (TileEnt.pad of them) plus 2*KL rows above
-// and below and 2*KL columns either side, wrapped periodically like k_fwd's
-// tiles; cells outside the valid band compute values no committed cell
-// depends on.  Pipeline steps that meet the source row or the receiver row
-// run a separate copy of the step body, so the others carry no per-cell
-// source or receiver work; receiver samples are kept in registers and stored
-// after the march.  Same stencil order and fused multiply-adds as k_fwd, so
-// wavefields and seismograms are bit-identical to it.  Launches always
-// advance KL steps (a shorter last launch runs k_fwd).
+// in window row t of p[n0], q[n0], alpha and the row's 1 - kappa (requested
+// SPD steps earlier) as level 0, and level l = 1..KL computes window row
+// t - 2l of p[n0+l], q[n0+l] from level l-1; levels run in increasing order,
+// so the lower neighbour two rows down is the value level l-1 produced in
+// the same step.  Each thread keeps the last rows of its own column of every
+// level in registers (centre, vertical neighbours, q) and reads the
+// horizontal neighbours from a three-row LDS ring per level, with one
+// barrier per pipeline step; the (alpha, 1 - kappa) pairs of its column's
+// rows in flight sit in a private LDS ring.  1 - kappa outside the
+// left/right damping strips, one value per padded row, comes from the
+// model's row table.  The window is the segment's rows (TileEnt.pad of them)
+// plus 2*KL rows above and below and 2*KL columns either side, wrapped
+// periodically like k_fwd's tiles; cells outside the valid band compute
+// values no committed cell depends on.  Blocks of SUN pipeline steps in
+// which some level meets the source row or the receiver row run a separate
+// copy of the block, so the others carry no per-cell source or receiver
+// work; receiver samples are kept in registers and stored after the march.  Same stencil order and
+// fused multiply-adds as k_fwd, so wavefields and seismograms are
+// bit-identical to it.  Launches always advance KL steps (a shorter last
+// launch runs k_fwd).
+// Row pointer base + ro (bytes), kept in scalar registers and opaque to the
+// compiler, and element of a row at a 32-bit lane byte offset: every access
+// of the streamed kernel is then one scalar-based address (saddr form)
+// instead of a hoisted 64-bit per-lane pointer plus a 64-bit add per step.
+typedef __attribute__((address_space(1))) float gfloat;   // global memory
+__device__ __forceinline__ gfloat* rowp(const float* base, int64_t ro) {
+  uint64_t v = reinterpret_cast<uint64_t>(base) + (uint64_t)ro;
+  asm volatile("" : "+s"(v));
+  return reinterpret_cast<gfloat*>(v);
+}
+__device__ __forceinline__ gfloat* lane(gfloat* row, unsigned off) {
+  return reinterpret_cast<gfloat*>(reinterpret_cast<__attribute__((address_space(1))) char*>(row) + off);
+}
+
 template <int NT, int KL>
 __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
   static_assert(2 * KL + SPD < SUN && SUN % 6 == 0 && SUN % SPD == 0, "pipeline rings too short");
-  constexpr int LW = NT + 4;   // two apron cells either side
   constexpr int H = 2 * KL;
-  __shared__ float RG[KL][3][LW];
+  // rows of the level rings, column-major: window column x (two apron
+  // columns either side) holds its KL*3 ring slots at [(x + 2) * RS], so a
+  // thread reaches every slot of its own and its neighbours' columns from one
+  // base address with immediate offsets; RS is odd, so the 64 lanes of a
+  // wavefront (stride RS) fall in distinct banks
+  constexpr int RS = 3 * KL + 1;
+  __shared__ float RG[(NT + 4) * RS];
+  __shared__ float2 AK[SUN][NT];  // (alpha, 1 - kappa) of window row u at [u % SUN], own column
   const unsigned c = threadIdx.x;
   const int ntiles = *a.n_tiles;
   const int n0 = a.n0, nt = a.nt;
@@ -516,6 +537,11 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const int U = IZ + 2 * H;           // window rows
     const int oz = te.z0 - H, ox = te.x0 - H;
     const unsigned gx = wrapi(ox + (int)c, NX);
+    // byte offset of the thread's column in a row, opaque to the compiler so
+    // that every access is (uniform row pointer) + (32-bit lane offset)
+    unsigned gx4 = gx * 4u;
+    asm volatile("" : "+v"(gx4));
+    const int64_t rowb = (int64_t)NX * 4;
     const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
     const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
     const int jx = damp_j(0, gx, nz, nx, nbc);
@@ -524,7 +550,7 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const int srow = w.src_cell / NX;
     const bool srcc = (int)gx == w.src_cell - srow * NX;
     // window rows holding the source row (twice when the window wraps), and
-    // the segment row holding the receiver row (-1 if none)
+    // the segment row holding the receiver row (none: far out of range)
     const int su0 = wrapi(srow - oz, NZ), su1 = su0 + NZ;
     const int ruz = rcv_row - te.z0;
     const int rui = ruz >= 0 && ruz < IZ ? ruz + H : -(1 << 20);
@@ -535,7 +561,7 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
       srcv[l - 1] = amp * ((it >= 0 && it < nt) ? a.wavelet[it] : 0.f);
       rec[l - 1] = 0.f;
     }
-    float pr[KL][6], qr[KL][3], ar[SUN], kr[SUN], pfp[SPD], pfq[SPD];
+    float pr[KL][6], qr[KL][3], pfp[SPD], pfq[SPD], pfa[SPD], pfk[SPD];
 #pragma unroll
     for (int l = 0; l < KL; ++l) {
 #pragma unroll
@@ -544,14 +570,12 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
       for (int s = 0; s < 3; ++s) qr[l][s] = 0.f;
     }
 #pragma unroll
-    for (int s = 0; s < SUN; ++s) ar[s] = kr[s] = 0.f;
-#pragma unroll
     for (int d = 0; d < SPD; ++d) {
-      const int64_t ro = (int64_t)wrapi(oz + d, NZ) * NX;
-      pfp[d] = (pin + ro)[gx];
-      pfq[d] = (qin + ro)[gx];
-      ar[d] = (alpha + ro)[gx];
-      kr[d] = kzt[oz + d];
+      const int64_t ro = (int64_t)wrapi(oz + d, NZ) * rowb;
+      pfp[d] = *lane(rowp(pin, ro), gx4);
+      pfq[d] = *lane(rowp(qin, ro), gx4);
+      pfa[d] = *lane(rowp(alpha, ro), gx4);
+      pfk[d] = kzt[oz + d];
     }
     int gzn = wrapi(oz + SPD, NZ);      // padded row requested next
     __syncthreads();
@@ -561,9 +585,10 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
       constexpr bool SPECIAL = decltype(sc)::value;
       {  // level 0: intake of window row t
         const float p0 = pfp[i % SPD];
-        RG[0][i % 3][c + 2] = p0;
+        RG[(c + 2) * RS + i % 3] = p0;
         pr[0][i % 6] = p0;
         qr[0][i % 3] = pfq[i % SPD];
+        AK[i][c] = make_float2(pfa[i % SPD], isx ? kx1 : pfk[i % SPD]);
       }
       static_for<1, KL + 1>([&](auto lc) {
         constexpr int l = decltype(lc)::value;
@@ -571,16 +596,16 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
         constexpr int B = 4 * SUN;                    // keeps the ring indices non-negative
         constexpr int s6 = (i - 2 * l + B) % 6, s3 = (i - 2 * l + B) % 3, sr = (i - 2 * l + B) % SUN;
         const float pc = pr[l - 1][s6];
-        const float (*T)[LW] = RG[l - 1];
-        float t1 = (T[s3][c + 3] - pc) + (T[s3][c + 1] - pc);
+        const float* T = RG + c * RS + (l - 1) * 3 + s3;   // T[k * RS]: window column c + k - 2
+        float t1 = (T[3 * RS] - pc) + (T[RS] - pc);
         t1 = t1 + (pr[l - 1][(i - 2 * l + 1 + B) % 6] - pc);
         t1 = t1 + (pr[l - 1][(i - 2 * l - 1 + B) % 6] - pc);
-        float t2 = (T[s3][c + 4] - pc) + (T[s3][c] - pc);
+        float t2 = (T[4 * RS] - pc) + (T[0] - pc);
         t2 = t2 + (pr[l - 1][(i - 2 * l + 2 + B) % 6] - pc);
         t2 = t2 + (pr[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
         const float L = fmaf(kC2, t1, kC3 * t2);
-        const float k1 = isx ? kx1 : kr[sr];
-        float qn = fmaf(k1, qr[l - 1][s3], ar[sr] * L);
+        const float2 ak = AK[sr][c];
+        float qn = fmaf(ak.y, qr[l - 1][s3], ak.x * L);
         if constexpr (SPECIAL) {
           if (u == su0 || u == su1) qn = srcc ? qn + srcv[l - 1] : qn;
         }
@@ -589,41 +614,44 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
           if (u == rui) rec[l - 1] = p1;
         }
         if constexpr (l < KL) {
-          RG[l][s3][c + 2] = p1;
+          RG[(c + 2) * RS + l * 3 + s3] = p1;
           pr[l][s6] = p1;
           qr[l][s3] = qn;
-        } else if (u >= H && u < H + IZ && colint) {  // committed cell of p[n0+KL], q[n0+KL]
-          const int64_t ro = (int64_t)(oz + u) * NX;
-          (pout + ro)[gx] = p1;
-          (qout + ro)[gx] = qn;
-          if (ckp) {
-            (ckp + ro)[gx] = p1;
-            (ckq + ro)[gx] = qn;
+        } else if (u >= H && u < H + IZ) {  // committed row of p[n0+KL], q[n0+KL]
+          const int64_t ro = (int64_t)(oz + u) * rowb;
+          gfloat* const op = rowp(pout, ro);
+          gfloat* const oq = rowp(qout, ro);
+          gfloat* const cp = ckp ? rowp(ckp, ro) : nullptr;
+          gfloat* const cq = ckp ? rowp(ckq, ro) : nullptr;
+          if (colint) {
+            *lane(op, gx4) = p1;
+            *lane(oq, gx4) = qn;
+            if (cp) {
+              *lane(cp, gx4) = p1;
+              *lane(cq, gx4) = qn;
+            }
           }
         }
       });
       {  // request of window row t + SPD
-        const int64_t ro = (int64_t)gzn * NX;
-        pfp[i % SPD] = (pin + ro)[gx];
-        pfq[i % SPD] = (qin + ro)[gx];
-        ar[(i + SPD) % SUN] = (alpha + ro)[gx];
-        kr[(i + SPD) % SUN] = kzt[oz + t + SPD];        // oz + t + SPD in [-H, NZ + H + SUN)
+        gfloat* const rp = rowp(pin, (int64_t)gzn * rowb);
+        gfloat* const rq = rowp(qin, (int64_t)gzn * rowb);
+        gfloat* const ra = rowp(alpha, (int64_t)gzn * rowb);
+        pfp[i % SPD] = *lane(rp, gx4);
+        pfq[i % SPD] = *lane(rq, gx4);
+        pfa[i % SPD] = *lane(ra, gx4);
+        pfk[i % SPD] = kzt[oz + t + SPD];             // oz + t + SPD in [-H, NZ + H + SUN)
         gzn = gzn + 1 == NZ ? 0 : gzn + 1;
       }
       __syncthreads();
     };
-    auto hits = [&](int t, int r) {  // does some level 1..KL compute window row r at step t
-      const int d = t - r;
-      return d >= 2 && d <= 2 * KL && !(d & 1);
-    };
+    // does some level 1..KL compute window row r in steps tb .. tb+SUN-1
+    auto meets = [&](int tb, int r) { return r + 2 <= tb + SUN - 1 && r + 2 * KL >= tb; };
     for (int tb = 0; tb < U; tb += SUN) {
-      static_for<0, SUN>([&](auto ic) {
-        const int t = tb + decltype(ic)::value;
-        if (hits(t, su0) || hits(t, su1) || hits(t, rui))
-          body(ic, std::true_type(), t);
-        else
-          body(ic, std::false_type(), t);
-      });
+      if (meets(tb, su0) || meets(tb, su1) || meets(tb, rui))
+        static_for<0, SUN>([&](auto ic) { body(ic, std::true_type(), tb + decltype(ic)::value); });
+      else
+        static_for<0, SUN>([&](auto ic) { body(ic, std::false_type(), tb + decltype(ic)::value); });
     }
     if (seis && rui >= H && colint && rc >= 0) {
 #pragma unroll
