@@ -3,7 +3,8 @@
 Each kernel kind (forward, forward recording the history, fused tangent,
 adjoint, Born on the saved forward) runs an rx x rz cell window per workgroup, advances k time steps
 per launch and deals window rows to wavefronts either as consecutive strips
-or round-robin (libseis ``seis_plan_set_config``).  The best choice depends on how many cells one
+or round-robin (libseis ``seis_plan_set_config``); the plain forward can
+also march rx-column strips down rz-row segments (strips = 2).  The best choice depends on how many cells one
 time step covers (batch size x model size: a single 70x70 model leaves most
 of the GPU idle with large windows) and on whether the forward writes the full
 history.  ``tune`` times every candidate on a live plan; ``apply`` looks a
@@ -40,6 +41,9 @@ _WINDOWS["record"] = [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128
                       (256, 32, 3), (128, 96, 6), (128, 128, 6)]
 # (rx, rz, k, strips): every window with both row ownerships
 CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WINDOWS.items()}
+# row-streamed forward (strips = 2): rx-column strips, rz-row segments (0:
+# whole columns), k levels
+CANDIDATES["forward"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 40, 80, 160) for k in (3, 4)]
 # tuning order: the adjoint and Born passes run after the forward that
 # records their history
 KINDS = ("forward", "record", "tangent", "adjoint", "born")
@@ -166,7 +170,7 @@ def main():
         ("batch16-ckpt", [g70] * 16, 50, 7),
         ("batch64-ckpt", [g70] * 64, 50, 7),
         ("serve-mix", [sizes[0]] * 12 + [sizes[1]] * 8 + [sizes[2]] * 4, 50, 7),
-        ("linesearch-fanout", [g70] * 800, 0, CAP_FORWARD),
+        ("linesearch-fanout", [g70] * 1600, 0, CAP_FORWARD),
         ("linesearch-plain", [g70], 0, CAP_FORWARD),
     ]
     table = load_table(args.out)
