@@ -835,6 +835,195 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
   }
 }
 
+// KL steps of the Born recursion (update_p_diff :470-509) on the saved
+// history, streamed like k_fws: a column strip of NT padded columns (one per
+// thread) marched down the rows of a segment; pipeline step t takes in
+// window row t of dp[n0], dq[n0] and the row's coefficients (alpha, dalpha,
+// 1 - kappa, dkappa) as level 0, and level l = 1..KL computes window row
+// t - 2l of (dp, dq)[n0+l] from level l-1 and the pair (Lf(p[n]), q[n]) of
+// n = n0+l-1 of that row, requested SPH steps ahead.  The coefficients of a
+// row sit in a private LDS ring of SUNB slots, written after the step's
+// levels have read the slot's previous row (the last level reads row t-2KL).
+// Same stencil order and fused multiply-add sequence as k_born, so dseis and
+// (dp, dq) are bit-identical to it (and to the fused tangent).  Launches
+// always advance KL steps (a shorter last launch runs k_born).
+constexpr int SPB = 3;    // steps between a level-0 row's request and its intake
+constexpr int SPH = 2;    // steps between a history row's request and its use
+template <int NT, int KL>
+__global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
+  // pipeline steps per unrolled block, also the coefficient ring length
+  constexpr int SUNB = 2 * KL <= 6 ? 6 : 12;
+  static_assert(2 * KL <= SUNB && SUNB % 6 == 0 && SUNB % SPB == 0 && SUNB % SPH == 0, "pipeline rings");
+  constexpr int H = 2 * KL;
+  constexpr int RS = 3 * KL + 1;          // column-major level rings, odd stride (see k_fws)
+  __shared__ float RG[(NT + 4) * RS];
+  __shared__ float4 CF[SUNB][NT];         // (alpha, dalpha, 1 - kappa, dkappa) of window row u at [u % SUNB]
+  const unsigned c = threadIdx.x;
+  const int ntiles = *a.n_tiles;
+  const int n0 = a.n0, nt = a.nt;
+  for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    const TileEnt te = a.tiles[e];
+    if (te.w < 0) continue;
+    const WaveDev& w = a.waves[te.w];
+    const ModelDev& m = a.models[w.model];
+    const int NZ = w.NZ, NX = w.NX, nz = m.nz, nx = m.nx, nbc = m.nbc;
+    const int64_t Pn = (int64_t)NZ * NX;
+    const float* dpin = a.arena + w.off[R_DP] + a.dp_in_slot * Pn;
+    const float* dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
+    float* dpout = a.arena + w.off[R_DP] + a.dp_out_slot * Pn;
+    float* dqout = a.arena + w.off[R_DQ] + a.dq_out_slot * Pn;
+    const float* hist = a.arena + w.off[R_HIST];   // float2 pairs, slot n at 2 * n * Pn floats
+    float* dseis = a.dseis + w.seis_off;
+    const float* alpha = a.alpha + m.cell_off;
+    const float* dalpha = a.dalpha + m.cell_off;
+    const float kscale = a.params[w.model].kscale;
+    const float dkscale = a.params[w.model].dkscale;
+    const float amp_d = a.wparam[te.w].amp_d;
+    const int n_rcv = m.n_rcv, rcv_row = m.rcv_row;
+    const int IZ = te.pad;
+    const int U = IZ + 2 * H;
+    const int oz = te.z0 - H, ox = te.x0 - H;
+    const unsigned gx = wrapi(ox + (int)c, NX);
+    unsigned gx4 = gx * 4u, gx8 = gx * 8u;
+    asm volatile("" : "+v"(gx4), "+v"(gx8));
+    const int64_t rowb = (int64_t)NX * 4;
+    const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
+    const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
+    const int jx = damp_j(0, gx, nz, nx, nbc);
+    const float jjx = (float)(jx * jx);
+    const int rc = rcv_index(gx, m);
+    const int srow = w.src_cell / NX;
+    const bool srcc = (int)gx == w.src_cell - srow * NX;
+    const int su0 = wrapi(srow - oz, NZ), su1 = su0 + NZ;
+    const int ruz = rcv_row - te.z0;
+    const int rui = ruz >= 0 && ruz < IZ ? ruz + H : -(1 << 20);
+    float srcv[KL], rec[KL];
+#pragma unroll
+    for (int l = 1; l <= KL; ++l) {
+      const int it = n0 + l - 2;
+      srcv[l - 1] = amp_d * ((it >= 0 && it < nt) ? a.wavelet[it] : 0.f);
+      rec[l - 1] = 0.f;
+    }
+    float pr[KL][6], qr[KL][3], pfp[SPB], pfq[SPB], pfa[SPB], pfd[SPB];
+    float2 hpf[KL][SPH];
+#pragma unroll
+    for (int l = 0; l < KL; ++l) {
+#pragma unroll
+      for (int s = 0; s < 6; ++s) pr[l][s] = 0.f;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) qr[l][s] = 0.f;
+#pragma unroll
+      for (int s = 0; s < SPH; ++s) hpf[l][s] = make_float2(0.f, 0.f);
+    }
+    // coefficients of window row u into the ring slot u % SUNB
+    auto put_cf = [&](int slot, int u, float al, float dal) {
+      const int jz = damp_j(wrapi(oz + u, NZ), nbc, nz, nx, nbc);
+      const float jj = isx ? jjx : (float)(jz * jz);
+      CF[slot][c] = make_float4(al, dal, 1.0f - kscale * jj, dkscale * jj);
+    };
+    auto hist_row = [&](int l, int u) {  // pair of level l, window row u (wrapped padded row)
+      const int64_t ro = (int64_t)(n0 + l - 1) * Pn * 8 + (int64_t)wrapi(oz + u, NZ) * rowb * 2;
+      const gfloat* hp = lane(rowp(hist, ro), gx8);
+      return make_float2(hp[0], hp[1]);
+    };
+#pragma unroll
+    for (int d = 0; d < SPB; ++d) {
+      const int64_t ro = (int64_t)wrapi(oz + d, NZ) * rowb;
+      pfp[d] = *lane(rowp(dpin, ro), gx4);
+      pfq[d] = *lane(rowp(dqin, ro), gx4);
+      pfa[d] = *lane(rowp(alpha, ro), gx4);
+      pfd[d] = *lane(rowp(dalpha, ro), gx4);
+    }
+    // history of the first steps: level l meets window row t - 2l at step t
+#pragma unroll
+    for (int d = 0; d < SPH; ++d) {
+#pragma unroll
+      for (int l = 1; l <= KL; ++l) hpf[l - 1][d] = hist_row(l, d - 2 * l);
+    }
+    int gzn = wrapi(oz + SPB, NZ);      // padded row of the next level-0 request
+    __syncthreads();
+    auto body = [&](auto ic, auto sc, const int t) {
+      constexpr int i = decltype(ic)::value;
+      constexpr bool SPECIAL = decltype(sc)::value;
+      float al0 = 0.f, dal0 = 0.f;
+      {  // level 0: intake of window row t (its coefficients go to the ring after the levels)
+        const float p0 = pfp[i % SPB];
+        RG[(c + 2) * RS + i % 3] = p0;
+        pr[0][i % 6] = p0;
+        qr[0][i % 3] = pfq[i % SPB];
+        al0 = pfa[i % SPB];
+        dal0 = pfd[i % SPB];
+      }
+      static_for<1, KL + 1>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        const int u = t - 2 * l;
+        constexpr int B = 4 * SUNB;
+        constexpr int s6 = (i - 2 * l + B) % 6, s3 = (i - 2 * l + B) % 3, sr = (i - 2 * l + B) % SUNB;
+        const float pc = pr[l - 1][s6];
+        const float* T = RG + c * RS + (l - 1) * 3 + s3;
+        float t1 = (T[3 * RS] - pc) + (T[RS] - pc);
+        t1 = t1 + (pr[l - 1][(i - 2 * l + 1 + B) % 6] - pc);
+        t1 = t1 + (pr[l - 1][(i - 2 * l - 1 + B) % 6] - pc);
+        float t2 = (T[4 * RS] - pc) + (T[0] - pc);
+        t2 = t2 + (pr[l - 1][(i - 2 * l + 2 + B) % 6] - pc);
+        t2 = t2 + (pr[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
+        const float Ld = fmaf(kC2, t1, kC3 * t2);
+        const float4 cf = CF[sr][c];
+        const float2 hc = hpf[l - 1][i % SPH];
+        float dqn = fmaf(cf.z, qr[l - 1][s3], cf.x * Ld);
+        dqn = fmaf(cf.y, hc.x, dqn);
+        dqn = fmaf(-cf.w, hc.y, dqn);
+        if constexpr (SPECIAL) {
+          if (u == su0 || u == su1) dqn = srcc ? dqn + srcv[l - 1] : dqn;
+        }
+        const float dp1 = pc + dqn;
+        if constexpr (SPECIAL) {
+          if (u == rui) rec[l - 1] = dp1;
+        }
+        if constexpr (l < KL) {
+          RG[(c + 2) * RS + l * 3 + s3] = dp1;
+          pr[l][s6] = dp1;
+          qr[l][s3] = dqn;
+        } else if (u >= H && u < H + IZ) {  // committed row of (dp, dq)[n0+KL]
+          const int64_t ro = (int64_t)(oz + u) * rowb;
+          gfloat* const op = rowp(dpout, ro);
+          gfloat* const oq = rowp(dqout, ro);
+          if (colint) {
+            *lane(op, gx4) = dp1;
+            *lane(oq, gx4) = dqn;
+          }
+        }
+        // this level's pair for its row SPH steps ahead
+        hpf[l - 1][i % SPH] = hist_row(l, u + SPH);
+      });
+      put_cf(i % SUNB, t, al0, dal0);
+      {  // request of window row t + SPB
+        const int64_t ro = (int64_t)gzn * rowb;
+        pfp[i % SPB] = *lane(rowp(dpin, ro), gx4);
+        pfq[i % SPB] = *lane(rowp(dqin, ro), gx4);
+        pfa[i % SPB] = *lane(rowp(alpha, ro), gx4);
+        pfd[i % SPB] = *lane(rowp(dalpha, ro), gx4);
+        gzn = gzn + 1 == NZ ? 0 : gzn + 1;
+      }
+      __syncthreads();
+    };
+    auto meets = [&](int tb, int r) { return r + 2 <= tb + SUNB - 1 && r + 2 * KL >= tb; };
+    for (int tb = 0; tb < U; tb += SUNB) {
+      if (meets(tb, su0) || meets(tb, su1) || meets(tb, rui))
+        static_for<0, SUNB>([&](auto ic) { body(ic, std::true_type(), tb + decltype(ic)::value); });
+      else
+        static_for<0, SUNB>([&](auto ic) { body(ic, std::false_type(), tb + decltype(ic)::value); });
+    }
+    if (rui >= H && colint && rc >= 0) {
+#pragma unroll
+      for (int l = 1; l <= KL; ++l) {
+        const int it = n0 + l - 2;
+        if (it >= 0 && it < nt) dseis[(int64_t)it * n_rcv + rc] = rec[l - 1];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- adjoint
 struct AdjArgs {
   const TileEnt* tiles;
@@ -1293,14 +1482,20 @@ const Shape kShapes[SEIS_KIND_COUNT][7] = {
 
 // Streamed forward (strips = 2): rx-column strips, one thread per column, k
 // levels per launch, rz rows per segment (0: the whole column).
-constexpr int kStreamNT[3] = {128, 192, 256};
+constexpr int kStreamNT[4] = {64, 128, 192, 256};
 constexpr int kStreamKMin = 3, kStreamKMax = 4;
 
+// Streamed Born (strips = 2): the same strips, 3 or 4 levels per launch.
+constexpr int kStreamBornKMin = 3, kStreamBornKMax = 4;
+
 bool stream_cfg(int kind, const KCfg& c) {
-  if (kind != SEIS_KIND_FORWARD || c.strips != 2) return false;
+  if (c.strips != 2) return false;
   bool nt_ok = false;
   for (int nt : kStreamNT) nt_ok = nt_ok || c.rx == nt;
-  return nt_ok && c.k >= kStreamKMin && c.k <= kStreamKMax && (c.rz == 0 || c.rz >= 8);
+  if (!nt_ok || !(c.rz == 0 || c.rz >= 8)) return false;
+  if (kind == SEIS_KIND_FORWARD) return c.k >= kStreamKMin && c.k <= kStreamKMax;
+  if (kind == SEIS_KIND_BORN) return c.k >= kStreamBornKMin && c.k <= kStreamBornKMax;
+  return false;
 }
 
 int shape_threads(int kind, int rx, int rz) {
@@ -1506,13 +1701,26 @@ const void* fws_ptr_nt(int k) {
 }
 
 const void* fws_ptr(const KCfg& c) {
+  if (c.rx == 64) return fws_ptr_nt<64>(c.k);
   if (c.rx == 128) return fws_ptr_nt<128>(c.k);
   if (c.rx == 192) return fws_ptr_nt<192>(c.k);
   return fws_ptr_nt<256>(c.k);
 }
 
+template <int NT>
+const void* bns_ptr_nt(int k) {
+  return k == 3 ? (const void*)k_bns<NT, 3> : (const void*)k_bns<NT, 4>;
+}
+
+const void* bns_ptr(const KCfg& c) {
+  if (c.rx == 64) return bns_ptr_nt<64>(c.k);
+  if (c.rx == 128) return bns_ptr_nt<128>(c.k);
+  if (c.rx == 192) return bns_ptr_nt<192>(c.k);
+  return bns_ptr_nt<256>(c.k);
+}
+
 const void* kernel_ptr(int kind, const KCfg& c) {
-  if (c.strips == 2) return fws_ptr(c);
+  if (c.strips == 2) return kind == SEIS_KIND_BORN ? bns_ptr(c) : fws_ptr(c);
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
 #define PTR_R(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, true>(c);
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
@@ -1553,7 +1761,8 @@ void launch_fws_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
 
 void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
   if (c.strips == 2) {
-    if (c.rx == 128) launch_fws_t<128>(c, grid, s, a);
+    if (c.rx == 64) launch_fws_t<64>(c, grid, s, a);
+    else if (c.rx == 128) launch_fws_t<128>(c, grid, s, a);
     else if (c.rx == 192) launch_fws_t<192>(c, grid, s, a);
     else launch_fws_t<256>(c, grid, s, a);
     return;
@@ -1584,7 +1793,20 @@ void launch_born_t(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
   else hipLaunchKernelGGL((k_born<RX, RZ, NT, false>), dim3(grid), dim3(NT), 0, s, a);
 }
 
+template <int NT>
+void launch_bns_t(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
+  if (c.k == 3) hipLaunchKernelGGL((k_bns<NT, 3>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_bns<NT, 4>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 void launch_born(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
+  if (c.strips == 2) {
+    if (c.rx == 64) launch_bns_t<64>(c, grid, s, a);
+    else if (c.rx == 128) launch_bns_t<128>(c, grid, s, a);
+    else if (c.rx == 192) launch_bns_t<192>(c, grid, s, a);
+    else launch_bns_t<256>(c, grid, s, a);
+    return;
+  }
 #define LB(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_born_t<RX, RZ, NT>(c, grid, s, a); return; }
   SEIS_BORN_SHAPES(LB)
 #undef LB
@@ -1731,14 +1953,27 @@ void enqueue_born(seis_plan* p, hipStream_t s) {
   a.alpha = p->d_alpha; a.dalpha = p->d_dalpha; a.wavelet = p->d_wavelet;
   a.arena = p->d_arena; a.dseis = p->d_dseis; a.nt = p->nt;
   a.Kt = c.k;
+  // a streamed Born pass always advances c.k steps; the shorter last launch
+  // runs k_born with the adjoint kind's window (a Born shape too)
+  const KCfg& ct = p->cfg[SEIS_KIND_ADJOINT];
   int n = 0, li = 0;
   while (n < p->nt + 1) {
-    const int K = std::min(c.k, p->nt + 1 - n);
+    int K = std::min(c.k, p->nt + 1 - n);
+    const bool tail = c.strips == 2 && K < c.k;
+    if (tail) K = std::min(K, ct.k);
     a.n0 = n;
     a.K = K;
     a.dp_in_slot = a.dq_in_slot = li & 1;
     a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
-    launch_born(c, p->tl[SEIS_KIND_BORN].grid, s, a);
+    if (tail) {
+      BornArgs at = a;
+      at.tiles = p->tl[SEIS_KIND_ADJOINT].d;
+      at.n_tiles = p->tl[SEIS_KIND_ADJOINT].d_count;
+      at.Kt = ct.k;
+      launch_born(ct, p->tl[SEIS_KIND_ADJOINT].grid, s, at);
+    } else {
+      launch_born(c, p->tl[SEIS_KIND_BORN].grid, s, a);
+    }
     n += K;
     ++li;
   }

@@ -163,7 +163,7 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
         assert np.array_equal(_np(gr), _np(gref)), ckpt
 
 
-STREAM_CONFIGS = [(128, 0, 4, 2), (128, 0, 3, 2), (192, 0, 4, 2), (256, 0, 3, 2), (128, 8, 4, 2),
+STREAM_CONFIGS = [(128, 0, 4, 2), (128, 0, 3, 2), (192, 0, 4, 2), (256, 0, 3, 2), (128, 8, 4, 2), (64, 0, 3, 2), (64, 13, 4, 2),
                   (192, 23, 3, 2)]
 
 
@@ -216,10 +216,41 @@ def test_streamed_forward_ragged_batch():
     with pytest.raises(ValueError):
         Ps.set_config("forward", 128, 0, 4, 2)  # 14 < 16
     with pytest.raises(ValueError):
-        P.set_config("record", 128, 0, 4, 2)    # streamed: plain forward kind only
+        P.set_config("record", 128, 0, 4, 2)    # streamed: plain forward and Born kinds only
+    with pytest.raises(ValueError):
+        P.set_config("born", 128, 0, 5, 2)      # streamed Born: 3 or 4 levels
 
 
-@pytest.mark.parametrize("cfg", [(64, 32, 1, 1), (128, 64, 4, 0), (256, 32, 3, 1), (64, 64, 8, 0)])
+def test_streamed_born_ragged_batch():
+    """Streamed Born over mixed grids, shot counts and source / receiver rows
+    equals the tiled Born (and the fused tangent), also with models inactive."""
+    kw = dict(nbc=8, nt=170)
+    geoms = [Geometry(nz=12, nx=16, src_x=(7,), src_z=0, rcv_z=3, rcv_x=(2, 5, 8, 11, 14), **kw),
+             Geometry(nz=20, nx=10, src_x=(0, 4, 9), src_z=2, rcv_z=0, rcv_x=(0, 3, 6, 9), **kw),
+             Geometry(nz=8, nx=30, n_shots=7, src_z=5, rcv_z=7, **kw),
+             Geometry(nz=40, nx=170, n_shots=3, **kw)]
+    rng = np.random.default_rng(6)
+    vec = np.concatenate([synth.to_vec(np.linspace(1800, 3600, gg.nz)[:, None] * np.ones((1, gg.nx))
+                                       + rng.uniform(-100, 100, (gg.nz, gg.nx))) for gg in geoms])
+    dv = np.concatenate([np.concatenate([rng.normal(0, 15, gg.nz * gg.nx), [2.0]]) for gg in geoms])
+    P = Propagator(geoms, ckpt_every=0)
+    _, ref = P.tangent(torch.tensor(vec), torch.tensor(dv))
+    ref = _np(ref)
+    P.forward_save(torch.tensor(vec))
+    for cfg in [(128, 0, 3, 2), (192, 17, 3, 2), (256, 0, 3, 2), (64, 0, 3, 2), (64, 30, 4, 2), (128, 0, 4, 2)]:
+        P.set_config("born", *cfg)
+        out = _np(P.tangent_saved(torch.tensor(vec), torch.tensor(dv)))
+        assert np.array_equal(out, ref), cfg
+    P.set_active([True, False, True, False])
+    out = _np(P.tangent_saved(torch.tensor(vec), torch.tensor(dv)))
+    for i in (0, 2):
+        a, b = P.seis_off[i], P.seis_off[i + 1]
+        assert np.array_equal(out[a:b], ref[a:b])
+
+
+@pytest.mark.parametrize("cfg", [(64, 32, 1, 1), (128, 64, 4, 0), (256, 32, 3, 1), (64, 64, 8, 0),
+                                 (128, 0, 3, 2), (192, 0, 3, 2), (256, 0, 3, 2), (128, 20, 3, 2), (64, 0, 3, 2), (64, 11, 4, 2),
+                                 (192, 9, 3, 2)])
 def test_born_saved_matches_fused(small_geoms, cfg):
     """Tangent and GN product from the saved forward are bit-identical to the
     passes that recompute the forward, and match the oracle."""
