@@ -323,10 +323,11 @@ def gn_machine(req: Request, n_outer: int = 10, n_cg: int = 20, n_scales: int = 
             return out
 
         dx = yield from conjugate_gradient_machine(apply_A, rhs, n_cg)
-        trial = [x_guess + float(sc) * dx for sc in scales]
-        misfits = yield Op("fwd", vecs=[prior.from_basis(x) for x in trial])
-        stats["fwd"] += len(trial)
-        costs = [float(prior.cost_grad(x)[0]) + m / N for x, m in zip(trial, misfits)]
+        # all trial points at once: row k is x_guess + scales[k] * dx
+        trial = x_guess[None, :] + torch.as_tensor(scales, dtype=dx.dtype, device=dx.device)[:, None] * dx[None, :]
+        misfits = yield Op("fwd", vecs=list(prior.from_basis_rows(trial)))
+        stats["fwd"] += len(scales)
+        costs = (prior.cost_rows(trial).cpu().numpy() + np.asarray(misfits, dtype=np.float64) / N).tolist()
         best = float(scales[int(np.argmin(costs))])
         vec = vec + best * prior.from_basis(dx)
         stats["misfit"] = min(costs)
@@ -410,6 +411,8 @@ class Executor:
         return out
 
     def _pack(self, P: Propagator, ids, vecs):
+        if list(ids) == list(range(P.n_models)):   # every model: one concatenation
+            return torch.cat([v.reshape(-1) for v in vecs]).to(device=self.device, dtype=torch.float32)
         buf = torch.zeros(P.n_vec, dtype=torch.float32, device=self.device)
         for i, v in zip(ids, vecs):
             a, b = int(P.vec_off[i]), int(P.vec_off[i + 1])

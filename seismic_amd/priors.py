@@ -38,6 +38,15 @@ class Prior:
     def basis_T(self, g: torch.Tensor) -> torch.Tensor:
         return g if self.basis is None else self.basis.T @ g
 
+    def from_basis_rows(self, X: torch.Tensor) -> torch.Tensor:
+        """from_basis of every row of X (S x n_basis)."""
+        return X if self.basis is None else X @ self.basis.T
+
+    def cost_rows(self, X: torch.Tensor) -> torch.Tensor:
+        """Prior cost (lam included) of every row of X, as a tensor; the
+        Gauss-Newton line search evaluates all its trial points with it."""
+        return torch.stack([torch.as_tensor(self.cost_grad(x)[0], dtype=X.dtype, device=X.device) for x in X])
+
 
 class RowTotalVariation(Prior):
     """Rows constant, smoothed |row differences| (core/seis_prior.py:67-109)."""
@@ -118,6 +127,14 @@ class L2Smoothness(Prior):
         cost = torch.dot(xv, Px)
         g = torch.cat([2 * Px, torch.zeros(1, dtype=x.dtype, device=x.device)])
         return self.lam * cost, self.lam * g
+
+    def cost_rows(self, X):
+        """lam * x^T P x per row, P x applied to all rows at once (x^T P x =
+        |D_z x|^2 + |D_x x|^2)."""
+        xm = X[:, :-1].reshape(-1, self.nz, self.nx)
+        d1 = torch.diff(xm, dim=1)
+        d2 = torch.diff(xm, dim=2)
+        return self.lam * ((d1 * d1).sum(dim=(1, 2)) + (d2 * d2).sum(dim=(1, 2)))
 
 
 class SquaredExponential(Prior):

@@ -94,3 +94,18 @@ def test_l2_apply_P_matches_dense():
     o = opri.L2Smoothness(5, 6)
     x = np.random.default_rng(1).normal(size=30)
     np.testing.assert_allclose(p.apply_P(torch.tensor(x)).numpy(), o.P @ x, rtol=1e-12)
+
+
+def test_prior_rows_match_single_evaluations():
+    """The line search's batched prior costs and basis expansion equal the
+    one-vector evaluations row by row."""
+    rng = np.random.default_rng(2)
+    for p in (priors.RowTotalVariation(7, 9, lam=3.0, device="cpu"), priors.TotalVariation(7, 9, lam=3.0, device="cpu"),
+              priors.L2Smoothness(7, 9, lam=3.0, device="cpu"), priors.SquaredExponential(6, 8, lam=2.0, device="cpu")):
+        X = torch.tensor(rng.normal(0, 50, (5, p.N)))
+        rows = p.cost_rows(X)
+        one = torch.stack([torch.as_tensor(p.cost_grad(x)[0], dtype=X.dtype) for x in X])
+        np.testing.assert_allclose(rows.numpy(), one.numpy(), rtol=1e-12)
+        V = p.from_basis_rows(X)
+        for k in range(5):
+            np.testing.assert_allclose(V[k].numpy(), p.from_basis(X[k]).numpy(), rtol=1e-12, atol=1e-12)
