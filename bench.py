@@ -133,7 +133,7 @@ def kernel_roofline(ex, B, g, reps=3):
         rx, rz, k, strips = plan.get_config(kind)
         launches = math.ceil(steps / k)
         cells = plan.n_models * g.n_shots * g.padded_cells
-        kname = "k_fws" if strips == 2 else KERNEL[name]
+        kname = ("k_bns" if name == "born" else "k_fws") if strips == 2 else KERNEL[name]
         out[name] = {"kernel": kname, "pass_s": t_pass, "launches": launches, "launch_s": t_pass / launches,
                      "cell_updates_per_launch": cells * k, "k": k, "window": [rx, rz], "strips": strips,
                      "models": plan.n_models}

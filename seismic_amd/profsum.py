@@ -51,8 +51,8 @@ def traffic(probe_log, fetch_csv, write_csv):
 
     def pattern(name, e):
         rx, rz = e["window"]
-        if e["strips"] == 2:   # row-streamed forward: k_fws<threads, levels>
-            return rf"k_fws<{rx}, {e['k']}>"
+        if e["strips"] == 2:   # row-streamed kernels: k_fws / k_bns<threads, levels>
+            return rf"k_bns<{rx}, {e['k']}>" if name == "born" else rf"k_fws<{rx}, {e['k']}>"
         st = "true" if e["strips"] else "false"
         return {"linesearch_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, false>",
                 "saving_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, true>",
