@@ -93,9 +93,16 @@ def apply(P, table: dict | None = None) -> dict:
                  if e.split(":")[0] == kk and e.split(":")[2] == mode]
         if not cands:
             continue
-        cfg = tuple(table[min(cands)[1]])
-        P.set_config(kind, *cfg)
-        applied[kind] = cfg
+        # nearest bucket first; a config the plan cannot take (a streamed one
+        # on padded grids under 4k rows) falls through to the next
+        for _, e in sorted(cands):
+            cfg = tuple(table[e])
+            try:
+                P.set_config(kind, *cfg)
+            except ValueError:
+                continue
+            applied[kind] = cfg
+            break
     return applied
 
 

@@ -92,3 +92,30 @@ def test_nt_below_wavelet_length_is_rejected():
     assert len(Geometry(nt=147).wavelet()) == 147
     with pytest.raises(ValueError):
         Geometry(nt=146).wavelet()
+
+
+def test_autotune_apply_falls_back_when_a_config_is_refused():
+    """apply() takes the nearest table bucket the plan accepts: a streamed
+    config refused for a short padded grid falls through to the next bucket,
+    and with none accepted the plan keeps its defaults."""
+    from seismic_amd import autotune
+
+    class FakePlan:
+        caps = 1
+        ckpt_every = 0
+        geoms = [Geometry(nz=2, nx=5, nbc=6, nt=150)]   # 14 padded rows
+
+        def __init__(self):
+            self.cfg = {}
+
+        def set_config(self, kind, rx, rz, k, strips=None):
+            if strips == 2 and 14 < 4 * k:
+                raise ValueError("streamed forward needs padded grids of at least 4k rows")
+            self.cfg[kind] = (rx, rz, k, strips)
+
+    b = autotune.key(FakePlan(), "forward").split(":")[1]
+    table = {f"forward:{b}:full": [128, 0, 4, 2], f"forward:{int(b) + 1}:full": [64, 32, 3, 1]}
+    P = FakePlan()
+    assert autotune.apply(P, table) == {"forward": (64, 32, 3, 1)}
+    P = FakePlan()
+    assert autotune.apply(P, {f"forward:{b}:full": [128, 0, 4, 2]}) == {} and P.cfg == {}
