@@ -242,10 +242,10 @@ def test_streamed_born_ragged_batch():
         out = _np(P.tangent_saved(torch.tensor(vec), torch.tensor(dv)))
         assert np.array_equal(out, ref), cfg
     P.set_active([True, False, True, False])
-    out = _np(P.tangent_saved(torch.tensor(vec), torch.tensor(dv)))
+    out = P.tangent_saved(torch.tensor(vec), torch.tensor(dv))   # inactive models' slices are not written
     for i in (0, 2):
         a, b = P.seis_off[i], P.seis_off[i + 1]
-        assert np.array_equal(out[a:b], ref[a:b])
+        assert np.array_equal(_np(out[a:b]), ref[a:b])
 
 
 @pytest.mark.parametrize("cfg", [(64, 32, 1, 1), (128, 64, 4, 0), (256, 32, 3, 1), (64, 64, 8, 0),
