@@ -847,7 +847,7 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
 // Same stencil order and fused multiply-add sequence as k_born, so dseis and
 // (dp, dq) are bit-identical to it (and to the fused tangent).  Launches
 // always advance KL steps (a shorter last launch runs k_born).
-constexpr int SPB = 3;    // steps between a level-0 row's request and its intake
+constexpr int SPB = 2;    // steps between a level-0 row's request and its intake
 constexpr int SPH = 2;    // steps between a history row's request and its use
 template <int NT, int KL>
 __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
