@@ -179,7 +179,12 @@ def cpu_baseline(g, seconds_hint=25.0):
     t_gnp = t_fev + t_fev - t_fwd           # tangent ~ forward + adjoint cost, then adjoint
     t_inv = N_OUTER * (t_fev + N_CG * t_gnp + N_SCALES * t_fwd)
     cells = n * (g.nt + 1) * og.NZ * og.NX
+    from oracle import priors as opri
+    t0 = time.perf_counter()
+    opri.label_thresholded_components(_flat_grid(0), 1.0)
+    t_label = time.perf_counter() - t0
     return {"value": 1.0 / t_inv, "unit": "inversions/s", "cores": workers, "kind": "port",
+            "label_flat_s_per_grid": t_label,
             "s_per_forward": t_fwd, "s_per_fev": t_fev, "fevs_per_s": 1.0 / t_fev,
             "forward_cell_updates_per_s": cells / t_fwd,
             "sample": f"NumPy float64 oracle, {g.nz}x{g.nx} nt={g.nt}, {n} shots in {workers} processes: "
@@ -361,6 +366,24 @@ def autograd_run(device, n_models=64, ckpt=50, reps=3):
             "plan_gb": gbytes / 1e9}
 
 
+def flat_areas_run(device, n_models=32, reps=5):
+    """Flat-area labelling of RestrictFlatAreas (label_thresholded_components,
+    core/seis_numerics.py:81-124) for a batch of 70x70 guesses in one launch
+    (the host baseline of the same labelling is in cpu_baseline)."""
+    import torch
+    from seismic_amd import priors
+    grids = [torch.tensor(_flat_grid(i), device=device) for i in range(n_models)]
+    t = _timed_s(lambda: priors.label_flat(grids, 1.0), reps)
+    _, counts = priors.label_flat(grids, 1.0)
+    return {"models": n_models, "grid": "70x70", "gpu_s": t, "gpu_grids_per_s": n_models / t,
+            "mean_areas": float(np.mean(counts))}
+
+
+def _flat_grid(i):
+    from seismic_amd import synth
+    return np.round(synth.curve_fault_model(70, 70, 700 + i) * 0.98)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -437,6 +460,9 @@ def main():
         log("[bench] autograd, 64 models")
         extras["autograd_64"] = autograd_run(dev)
         log(f"[bench] autograd: {extras['autograd_64']['step_s']:.2f} s per step")
+        log("[bench] flat-area labelling, 32 grids")
+        extras["flat_areas_32"] = flat_areas_run(dev)
+        log(f"[bench] flat areas: {extras['flat_areas_32']['gpu_s'] * 1e3:.3f} ms per batch of 32")
 
     serving = None
     if not args.no_serve:

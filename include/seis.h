@@ -205,6 +205,20 @@ int seis_plan_get_config(const seis_plan_t* plan, int32_t kind, int32_t* rx,
 /* Device bytes held by the plan. */
 int64_t seis_plan_device_bytes(const seis_plan_t* plan);
 
+/* Flat areas of n_models velocity grids (no plan needed).  Grid m has
+ * nz[m] x nx[m] float64 velocities (host arrays of sizes; row-major, depth =
+ * row), concatenated on the device in vel.  Neighbouring cells (4-
+ * connectivity) join when |v[a] - v[b]| <= threshold; labels (int32 per
+ * cell, same concatenation) number the connected components 0..counts[m]-1
+ * in order of each component's smallest flat cell index, as scipy's
+ * connected_components numbers them.  Replaces label_thresholded_components
+ * (core/seis_numerics.py:81-124) as called by RestrictFlatAreas._adapt
+ * (core/seis_prior.py:256-262) with connectivity 4.  nz*nx <= 24576 per
+ * grid; one workgroup per grid. */
+int seis_label_flat(const int32_t* nz /* host */, const int32_t* nx /* host */,
+                    int32_t n_models, const double* vel, double threshold,
+                    int32_t* labels, int32_t* counts, seis_stream_t stream);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* seis_last_error(void);
 

@@ -99,3 +99,77 @@ class L2Smoothness:
         cost = float(xv @ Px)
         g = np.concatenate([2 * Px, [0.0]])
         return self.lam * cost, self.lam * g
+
+
+def label_thresholded_components(A, X, connectivity=4):
+    """core/seis_numerics.py:81-124: connected components of the grid graph
+    whose edges join neighbouring pixels (4- or 8-connectivity) with
+    |A[n] - A[p]| <= X.  Labels 0..n_labels-1 are numbered in order of each
+    component's smallest flat pixel index, the order scipy's
+    connected_components (the reference's call, :117-120) assigns for an
+    undirected graph.  Restated with union-find (smallest index as root);
+    pinned against scipy itself in tests/test_oracle.py."""
+    A = np.asarray(A, dtype=np.float64)
+    H, W = A.shape
+    N = H * W
+    parent = np.arange(N)
+
+    def find(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    neigh = [(1, 0), (0, 1)] if connectivity == 4 else [(1, 0), (0, 1), (1, 1), (1, -1)]
+    for i in range(H):
+        for j in range(W):
+            for di, dj in neigh:   # each undirected edge once; |a-b| is symmetric (:105-110)
+                ni, nj = i + di, j + dj
+                if 0 <= ni < H and 0 <= nj < W and abs(A[ni, nj] - A[i, j]) <= X:
+                    a, b = find(i * W + j), find(ni * W + nj)
+                    if a != b:
+                        parent[max(a, b)] = min(a, b)
+    roots = np.array([find(i) for i in range(N)])
+    order = {r: k for k, r in enumerate(sorted(set(roots.tolist())))}
+    labels = np.array([order[r] for r in roots.tolist()], dtype=np.int64)
+    return labels.reshape(H, W), len(order)
+
+
+class RestrictFlatAreas:
+    """core/seis_prior.py:244-281: areas that are flat (4-connected, steps of
+    at most diff_threshold) in a velocity guess become single parameters.
+
+    The basis is built as in _adapt (:258-272): one indicator column per
+    label, empty columns dropped, min_vel appended (block_diag), every column
+    scaled to sum 1.  cost_grad evaluates the underlying prior (identity
+    basis, :252-254) at basis @ x and maps its gradient back (:274-281)."""
+
+    def __init__(self, underlying, diff_threshold=1.0):
+        self.underlying = underlying
+        self.diff_threshold = diff_threshold
+        self.lam = underlying.lam
+        self.nz, self.nx = underlying.nz, underlying.nx
+        self.basis = np.eye(self.nz * self.nx + 1)
+        self.N = self.basis.shape[1]
+
+    def adapt(self, vec_guess):
+        v = np.asarray(vec_guess, dtype=np.float64).reshape(-1)[:self.nz * self.nx].reshape(self.nz, self.nx)
+        labels, count = label_thresholded_components(v, self.diff_threshold, connectivity=4)
+        n = self.nz * self.nx
+        bf = np.zeros((n, count + 1))
+        bf[np.arange(n), labels.ravel()] = 1.0
+        bf = bf[:, bf.sum(axis=0) > 0]
+        B = np.zeros((n + 1, bf.shape[1] + 1))
+        B[:n, :bf.shape[1]] = bf
+        B[n, -1] = 1.0
+        self.basis = B / B.sum(axis=0)
+        self.N = self.basis.shape[1]
+        if hasattr(self.underlying, "P"):   # precision on the area coordinates (Gauss-Newton)
+            Bv = self.basis[:n, :-1]
+            self.P = Bv.T @ self.underlying.P @ Bv
+        return labels, count
+
+    def cost_grad(self, x):
+        x = np.asarray(x, dtype=np.float64).reshape(-1)
+        c, g = self.underlying.cost_grad(self.basis @ x)
+        return c, self.basis.T @ g

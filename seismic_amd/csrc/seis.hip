@@ -1431,6 +1431,90 @@ __global__ __launch_bounds__(NTHR) void k_misfit_part(const ModelDev* models, co
   if (threadIdx.x == 0) part[(int64_t)blockIdx.y * kRedBlocks + blockIdx.x] = acc;
 }
 
+// ---------------------------------------------------------------- flat-area labelling
+constexpr int kLabelMax = 24576;    // most cells of one labelled grid (LDS: 5 bytes per cell)
+constexpr int kLabelThreads = 1024;
+
+// Connected components of one velocity grid per workgroup, where 4-neighbours
+// are joined when |v[n] - v[p]| <= thr (label_thresholded_components,
+// core/seis_numerics.py:81-124, used by RestrictFlatAreas,
+// core/seis_prior.py:256-262).  Edges are evaluated once in float64 (bit 0:
+// right neighbour, bit 1: lower neighbour; the test is symmetric).  Every
+// cell starts with its own index as label and repeatedly takes the minimum
+// over its joined neighbours and over its label's label (pointer jumping);
+// labels only decrease and stay indices of cells of the same component, so
+// the sweep ends (no change in a full pass) with every cell holding its
+// component's smallest index.  Roots are then numbered in index order by a
+// workgroup prefix sum: the numbering of scipy's connected_components on
+// the same graph (components in order of their smallest node).
+__global__ __launch_bounds__(kLabelThreads) void k_label(const int* dims, const int64_t* offs, const double* vel,
+                                                          double thr, int* labels, int* counts) {
+  __shared__ int L[kLabelMax];
+  __shared__ unsigned char E[kLabelMax];
+  __shared__ int scan[kLabelThreads];
+  __shared__ int changed;
+  const int m = blockIdx.x, t = threadIdx.x;
+  const int H = dims[2 * m], W = dims[2 * m + 1], N = H * W;
+  const double* A = vel + offs[m];
+  for (int i = t; i < N; i += kLabelThreads) {
+    const int r = i / W, c = i - r * W;
+    const double a = A[i];
+    unsigned char e = 0;
+    if (c + 1 < W && fabs(A[i + 1] - a) <= thr) e |= 1;
+    if (r + 1 < H && fabs(A[i + W] - a) <= thr) e |= 2;
+    E[i] = e;
+    L[i] = i;
+  }
+  __syncthreads();
+  for (;;) {
+    if (t == 0) changed = 0;
+    __syncthreads();
+    bool mine = false;
+    for (int i = t; i < N; i += kLabelThreads) {
+      const int r = i / W, c = i - r * W;
+      const unsigned char e = E[i];
+      int v = L[i];
+      if (e & 1) v = min(v, L[i + 1]);
+      if (e & 2) v = min(v, L[i + W]);
+      if (c > 0 && (E[i - 1] & 1)) v = min(v, L[i - 1]);
+      if (r > 0 && (E[i - W] & 2)) v = min(v, L[i - W]);
+      v = min(v, L[v]);
+      if (v < L[i]) {
+        L[i] = v;
+        mine = true;
+      }
+    }
+    if (mine) changed = 1;
+    __syncthreads();
+    const bool again = changed != 0;
+    __syncthreads();   // every thread has read `changed` before it is reset
+    if (!again) break;
+  }
+  // number the roots (L[i] == i) in index order: contiguous chunks per thread
+  const int C = (N + kLabelThreads - 1) / kLabelThreads;
+  const int b0 = min(N, t * C), b1 = min(N, b0 + C);
+  int cnt = 0;
+  for (int i = b0; i < b1; ++i) cnt += L[i] == i;
+  scan[t] = cnt;
+  __syncthreads();
+  for (int d = 1; d < kLabelThreads; d <<= 1) {  // inclusive Hillis-Steele scan
+    const int add = t >= d ? scan[t - d] : 0;
+    __syncthreads();
+    scan[t] += add;
+    __syncthreads();
+  }
+  int rank = scan[t] - cnt;
+  for (int i = b0; i < b1; ++i)
+    if (L[i] == i) L[i] = -(rank++) - 1;   // a root holds -(its number) - 1
+  __syncthreads();
+  int* out = labels + offs[m];
+  for (int i = t; i < N; i += kLabelThreads) {
+    const int v = L[i];
+    out[i] = v < 0 ? -v - 1 : -L[v] - 1;
+  }
+  if (t == 0) counts[m] = scan[kLabelThreads - 1];
+}
+
 __global__ void k_misfit_final(const ModelDev* models, int n_models, const double* part, double* out) {
   const int mi = blockIdx.x * blockDim.x + threadIdx.x;
   if (mi >= n_models || !models[mi].active) return;
@@ -2187,6 +2271,34 @@ int seis_plan_destroy(seis_plan_t* plan) {
   if (!plan) return SEIS_OK;
   (void)hipDeviceSynchronize();
   plan_free(plan);
+  return SEIS_OK;
+}
+
+int seis_label_flat(const int32_t* nz, const int32_t* nx, int32_t n_models, const double* vel, double threshold,
+                    int32_t* labels, int32_t* counts, seis_stream_t stream) {
+  if (!nz || !nx || n_models < 1 || !vel || !labels || !counts) return fail(SEIS_EINVAL, "NULL argument or no grids");
+  if (!(threshold >= 0.0)) return fail(SEIS_EINVAL, "threshold must be >= 0");
+  std::vector<int> dims(2 * (size_t)n_models);
+  std::vector<int64_t> offs(n_models);
+  int64_t off = 0;
+  for (int m = 0; m < n_models; ++m) {
+    if (nz[m] < 1 || nx[m] < 1 || (int64_t)nz[m] * nx[m] > kLabelMax)
+      return fail(SEIS_EINVAL, "grid " + std::to_string(m) + ": need 1 <= nz*nx <= " + std::to_string(kLabelMax));
+    dims[2 * m] = nz[m];
+    dims[2 * m + 1] = nx[m];
+    offs[m] = off;
+    off += (int64_t)nz[m] * nx[m];
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t bd = sizeof(int) * dims.size(), bo = sizeof(int64_t) * offs.size();
+  char* d = nullptr;
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), bd + bo, s));
+  HIP_TRY(hipMemcpyAsync(d, dims.data(), bd, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + bd, offs.data(), bo, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_label, dim3(n_models), dim3(kLabelThreads), 0, s, reinterpret_cast<const int*>(d),
+                     reinterpret_cast<const int64_t*>(d + bd), vel, threshold, labels, counts);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipFreeAsync(d, s));
   return SEIS_OK;
 }
 

@@ -166,3 +166,125 @@ class SquaredExponential(Prior):
         cost = torch.dot(xv, Px)
         g = torch.cat([2 * Px, torch.zeros(1, dtype=x.dtype, device=x.device)])
         return self.lam * cost, self.lam * g
+
+
+def label_flat(grids, threshold: float = 1.0):
+    """Flat areas of velocity grids on the GPU (libseis ``seis_label_flat``,
+    the label_thresholded_components of core/seis_numerics.py:81-124 with
+    4-connectivity): neighbours join when their velocities (compared in
+    float64) differ by at most ``threshold``.  ``grids`` is a list of (nz, nx)
+    tensors; returns (labels, counts): int64 (nz, nx) label tensors on the
+    grids' device, numbered as scipy numbers components, and the component
+    counts."""
+    import ctypes
+
+    from . import _lib
+    if not grids:
+        return [], []
+    dev = grids[0].device
+    if dev.type != "cuda":
+        raise RuntimeError("label_flat needs a ROCm GPU (torch.cuda)")
+    flat = torch.cat([g.reshape(-1).to(device=dev, dtype=torch.float64) for g in grids]).contiguous()
+    n = len(grids)
+    nz = (ctypes.c_int32 * n)(*[int(g.shape[0]) for g in grids])
+    nx = (ctypes.c_int32 * n)(*[int(g.shape[1]) for g in grids])
+    labels = torch.empty(flat.numel(), dtype=torch.int32, device=dev)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().seis_label_flat(nz, nx, n, ctypes.c_void_p(flat.data_ptr()), float(threshold),
+                                              ctypes.c_void_p(labels.data_ptr()), ctypes.c_void_p(counts.data_ptr()),
+                                              stream), "seis_label_flat")
+    out, pos = [], 0
+    for g in grids:
+        k = int(g.shape[0]) * int(g.shape[1])
+        out.append(labels[pos:pos + k].to(torch.int64).reshape(g.shape[0], g.shape[1]))
+        pos += k
+    return out, counts.cpu().tolist()
+
+
+class RestrictFlatAreas(Prior):
+    """Flat areas of a velocity guess become single parameters
+    (core/seis_prior.py:244-281).  ``adapt(vec_guess)`` labels the areas on
+    the GPU (``label_flat``) and sets the basis of _adapt (:258-272): one
+    column per area (indicator / area size, so columns sum to 1) and min_vel;
+    the basis is applied through the labels instead of a dense matrix.
+    Costs and gradients are the underlying prior's (identity basis, :253)
+    at basis @ x, mapped back by basis^T (:274-281).  Before ``adapt`` the
+    basis is the identity."""
+
+    def __init__(self, underlying: Prior, diff_threshold: float = 1.0):
+        super().__init__(underlying.nz, underlying.nx, underlying.device)
+        if underlying.basis is not None:
+            raise ValueError("the underlying prior must have the identity basis (core/seis_prior.py:253)")
+        self.underlying = underlying
+        self.diff_threshold = diff_threshold
+        self.lam = underlying.lam
+        self.labels = None
+
+    @property
+    def N(self) -> int:
+        return self.nz * self.nx + 1 if self.labels is None else self.count + 1
+
+    def adapt(self, vec_guess: torch.Tensor):
+        n = self.nz * self.nx
+        v = torch.as_tensor(vec_guess).reshape(-1)[:n].to(self.device, torch.float64).reshape(self.nz, self.nx)
+        (lab,), (count,) = label_flat([v], self.diff_threshold)
+        self.labels = lab.reshape(-1)
+        self.count = count
+        self.sizes = torch.bincount(self.labels, minlength=count).to(torch.float64)
+        return lab, count
+
+    def _scatter(self, g):          # sum over each area of the velocity part
+        out = torch.zeros(self.count, dtype=g.dtype, device=g.device)
+        return out.index_add_(0, self.labels, g)
+
+    def to_basis(self, vec):
+        """solve(B^T B, B^T vec): B^T B is diagonal (1/size), so each area's
+        coordinate is the sum of its velocities."""
+        if self.labels is None:
+            return vec.clone()
+        n = self.nz * self.nx
+        return torch.cat([self._scatter(vec[:n]), vec[n:]])
+
+    def from_basis(self, x):
+        if self.labels is None:
+            return x
+        return torch.cat([x[self.labels] / self.sizes[self.labels], x[-1:]])
+
+    def from_basis_rows(self, X):
+        if self.labels is None:
+            return X
+        return torch.cat([X[:, self.labels] / self.sizes[self.labels], X[:, -1:]], dim=1)
+
+    def basis_T(self, g):
+        if self.labels is None:
+            return g
+        n = self.nz * self.nx
+        return torch.cat([self._scatter(g[:n]) / self.sizes, g[n:]])
+
+    def dense_basis(self) -> torch.Tensor:
+        """The (nz*nx+1) x N basis matrix of _adapt (for callers that take a
+        dense basis)."""
+        n = self.nz * self.nx
+        if self.labels is None:
+            return torch.eye(n + 1, dtype=torch.float64, device=self.device)
+        B = torch.zeros(n + 1, self.count + 1, dtype=torch.float64, device=self.device)
+        B[torch.arange(n, device=self.device), self.labels] = 1.0 / self.sizes[self.labels]
+        B[n, self.count] = 1.0
+        return B
+
+    def cost_grad(self, x):
+        c, g = self.underlying.cost_grad(self.from_basis(x))
+        return c, self.basis_T(g)
+
+    def cost_rows(self, X):
+        return self.underlying.cost_rows(self.from_basis_rows(X))
+
+    def apply_P(self, xv):
+        """B_v^T P B_v on the area coordinates (Gauss-Newton), for an
+        underlying prior with a precision matrix."""
+        if self.labels is None:
+            return self.underlying.apply_P(xv)
+        Pv = self.underlying.apply_P(xv[self.labels] / self.sizes[self.labels])
+        return self._scatter(Pv) / self.sizes

@@ -338,3 +338,79 @@ def test_golden_fixtures_match_oracle():
     m = float(np.sum((full - d["target"].astype(np.float64)) ** 2))
     assert abs(m - float(d["misfit"])) <= 1e-12 * float(d["misfit"])
     assert d["grad"].shape == (g.nz * g.nx + 1,) and np.all(np.isfinite(d["grad"]))
+
+
+def _scipy_labels(A, X, connectivity):
+    """The reference's construction (core/seis_numerics.py:98-120): directed
+    adjacency for every pixel and neighbour with |diff| <= X, then scipy's
+    connected_components on the undirected graph."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    H, W = A.shape
+    neigh = [(1, 0), (-1, 0), (0, 1), (0, -1)]
+    if connectivity == 8:
+        neigh += [(1, 1), (1, -1), (-1, 1), (-1, -1)]
+    r, c = [], []
+    for i in range(H):
+        for j in range(W):
+            for di, dj in neigh:
+                ni, nj = i + di, j + dj
+                if 0 <= ni < H and 0 <= nj < W and abs(A[ni, nj] - A[i, j]) <= X:
+                    r.append(i * W + j)
+                    c.append(ni * W + nj)
+    adj = coo_matrix((np.ones(len(r), dtype=bool), (r, c)), shape=(H * W, H * W)).tocsr()
+    n, lab = connected_components(adj, directed=False, return_labels=True)
+    return lab.reshape(H, W), n
+
+
+@pytest.mark.parametrize("shape,X,conn", [((9, 13), 1.0, 4), ((20, 20), 1.0, 4), ((17, 6), 0.0, 4),
+                                          ((12, 12), 2.0, 8), ((1, 30), 1.0, 4), ((30, 1), 1.0, 4)])
+def test_label_thresholded_components_matches_scipy(shape, X, conn):
+    """The union-find restatement gives scipy's labels (numbering included)
+    on integer-valued grids, where steps equal to the threshold occur."""
+    rng = np.random.default_rng(shape[0] * 31 + shape[1])
+    for trial in range(4):
+        A = np.round(rng.normal(0, 1.2 + trial, shape))
+        lab, n = priors.label_thresholded_components(A, X, conn)
+        ref, nref = _scipy_labels(A, X, conn)
+        assert n == nref
+        assert np.array_equal(lab, ref)
+
+
+def test_label_thresholded_components_special_cases():
+    flat = np.full((5, 7), 3.0)
+    lab, n = priors.label_thresholded_components(flat, 0.0)
+    assert n == 1 and np.all(lab == 0)
+    ramp = np.arange(35, dtype=np.float64).reshape(5, 7) * 2.0
+    lab, n = priors.label_thresholded_components(ramp, 1.0)
+    assert n == 35 and np.array_equal(lab.ravel(), np.arange(35))
+    two = np.zeros((4, 6))
+    two[:, 3:] = 10.0
+    lab, n = priors.label_thresholded_components(two, 1.0)
+    assert n == 2 and np.all(lab[:, :3] == 0) and np.all(lab[:, 3:] == 1)
+
+
+def test_restrict_flat_areas_basis_and_gradient():
+    """Basis of RestrictFlatAreas (core/seis_prior.py:258-272): one column per
+    flat area plus min_vel, each summing to 1 and each pixel in exactly one
+    column; its gradient (:274-281) matches central differences."""
+    rng = np.random.default_rng(3)
+    nz, nx = 8, 9
+    v = np.round(rng.normal(2000, 1.0, (nz, nx)))
+    v[:, :4] += 300.0
+    p = priors.RestrictFlatAreas(priors.TotalVariation(nz, nx, lam=1.0), diff_threshold=1.0)
+    labels, count = p.adapt(np.concatenate([v.ravel(), [1500.0]]))
+    B = p.basis
+    assert B.shape == (nz * nx + 1, count + 1)
+    np.testing.assert_allclose(B.sum(axis=0), 1.0)
+    assert np.all((B[:-1] > 0).sum(axis=1) == 1)
+    for k in range(count):
+        assert np.all((B[:-1, k] > 0) == (labels.ravel() == k))
+    x = rng.normal(0, 100, p.N)
+    c, g = p.cost_grad(x)
+    h = 1e-3
+    for k in rng.choice(p.N, 5, replace=False):
+        e = np.zeros(p.N)
+        e[k] = h
+        fd = (p.cost_grad(x + e)[0] - p.cost_grad(x - e)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k]))
