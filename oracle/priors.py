@@ -101,6 +101,42 @@ class L2Smoothness:
         return self.lam * cost, self.lam * g
 
 
+class SquaredExponential:
+    """core/seis_prior.py:178-242 on an (nz, nx) grid centred as :199-207.
+    K from the SE kernel, a constant offset, a depth slope and noise; P =
+    K^-1 (:210).  transform (:213-219): basis U[:, s > cutoff] sqrt(s) from
+    the SVD of K, min_vel appended, and P = I."""
+
+    def __init__(self, nz, nx, lam=1e-8, length_scale=np.log(32.4), noise=0.1, sigma=183.4, sigma_mean=520.0,
+                 sigma_slope=31.4, transform=False, svd_cutoff=0.0):
+        self.nz, self.nx, self.lam = nz, nx, lam
+        y = (np.arange(nz) - nz // 2)[:, None] + np.zeros((1, nx))
+        x = (np.arange(nx) - nx // 2)[None, :] + np.zeros((nz, 1))
+        x = x.reshape(-1, 1)
+        y = y.reshape(-1, 1)
+        dist = np.sqrt((x - x.T) ** 2 + (y - y.T) ** 2)
+        K = sigma ** 2 * np.exp(-dist ** 2 / (2 * length_scale ** 2)) + sigma_mean ** 2
+        K = K + sigma_slope ** 2 * (y @ y.T) + noise ** 2 * np.eye(nz * nx)
+        self.K = K
+        self.P = np.linalg.inv(K)
+        if transform:
+            U, sv, _ = np.linalg.svd(K)
+            keep = sv > svd_cutoff
+            B = np.zeros((nz * nx + 1, int(keep.sum()) + 1))
+            B[:-1, :-1] = U[:, keep] @ np.diag(np.sqrt(sv[keep]))
+            B[-1, -1] = 1.0
+            self.basis = B
+            self.P = np.eye(B.shape[1] - 1)
+        else:
+            self.basis = np.eye(nz * nx + 1)
+        self.N = self.basis.shape[1]
+
+    def cost_grad(self, x):
+        x = np.asarray(x, dtype=np.float64).reshape(-1)
+        Px = self.P @ x[:-1]
+        return self.lam * float(x[:-1] @ Px), self.lam * np.concatenate([2 * Px, [0.0]])
+
+
 def label_thresholded_components(A, X, connectivity=4):
     """core/seis_numerics.py:81-124: connected components of the grid graph
     whose edges join neighbouring pixels (4- or 8-connectivity) with

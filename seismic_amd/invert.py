@@ -294,14 +294,17 @@ def conjugate_gradient_machine(apply_A, b: torch.Tensor, n_iter: int):
     return x
 
 
-def gn_machine(req: Request, n_outer: int = 10, n_cg: int = 20, n_scales: int = 50):
+def gn_machine(req: Request, n_outer: int = 10, n_cg: int | None = 20, n_scales: int = 50):
     """n_outer passes of seis_to_vel_gn (core/seis_invert.py:63-107).
 
     Each pass: rhs = -B^T J^T r / N - lam P x  (:77-78); solve
     (lam P (+) 0 + J^T J / N) dx = rhs by n_cg CG steps, with J^T J products
     from one tangent + one adjoint pass each (replacing the dense J of
     :84-92); line search over linspace(-0.1, 1.1, n_scales) of the forward
-    cost (:95-102).  The prior must provide apply_P.
+    cost (:95-102).  n_cg = None solves densely as the reference does
+    (:89-92): the matrix is assembled from one product per basis vector and
+    solved with torch.linalg.solve, for bases too ill-conditioned for CG (the
+    SE prior's SVD modes).  The prior must provide apply_P.
     """
     prior = req.prior
     N = req.n_data
@@ -322,7 +325,15 @@ def gn_machine(req: Request, n_outer: int = 10, n_cg: int = 20, n_scales: int = 
             out[:-1] += prior.lam * prior.apply_P(p[:-1])
             return out
 
-        dx = yield from conjugate_gradient_machine(apply_A, rhs, n_cg)
+        if n_cg is None:
+            cols = []
+            for i in range(rhs.numel()):
+                e = torch.zeros_like(rhs)
+                e[i] = 1.0
+                cols.append((yield from apply_A(e)))
+            dx = torch.linalg.solve(torch.stack(cols, dim=1), rhs)
+        else:
+            dx = yield from conjugate_gradient_machine(apply_A, rhs, n_cg)
         # all trial points at once: row k is x_guess + scales[k] * dx
         trial = x_guess[None, :] + torch.as_tensor(scales, dtype=dx.dtype, device=dx.device)[:, None] * dx[None, :]
         misfits = yield Op("fwd", vecs=list(prior.from_basis_rows(trial)))

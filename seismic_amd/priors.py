@@ -138,34 +138,56 @@ class L2Smoothness(Prior):
 
 
 class SquaredExponential(Prior):
-    """Gaussian-process prior (core/seis_prior.py:181-242), identity basis.
+    """Gaussian-process prior (core/seis_prior.py:178-242).
 
     K = sigma^2 exp(-d^2 / 2 l^2) + sigma_mean^2 + sigma_slope^2 y y^T + noise^2 I
-    over the (nz, nx) grid centred as in the reference; P = K^-1 (dense).
-    """
+    over the (nz, nx) grid centred as in the reference (:199-207).  Without
+    ``transform`` the basis is the identity and P = K^-1 (:210); with it the
+    basis is U sqrt(s) over the modes of K with s > svd_cutoff, plus min_vel,
+    and P = I (:213-219).  K is symmetric positive definite, so its SVD is
+    its eigendecomposition: the modes come from ``eigh``, in descending
+    order; a mode's sign may differ from the SVD's, which leaves every
+    velocity and cost unchanged."""
 
     def __init__(self, nz, nx, lam=1e-8, length_scale=math.log(32.4), noise=0.1, sigma=183.4,
-                 sigma_mean=520.0, sigma_slope=31.4, device="cuda"):
+                 sigma_mean=520.0, sigma_slope=31.4, transform=False, svd_cutoff=0.0, device="cuda"):
         super().__init__(nz, nx, device)
         self.lam = lam
+        self.transform = transform
         y = (torch.arange(nz, dtype=torch.float64) - nz // 2)[:, None] + torch.zeros(1, nx, dtype=torch.float64)
         x = (torch.arange(nx, dtype=torch.float64) - nx // 2)[None, :] + torch.zeros(nz, 1, dtype=torch.float64)
         x = x.reshape(-1, 1)
         y = y.reshape(-1, 1)
         d2 = (x - x.T) ** 2 + (y - y.T) ** 2
         K = sigma ** 2 * torch.exp(-d2 / (2 * length_scale ** 2)) + sigma_mean ** 2
-        K = K + sigma_slope ** 2 * (y @ y.T) + noise ** 2 * torch.eye(nz * nx, dtype=torch.float64)
-        self.P = torch.linalg.inv(K.to(self.device))
+        K = (K + sigma_slope ** 2 * (y @ y.T) + noise ** 2 * torch.eye(nz * nx, dtype=torch.float64)).to(self.device)
+        if transform:
+            s, U = torch.linalg.eigh(K)
+            s, U = s.flip(0), U.flip(1)
+            keep = s > svd_cutoff
+            k = int(keep.sum())
+            B = torch.zeros(nz * nx + 1, k + 1, dtype=torch.float64, device=self.device)
+            B[:-1, :k] = U[:, keep] * torch.sqrt(s[keep])[None, :]
+            B[-1, k] = 1.0
+            self.basis = B
+            self.P = None                      # identity on the modes
+        else:
+            self.P = torch.linalg.inv(K)
 
     def apply_P(self, xv):
-        return self.P @ xv
+        return xv.clone() if self.P is None else self.P @ xv
 
     def cost_grad(self, x):
         xv = x[:-1]
-        Px = self.P @ xv
+        Px = self.apply_P(xv)
         cost = torch.dot(xv, Px)
         g = torch.cat([2 * Px, torch.zeros(1, dtype=x.dtype, device=x.device)])
         return self.lam * cost, self.lam * g
+
+    def cost_rows(self, X):
+        Xv = X[:, :-1]
+        PX = Xv if self.P is None else Xv @ self.P     # P symmetric
+        return self.lam * (Xv * PX).sum(dim=1)
 
 
 def label_flat(grids, threshold: float = 1.0):

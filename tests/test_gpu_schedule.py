@@ -1,0 +1,69 @@
+"""Chained schedules and family dispatch (core/seis_model.py) on the GPU:
+the flat-patch KPI on the labelling kernel against the oracle, dispatch of
+a laterally symmetric and a faulted 70x70 case, and batched chains of
+different families equal to the same chains run one at a time.  The chain
+logic itself is pinned step for step against the oracle's chain on the CPU
+(tests/test_invert_host.py)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd2d
+from oracle import invert as oinv
+from seismic_amd import invert, schedule, synth
+from seismic_amd.geometry import Geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def test_kpi_fault_a_matches_oracle():
+    rng = np.random.default_rng(4)
+    grids = [synth.curve_fault_model(70, 70, 3), np.round(rng.normal(3000, 8, (30, 40))),
+             np.full((12, 12), 2500.0), synth.make_model("flat", 70, 70, 2)]
+    for v in grids:
+        assert schedule.kpi_fault_a(torch.tensor(v, device="cuda")) == oinv.kpi_fault_a(v)
+
+
+def _request(true, start, g, og):
+    target = fd2d.vel_to_seis(synth.to_vec(true), og)[0]
+    return invert.Request(geom=g, target=torch.tensor(target, dtype=torch.float32, device="cuda"), prior=None,
+                          start=torch.tensor(start, dtype=torch.float64, device="cuda"))
+
+
+def test_dispatch_symmetric_and_faulted():
+    """The reference's KPIs pick FlatVel for a laterally invariant model
+    (shot 0 mirrors shot 4) and a TV2D family for a faulted one."""
+    kw = dict(nt=400)
+    g, og = Geometry(**kw), fd2d.Geometry(**kw)
+    flat = synth.make_model("flat", 70, 70, 5)
+    fault = synth.curve_fault_model(70, 70, 6)
+    start = synth.to_vec(synth.depth_linear_start(70, 70, 1500, 4000), min_vel=1500.0)
+    P = schedule.style_a_precision(70, 70, "cuda")
+    assert schedule.classify(_request(flat, start, g, og), P) == "FlatVel"
+    fam = schedule.classify(_request(fault, synth.to_vec(fault), g, og), P)
+    print("faulted family", fam, flush=True)
+    assert fam in ("TV2D", "TV2Deasy")
+
+
+def test_batched_chains_equal_plain():
+    """TV2D (three stages, the last on RestrictFlatAreas), FlatVel and
+    Style_A (SE modes, then a dense Gauss-Newton pass) chains in test mode,
+    batched together and run one at a time: identical results, lower
+    misfit than the start."""
+    kw = dict(nz=20, nx=24, nbc=16, nt=300)
+    g, og = Geometry(**kw), fd2d.Geometry(**kw)
+    reqs = []
+    fams = ["TV2D", "FlatVel", "Style_A", "TV2D"]
+    for i, fam in enumerate(fams):
+        true = synth.curve_fault_model(g.nz, g.nx, 20 + i)
+        start = synth.to_vec(np.round(true * 0.96), min_vel=1500.0)
+        reqs.append(_request(true, start, g, og))
+    clone = lambda: [invert.Request(geom=r.geom, target=r.target, prior=None, start=r.start.clone()) for r in reqs]
+    vb, _ = schedule.infer(clone(), test_mode=True, batched=True, families=fams)
+    vp, _ = schedule.infer(clone(), test_mode=True, batched=False, families=fams)
+    for a, b, r, fam in zip(vb, vp, reqs, fams):
+        assert torch.equal(a, b), fam
+        m = lambda v: np.mean((fd2d.vel_to_seis(v.cpu().numpy(), og)[0] - r.target.cpu().numpy()) ** 2)
+        m0, m1 = m(r.start), m(a)
+        print(fam, "misfit", m0, "->", m1, flush=True)
+        assert m1 < m0

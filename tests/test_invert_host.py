@@ -138,3 +138,44 @@ def test_gn_machine_restrict_flat_areas_equals_oracle():
     m = invert.gn_machine(r, n_outer=1, n_cg=4, n_scales=7)
     out = invert.run_machines({0: m}, OracleExecutor(og, target))[0].numpy()
     np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-7)
+
+
+def test_chain_machine_equals_oracle_chain():
+    """Stages chained as kgs.ChainedModel (LBFGS with 2-D TV, LBFGS with row
+    TV, one Gauss-Newton pass on the SE prior's SVD modes with as many CG
+    steps as modes) reproduce the oracle's chain, step for step."""
+    from seismic_amd import schedule
+    g, og, target, start = _setup(5)
+    stages = [schedule.Stage(lambda nz, nx, d: priors.TotalVariation(nz, nx, lam=1e-8, device="cpu"), maxiter=4,
+                             tolerance_grad=1e-7),
+              schedule.Stage(lambda nz, nx, d: priors.RowTotalVariation(nz, nx, lam=1e-8, device="cpu"), maxiter=3,
+                             tolerance_change=1e-9),
+              schedule.Stage(lambda nz, nx, d: priors.SquaredExponential(nz, nx, lam=1e-12, transform=True,
+                                                                         svd_cutoff=1e4, device="cpu"),
+                             maxiter=0, n_scales=7)]
+    ostages = [dict(prior=lambda: opri.TotalVariation(g.nz, g.nx, lam=1e-8), maxiter=4, tolerance_grad=1e-7),
+               dict(prior=lambda: opri.RowTotalVariation(g.nz, g.nx, lam=1e-8), maxiter=3, tolerance_change=1e-9),
+               dict(prior=lambda: opri.SquaredExponential(g.nz, g.nx, lam=1e-12, transform=True, svd_cutoff=1e4),
+                    maxiter=0, n_scales=7)]
+    ref = oinv.chain_invert(start, target, ostages, og)
+    r = invert.Request(geom=g, target=torch.tensor(target), prior=None, start=torch.tensor(start))
+    m = schedule.chain_machine(r, schedule.Schedule(stages))
+    out = invert.run_machines({0: m}, OracleExecutor(og, target))[0].numpy()
+    np.testing.assert_allclose(out, ref, rtol=1e-7, atol=1e-6)
+    assert len(r.meta["stages"]) == 3
+
+
+def test_schedule_kpis_match_oracle():
+    from seismic_amd import schedule
+    rng = np.random.default_rng(8)
+    s = rng.normal(0, 1, (5, 40, 12))
+    s[4] = s[0][:, ::-1] + 1e-6 * rng.normal(0, 1, (40, 12))
+    assert schedule.kpi_flat_vel(torch.tensor(s.ravel()), 5, 40, 12) == pytest.approx(
+        oinv.kpi_flat_vel(s.ravel(), 5, 40, 12), rel=1e-12)
+    v = np.round(rng.normal(3000, 20, (13, 17)))
+    assert schedule.kpi_style_b(torch.tensor(v)) == oinv.kpi_style_b(v)
+    se = priors.SquaredExponential(6, 8, lam=1e-12, device="cpu")
+    o = opri.SquaredExponential(6, 8, lam=1e-12)
+    x = rng.normal(2500, 300, 6 * 8 + 1)
+    # K^-1 of the SE kernel is ill-conditioned (sigma_mean^2 offset): agreement to 1e-7
+    assert schedule.kpi_style_a(torch.tensor(x), se.P) == pytest.approx(float(x[:-1] @ o.P @ x[:-1]), rel=1e-7)

@@ -414,3 +414,23 @@ def test_restrict_flat_areas_basis_and_gradient():
         e[k] = h
         fd = (p.cost_grad(x + e)[0] - p.cost_grad(x - e)[0]) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k]))
+
+
+def test_squared_exponential_oracle_closed_forms():
+    """SE prior (core/seis_prior.py:196-220): K P = I; with every SVD mode
+    kept, B_v B_v^T = K; the product's eigh modes span the same space (B B^T
+    equal) and give the same costs."""
+    from seismic_amd import priors as ppri
+    o = priors.SquaredExponential(6, 7, lam=1.5)
+    np.testing.assert_allclose(o.K @ o.P, np.eye(42), atol=1e-6)
+    ot = priors.SquaredExponential(6, 7, lam=1.5, transform=True, svd_cutoff=0.0)
+    Bv = ot.basis[:-1, :-1]
+    np.testing.assert_allclose(Bv @ Bv.T, o.K, rtol=1e-9, atol=1e-6 * np.abs(o.K).max())
+    for cut in (1.0, 1e4):
+        ot = priors.SquaredExponential(6, 7, lam=1.5, transform=True, svd_cutoff=cut)
+        pt = ppri.SquaredExponential(6, 7, lam=1.5, transform=True, svd_cutoff=cut, device="cpu")
+        Bp = pt.basis.numpy()
+        assert Bp.shape == ot.basis.shape
+        np.testing.assert_allclose(Bp @ Bp.T, ot.basis @ ot.basis.T, atol=1e-7 * np.abs(o.K).max())
+        x = np.random.default_rng(1).normal(0, 3, ot.N)
+        assert float(pt.cost_grad(torch.tensor(x))[0]) == pytest.approx(ot.cost_grad(x)[0], rel=1e-12)
