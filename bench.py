@@ -379,6 +379,39 @@ def flat_areas_run(device, n_models=32, reps=5):
             "mean_areas": float(np.mean(counts))}
 
 
+def schedule_run(device, n_each=8):
+    """Classified, chained inversions (core/seis_model.py, test-mode stage
+    lengths): n_each laterally invariant layered models (FlatVel) and n_each
+    faulted ones (TV2D: two TV stages and one on the flat areas), all chains
+    batched over one executor."""
+    import torch
+    from seismic_amd import invert, schedule, synth
+    from seismic_amd.geometry import Geometry
+    from seismic_amd.propagator import Propagator
+    g = Geometry(nt=1000)
+    trues = [synth.make_model("flat", 70, 70, 900 + i) for i in range(n_each)] + \
+            [synth.curve_fault_model(70, 70, 950 + i) for i in range(n_each)]
+    P = Propagator([g] * len(trues), caps=1, device=device)
+    obs = P.forward(torch.tensor(np.concatenate([synth.to_vec(t) for t in trues]), device=device)).clone()
+    P.close()
+    reqs = []
+    for i, t in enumerate(trues):
+        start = synth.to_vec(np.round(t * 0.97), min_vel=1500.0)
+        reqs.append(invert.Request(geom=g, target=obs[i * g.n_data:(i + 1) * g.n_data].clone(), prior=None,
+                                   start=torch.tensor(start, dtype=torch.float64, device=device), rid=i))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out, fams = schedule.infer(reqs, test_mode=True, batched=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fevs = [sum(st.get("fev", 0) for st in r.meta.get("stages", [])) for r in reqs]
+    mae = [float(np.abs(o.cpu().numpy()[:-1] - t.ravel()).mean()) for o, t in zip(out, trues)]
+    return {"requests": len(reqs), "families": {f: fams.count(f) for f in sorted(set(fams))}, "wall_s": dt,
+            "inversions_per_s": len(reqs) / dt, "fevs_per_request_mean": float(np.mean(fevs)),
+            "mae_to_true_m_per_s": {f: float(np.mean([m for m, ff in zip(mae, fams) if ff == f]))
+                                    for f in sorted(set(fams))}, "stage_lengths": "test mode"}
+
+
 def _flat_grid(i):
     from seismic_amd import synth
     return np.round(synth.curve_fault_model(70, 70, 700 + i) * 0.98)
@@ -463,6 +496,10 @@ def main():
         log("[bench] flat-area labelling, 32 grids")
         extras["flat_areas_32"] = flat_areas_run(dev)
         log(f"[bench] flat areas: {extras['flat_areas_32']['gpu_s'] * 1e3:.3f} ms per batch of 32")
+        log("[bench] classified chained inversions, 16 requests")
+        extras["schedules_16"] = schedule_run(dev)
+        log(f"[bench] schedules: {extras['schedules_16']['inversions_per_s']:.2f} inversions/s "
+            f"{extras['schedules_16']['families']}")
 
     serving = None
     if not args.no_serve:

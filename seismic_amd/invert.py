@@ -331,7 +331,7 @@ def gn_machine(req: Request, n_outer: int = 10, n_cg: int | None = 20, n_scales:
                 e = torch.zeros_like(rhs)
                 e[i] = 1.0
                 cols.append((yield from apply_A(e)))
-            dx = torch.linalg.solve(torch.stack(cols, dim=1), rhs)
+            dx = torch.linalg.solve(torch.stack(cols, dim=1).cpu(), rhs.cpu()).to(rhs.device)   # host LAPACK
         else:
             dx = yield from conjugate_gradient_machine(apply_A, rhs, n_cg)
         # all trial points at once: row k is x_guess + scales[k] * dx

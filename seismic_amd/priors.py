@@ -30,7 +30,8 @@ class Prior:
         if self.basis is None:
             return vec.clone()
         B = self.basis
-        return torch.linalg.solve(B.T @ B, B.T @ vec)
+        # small dense solve on the host (LAPACK), like the dense GN solve
+        return torch.linalg.solve((B.T @ B).cpu(), (B.T @ vec).cpu()).to(vec.device)
 
     def from_basis(self, x: torch.Tensor) -> torch.Tensor:
         return x if self.basis is None else self.basis @ x
@@ -137,6 +138,39 @@ class L2Smoothness(Prior):
         return self.lam * ((d1 * d1).sum(dim=(1, 2)) + (d2 * d2).sum(dim=(1, 2)))
 
 
+_SE_CACHE: dict = {}
+
+
+def _se_factors(nz, nx, length_scale, noise, sigma, sigma_mean, sigma_slope, transform, svd_cutoff, device):
+    """K^-1, or the SVD-mode basis, of the SE kernel: factored once per
+    parameter set on the host (LAPACK, float64) and shared read-only on each
+    device, as the reference prepares its prior once (Prior.prep,
+    core/seis_prior.py:29-35)."""
+    key = (nz, nx, length_scale, noise, sigma, sigma_mean, sigma_slope, transform, svd_cutoff, str(device))
+    if key in _SE_CACHE:
+        return _SE_CACHE[key]
+    y = (torch.arange(nz, dtype=torch.float64) - nz // 2)[:, None] + torch.zeros(1, nx, dtype=torch.float64)
+    x = (torch.arange(nx, dtype=torch.float64) - nx // 2)[None, :] + torch.zeros(nz, 1, dtype=torch.float64)
+    x = x.reshape(-1, 1)
+    y = y.reshape(-1, 1)
+    d2 = (x - x.T) ** 2 + (y - y.T) ** 2
+    K = sigma ** 2 * torch.exp(-d2 / (2 * length_scale ** 2)) + sigma_mean ** 2
+    K = K + sigma_slope ** 2 * (y @ y.T) + noise ** 2 * torch.eye(nz * nx, dtype=torch.float64)
+    if transform:
+        s, U = torch.linalg.eigh(K)
+        s, U = s.flip(0), U.flip(1)
+        keep = s > svd_cutoff
+        k = int(keep.sum())
+        B = torch.zeros(nz * nx + 1, k + 1, dtype=torch.float64)
+        B[:-1, :k] = U[:, keep] * torch.sqrt(s[keep])[None, :]
+        B[-1, k] = 1.0
+        out = (B.to(device), None)
+    else:
+        out = (None, torch.linalg.inv(K).to(device))
+    _SE_CACHE[key] = out
+    return out
+
+
 class SquaredExponential(Prior):
     """Gaussian-process prior (core/seis_prior.py:178-242).
 
@@ -147,32 +181,19 @@ class SquaredExponential(Prior):
     and P = I (:213-219).  K is symmetric positive definite, so its SVD is
     its eigendecomposition: the modes come from ``eigh``, in descending
     order; a mode's sign may differ from the SVD's, which leaves every
-    velocity and cost unchanged."""
+    velocity and cost unchanged.  The factors are computed once per
+    parameter set and device (``_se_factors``) and shared."""
 
     def __init__(self, nz, nx, lam=1e-8, length_scale=math.log(32.4), noise=0.1, sigma=183.4,
                  sigma_mean=520.0, sigma_slope=31.4, transform=False, svd_cutoff=0.0, device="cuda"):
         super().__init__(nz, nx, device)
         self.lam = lam
         self.transform = transform
-        y = (torch.arange(nz, dtype=torch.float64) - nz // 2)[:, None] + torch.zeros(1, nx, dtype=torch.float64)
-        x = (torch.arange(nx, dtype=torch.float64) - nx // 2)[None, :] + torch.zeros(nz, 1, dtype=torch.float64)
-        x = x.reshape(-1, 1)
-        y = y.reshape(-1, 1)
-        d2 = (x - x.T) ** 2 + (y - y.T) ** 2
-        K = sigma ** 2 * torch.exp(-d2 / (2 * length_scale ** 2)) + sigma_mean ** 2
-        K = (K + sigma_slope ** 2 * (y @ y.T) + noise ** 2 * torch.eye(nz * nx, dtype=torch.float64)).to(self.device)
+        B, P = _se_factors(nz, nx, float(length_scale), float(noise), float(sigma), float(sigma_mean),
+                           float(sigma_slope), bool(transform), float(svd_cutoff), self.device)
         if transform:
-            s, U = torch.linalg.eigh(K)
-            s, U = s.flip(0), U.flip(1)
-            keep = s > svd_cutoff
-            k = int(keep.sum())
-            B = torch.zeros(nz * nx + 1, k + 1, dtype=torch.float64, device=self.device)
-            B[:-1, :k] = U[:, keep] * torch.sqrt(s[keep])[None, :]
-            B[-1, k] = 1.0
             self.basis = B
-            self.P = None                      # identity on the modes
-        else:
-            self.P = torch.linalg.inv(K)
+        self.P = P                                 # None: identity on the modes
 
     def apply_P(self, xv):
         return xv.clone() if self.P is None else self.P @ xv
