@@ -434,3 +434,33 @@ def test_squared_exponential_oracle_closed_forms():
         np.testing.assert_allclose(Bp @ Bp.T, ot.basis @ ot.basis.T, atol=1e-7 * np.abs(o.K).max())
         x = np.random.default_rng(1).normal(0, 3, ot.N)
         assert float(pt.cost_grad(torch.tensor(x))[0]) == pytest.approx(ot.cost_grad(x)[0], rel=1e-12)
+
+
+def test_dispatch_kpis_on_constructed_cases():
+    """The dispatch KPIs (core/seis_model.py:133, :204-215) on inputs whose
+    values follow from the definitions: a mirrored shot pair gives 0 and an
+    offset c on the last shot gives exactly |c|; round(v/3) bins of known
+    sizes; areas of known sizes separated by steps just above 10 m/s, joined
+    by steps of exactly 10."""
+    rng = np.random.default_rng(3)
+    s = rng.normal(0, 1, (5, 30, 9))
+    s[4] = s[0][:, ::-1]
+    assert invert.kpi_flat_vel(s.ravel(), 5, 30, 9) == 0.0
+    s[4] += 0.25
+    assert invert.kpi_flat_vel(s.ravel(), 5, 30, 9) == pytest.approx(0.25, rel=1e-12)
+    s[1] += 7.0                                   # only shots 0 and 4 enter
+    assert invert.kpi_flat_vel(s.ravel(), 5, 30, 9) == pytest.approx(0.25, rel=1e-12)
+
+    # bins: 3000 and 3001 share bin 1000 (17 cells), 3003 is bin 1001 (12), 3006 bin 1002 (6)
+    v = np.array([3000.0] * 10 + [3001.0] * 7 + [3003.0] * 12 + [3006.0] * 6)
+    assert invert.kpi_style_b(rng.permutation(v).reshape(5, 7)) == 17
+
+    # three vertical strips of widths 3, 5, 2 on 6 rows, steps 10.5 between them
+    g = np.zeros((6, 10))
+    g[:, 3:8] = 10.5
+    g[:, 8:] = 21.0
+    assert invert.kpi_fault_a(g) == 6 * 5 + 6 * 3
+    g[:, 8:] = 20.5                               # a step of exactly 10 joins the last two strips
+    assert invert.kpi_fault_a(g) == 6 * 7 + 6 * 3
+    assert invert.kpi_fault_a(np.full((4, 4), 2000.0)) == 16    # one area: every cell
+    assert invert.kpi_fault_a(np.arange(12.0).reshape(3, 4) * 11) == 2    # all singletons
