@@ -412,6 +412,57 @@ def schedule_run(device, n_each=8):
                                     for f in sorted(set(fams))}, "stage_lengths": "test mode"}
 
 
+def formats_run(device, n_files=4, per_file=32, batch=32, reps=3):
+    """Competition-shaped files (core/kaggle_support.py:192-345, 554-587):
+    n_files x per_file training samples ((N, 5, 1000, 70) float32 seismograms
+    from the GPU forward of layered models, (N, 1, 70, 70) velocities)
+    written to a temporary directory, then streamed to HBM by the loader
+    (page-cached reads: the files were just written) and written out as a
+    submission file."""
+    import shutil
+    import tempfile
+    import torch
+    from seismic_amd import datasets, synth
+    from seismic_amd.geometry import Geometry
+    from seismic_amd.propagator import Propagator
+    g = Geometry(nt=1000)
+    n = n_files * per_file
+    trues = np.stack([synth.make_model("flat", 70, 70, 1200 + i) for i in range(per_file)]).astype(np.float32)
+    P = Propagator([g] * per_file, caps=1, device=device)
+    seis = P.forward(torch.tensor(np.concatenate([synth.to_vec(t) for t in trues]), device=device))
+    seis = seis.reshape(per_file, 5, 1000, 70).cpu().numpy()
+    P.close()
+    root = tempfile.mkdtemp(prefix="seis_formats_")
+    try:
+        for k in range(n_files):
+            d = os.path.join(root, "train_samples", "FlatVel_A")
+            os.makedirs(d, exist_ok=True)
+            np.save(os.path.join(d, f"seis_{k}.npy"), seis)
+            np.save(os.path.join(d, f"vel_{k}.npy"), trues[:, None])
+        samples = datasets.load_train(root)
+        assert len(samples) == n
+        best = float("inf")
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for b in datasets.Loader(samples, batch=batch, nt=999, device=device):
+                pass
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        nbytes = n * (5 * 999 * 70 * 4 + 4901 * 8)
+        grids = [torch.tensor(trues[i % per_file] + 0.37, device=device) for i in range(n)]
+        sub = os.path.join(root, "submission.csv")
+        t0 = time.perf_counter()
+        datasets.write_submission(samples, grids, sub)
+        t_sub = time.perf_counter() - t0
+        rows = n * 70
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    return {"samples": n, "batch": batch, "loader_s": best, "loader_samples_per_s": n / best,
+            "loader_gb_per_s": nbytes / best / 1e9, "submission_rows": rows, "submission_s": t_sub,
+            "submission_rows_per_s": rows / t_sub, "reads": "page-cached"}
+
+
 def _flat_grid(i):
     from seismic_amd import synth
     return np.round(synth.curve_fault_model(70, 70, 700 + i) * 0.98)
@@ -500,6 +551,10 @@ def main():
         extras["schedules_16"] = schedule_run(dev)
         log(f"[bench] schedules: {extras['schedules_16']['inversions_per_s']:.2f} inversions/s "
             f"{extras['schedules_16']['families']}")
+        log("[bench] on-disk formats, 128 samples")
+        extras["formats_128"] = formats_run(dev)
+        log(f"[bench] formats: loader {extras['formats_128']['loader_gb_per_s']:.2f} GB/s, submission "
+            f"{extras['formats_128']['submission_rows_per_s']:.0f} rows/s")
 
     serving = None
     if not args.no_serve:

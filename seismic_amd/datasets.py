@@ -1,0 +1,339 @@
+"""Competition data on disk: samples, streaming into HBM, result cache,
+submission file and score (core/kaggle_support.py:192-345, 414-476,
+529-587).
+
+Layouts read (as load_all_train_data / load_all_test_data, :285-345):
+
+- train: ``<data_dir>/train_samples/<family>/seis<D>.npy`` with
+  ``vel<D>.npy`` (all ``seis*`` files in name order), then
+  ``<family>/data/data<i>.npy`` with ``<family>/model/model<i>.npy`` for
+  i = 1 .. number of files in ``data/``; seismograms ``(N, 5, 1000, 70)``,
+  velocities ``(N, 1, 70, 70)``; one sample per index;
+- test: every file in ``<data_dir>/test/`` (name order) is one sample of
+  shape ``(5, 1000, 70)``.
+
+Only the first ``nt`` (999) of the 1000 time rows are used (:205-209).
+
+``Loader`` streams batches to the device: a reader thread copies the
+memory-mapped slices of a batch (one strided slice per run of consecutive
+indices in one file) into a pinned host slab, the slab goes to HBM with one
+asynchronous copy on a side stream, and the consumer's stream waits on that
+copy's event; ``depth`` slabs rotate, so reading batch i+1 from disk
+overlaps the copy and the inversions of batch i.
+
+``ResultCache`` keeps one result per sample under the reference's cache
+names (:279-280, :418-470).  Results are written as ``.npy`` parameter
+vectors (grid + min_vel, float64) and read back with pickling disabled; the
+reference's dill cache files are not read.
+"""
+from __future__ import annotations
+
+import csv
+import glob
+import os
+import queue
+import threading
+from dataclasses import dataclass
+from typing import Callable, Sequence
+
+import numpy as np
+import torch
+
+N_SHOTS, N_T_FILE, N_RCV, NZ, NX = 5, 1000, 70, 70, 70
+X_COLUMNS = np.arange(1, 70, 2)        # submission columns x_1, x_3, ..., x_69
+
+
+@dataclass
+class Sample:
+    """Data (:264-280): where one sample's seismogram (and, for training
+    data, its true velocity) lives."""
+
+    seis_file: str
+    seis_ind: int | None = None        # None: the file is one sample
+    vel_file: str | None = None
+    vel_ind: int | None = None
+    family: str = "test"
+    is_train: bool = False
+
+    def cache_name(self) -> str:
+        """Data.cache_name (:279-280)."""
+        return os.path.basename(self.seis_file[:-4] + "__" + self.family + "__" + str(self.seis_ind))
+
+
+def _n_rows(path: str) -> int:
+    return int(np.load(path, mmap_mode="r").shape[0])
+
+
+def load_train(data_dir: str, validation_only: bool = False) -> list[Sample]:
+    """load_all_train_data (:285-329): families in name order; in each, the
+    ``seis*`` / ``vel*`` pairs, then ``data/data<i>`` / ``model/model<i>``;
+    ``validation_only`` keeps the first file of each kind."""
+    out: list[Sample] = []
+    for d in sorted(glob.glob(os.path.join(data_dir, "train_samples", "*"))):
+        fam = d[max(d.rfind("/"), d.rfind("\\")) + 1:]
+        pairs = []
+        seis_files = sorted(glob.glob(d + "/seis*.npy"))
+        for f in seis_files[:1] if validation_only else seis_files:
+            tag = f[f.rfind("seis") + 4:f.rfind(".")]
+            pairs.append((d + "/seis" + tag + ".npy", d + "/vel" + tag + ".npy"))
+        if os.path.isdir(d + "/data/"):
+            n = len(glob.glob(d + "/data/*.npy"))
+            for i in range(1, (min(n, 1) if validation_only else n) + 1):
+                pairs.append((d + "/data/data" + str(i) + ".npy", d + "/model/model" + str(i) + ".npy"))
+        for seis, vel in pairs:
+            out.extend(Sample(seis, k, vel, k, fam, True) for k in range(_n_rows(seis)))
+    return out
+
+
+def load_test(data_dir: str) -> list[Sample]:
+    """load_all_test_data (:331-345)."""
+    return [Sample(f) for f in sorted(glob.glob(os.path.join(data_dir, "test", "*")))]
+
+
+# --------------------------------------------------------------------- reading
+def _runs(samples: Sequence[Sample], attr_file: str, attr_ind: str):
+    """Split a batch into runs (file, first index, count, first position) of
+    consecutive indices in one file, so each run is one strided read."""
+    runs = []
+    for pos, s in enumerate(samples):
+        f, k = getattr(s, attr_file), getattr(s, attr_ind)
+        if runs and runs[-1][0] == f and k is not None and runs[-1][1] is not None \
+                and runs[-1][1] + runs[-1][2] == k:
+            runs[-1][2] += 1
+        else:
+            runs.append([f, k, 1, pos])
+    return runs
+
+
+def read_seismograms(samples: Sequence[Sample], out: np.ndarray, nt: int = 999) -> None:
+    """Seismogram.load_to_memory (:205-209) for a batch: the first ``nt``
+    time rows of each sample into rows of ``out`` ((>= len, 5*nt*70)
+    float32), shot-major (shot, t, receiver) as the propagator's data."""
+    dst = out[:len(samples)].reshape(len(samples), N_SHOTS, nt, N_RCV)
+    for f, k0, cnt, pos in _runs(samples, "seis_file", "seis_ind"):
+        a = np.load(f, mmap_mode="r")
+        if k0 is None:
+            np.copyto(dst[pos], a[:, :nt, :], casting="same_kind")
+        else:
+            np.copyto(dst[pos:pos + cnt], a[k0:k0 + cnt, :, :nt, :], casting="same_kind")
+
+
+def read_velocities(samples: Sequence[Sample], out: np.ndarray) -> None:
+    """Velocity.load_to_memory + to_vector (:239-249) for a batch: rows of
+    ``out`` ((>= len, 4901) float64) get the 70x70 grid (row-major) and its
+    minimum as min_vel."""
+    n = len(samples)
+    grid = out[:n, :NZ * NX].reshape(n, NZ, NX)
+    for f, k0, cnt, pos in _runs(samples, "vel_file", "vel_ind"):
+        np.copyto(grid[pos:pos + cnt], np.load(f, mmap_mode="r")[k0:k0 + cnt, 0, :, :])
+    out[:n, -1] = grid.reshape(n, -1).min(axis=1)
+
+
+@dataclass
+class Batch:
+    samples: list
+    seis: torch.Tensor                 # (B, 5*nt*70) float32 on the device
+    vel: torch.Tensor | None           # (B, 4901) float64 on the device (training data)
+
+
+class Loader:
+    """Iterates over ``samples`` in batches of ``batch`` as device tensors
+    (see the module docstring for the pipeline).  ``velocities`` defaults
+    to whether every sample is training data."""
+
+    def __init__(self, samples: Sequence[Sample], batch: int = 32, nt: int = 999, device="cuda",
+                 depth: int = 2, velocities: bool | None = None):
+        if not 1 <= nt <= N_T_FILE:
+            raise ValueError(f"nt must be in 1..{N_T_FILE}")
+        if batch < 1 or depth < 1:
+            raise ValueError("batch and depth must be >= 1")
+        self.samples = list(samples)
+        self.batch, self.nt, self.depth = batch, nt, depth
+        self.device = torch.device(device)
+        self.velocities = all(s.is_train for s in self.samples) if velocities is None else velocities
+        if self.velocities and not all(s.vel_file is not None for s in self.samples):
+            raise ValueError("velocities requested for samples without a velocity file")
+
+    def __len__(self):
+        return (len(self.samples) + self.batch - 1) // self.batch
+
+    def __iter__(self):
+        cuda = self.device.type == "cuda"
+        width = N_SHOTS * self.nt * N_RCV
+        slabs = [(torch.empty((self.batch, width), dtype=torch.float32, pin_memory=cuda),
+                  torch.empty((self.batch, NZ * NX + 1), dtype=torch.float64, pin_memory=cuda)
+                  if self.velocities else None) for _ in range(self.depth)]
+        done = [None] * self.depth                 # copy-finished event per slab
+        free: queue.Queue = queue.Queue()
+        ready: queue.Queue = queue.Queue()
+        for k in range(self.depth):
+            free.put(k)
+        chunks = [self.samples[i:i + self.batch] for i in range(0, len(self.samples), self.batch)]
+
+        def reader():
+            try:
+                for chunk in chunks:
+                    k = free.get()
+                    if k is None:
+                        return
+                    if done[k] is not None:
+                        done[k].synchronize()      # the slab's previous copy has left it
+                    read_seismograms(chunk, slabs[k][0].numpy(), self.nt)
+                    if self.velocities:
+                        read_velocities(chunk, slabs[k][1].numpy())
+                    ready.put((k, chunk, None))
+            except BaseException as e:             # hand the error to the consumer
+                ready.put((None, None, e))
+
+        th = threading.Thread(target=reader, daemon=True)
+        th.start()
+        side = torch.cuda.Stream(device=self.device) if cuda else None
+        try:
+            for _ in chunks:
+                k, chunk, err = ready.get()
+                if err is not None:
+                    raise err
+                n = len(chunk)
+                seis = torch.empty((n, width), dtype=torch.float32, device=self.device)
+                vel = torch.empty((n, NZ * NX + 1), dtype=torch.float64, device=self.device) \
+                    if self.velocities else None
+                if cuda:
+                    side.wait_stream(torch.cuda.current_stream(self.device))
+                    with torch.cuda.stream(side):
+                        seis.copy_(slabs[k][0][:n], non_blocking=True)
+                        if vel is not None:
+                            vel.copy_(slabs[k][1][:n], non_blocking=True)
+                        ev = torch.cuda.Event()
+                        ev.record(side)
+                    done[k] = ev
+                    torch.cuda.current_stream(self.device).wait_event(ev)
+                else:
+                    seis.copy_(slabs[k][0][:n])
+                    if vel is not None:
+                        vel.copy_(slabs[k][1][:n])
+                free.put(k)
+                yield Batch(chunk, seis, vel)
+        finally:
+            free.put(None)
+            th.join()
+
+
+# --------------------------------------------------------------------- result cache
+class ResultCache:
+    """Per-sample results of one model under ``<root>/<name>/`` (Model.infer,
+    :418-470), named by ``Sample.cache_name``."""
+
+    def __init__(self, root: str, name: str, read: bool = True, write: bool = True):
+        self.dir = os.path.join(root, name)
+        self.read, self.write = read, write
+        if write:
+            os.makedirs(self.dir, exist_ok=True)
+
+    def _path(self, s: Sample) -> str:
+        return os.path.join(self.dir, s.cache_name() + ".npy")
+
+    def get(self, s: Sample) -> np.ndarray | None:
+        if not self.read:
+            return None
+        p = self._path(s)
+        return np.load(p, allow_pickle=False) if os.path.exists(p) else None
+
+    def put(self, s: Sample, vec) -> None:
+        if not self.write:
+            return
+        v = vec.detach().to("cpu", torch.float64).numpy() if isinstance(vec, torch.Tensor) \
+            else np.asarray(vec, dtype=np.float64)
+        tmp = self._path(s) + ".tmp"
+        with open(tmp, "wb") as f:
+            np.save(f, v, allow_pickle=False)
+        os.replace(tmp, self._path(s))             # a reader never sees a partial file
+
+
+# --------------------------------------------------------------------- submission and score
+def write_submission(samples: Sequence[Sample], grids, path: str, do_round: bool = False,
+                     do_range: bool = True) -> None:
+    """write_submission_file (:554-587): header ``oid_ypos,x_1,...,x_69``,
+    then per sample 70 rows ``<file stem>_y_<row>`` with the odd columns of
+    its grid, as float32 (or rounded to int32), clipped to 1500..4500 when
+    ``do_range``.  Written with ``csv.writer`` (``\r\n`` line ends, floats
+    as Python's repr of the float32 value), one ``writerows`` per sample:
+    the C writer is faster than formatting the cells with NumPy strings."""
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["oid_ypos"] + [f"x_{x}" for x in X_COLUMNS])
+        for s, g in zip(samples, grids):
+            arr = (g.detach().cpu().numpy() if isinstance(g, torch.Tensor) else np.asarray(g)).astype(np.float32)
+            if arr.shape != (NZ, NX):
+                raise ValueError(f"submission grids are {NZ}x{NX}, got {arr.shape}")
+            if do_round:
+                with np.errstate(invalid="ignore"):       # NaN casts as the reference's does
+                    arr = np.round(arr).astype(np.int32)
+            if do_range:
+                arr = np.clip(arr, 1500, 4500)
+            stem = os.path.basename(s.seis_file)[:-4] + "_y_"
+            w.writerows([stem + str(y)] + row for y, row in enumerate(arr[:, X_COLUMNS].tolist()))
+
+
+def score_metric(true_vel: torch.Tensor, guess: torch.Tensor, families: Sequence[str]):
+    """score_metric (:529-550) on device tensors of grids ((B, 70, 70) or
+    parameter vectors (B, 4901), min_vel ignored): per-sample MAE of the true
+    grid against the rounded guess; returns (score, {family: score},
+    per-sample list)."""
+    n = len(families)
+    t = true_vel.to(torch.float64).reshape(n, -1)[:, :NZ * NX]
+    g = guess.to(torch.float64).reshape(n, -1)[:, :NZ * NX]
+    per = torch.mean(torch.abs(t - torch.round(g)), dim=1).cpu().numpy().tolist()
+    fam: dict = {}
+    for e, f in zip(per, families):
+        fam.setdefault(f, []).append(e)
+    return float(np.mean(per)), {k: float(np.mean(v)) for k, v in fam.items()}, per
+
+
+# --------------------------------------------------------------------- the pipeline
+def infer_dataset(samples: Sequence[Sample], start: Callable | None = None, cache: ResultCache | None = None,
+                  batch: int = 32, test_mode: bool = False, families: Sequence[str] | None = None,
+                  nt: int = 999, device="cuda", ckpt_every: int = 0):
+    """Model.infer with ModelSplit (:414-476, core/seis_model.py:129-168)
+    over samples on disk: cached results are read back, the rest stream
+    through ``Loader`` and each batch is classified and inverted as one set
+    of chains; new results go to the cache.  ``start(sample, seis_row)``
+    gives each starting parameter vector (float64, 4901) -- the reference
+    takes it from its network; the default is 1500 -> 4500 m/s linear in
+    depth.  Returns (parameter vectors on the device, families; None for
+    cached samples)."""
+    from . import schedule, synth
+    from .geometry import Geometry
+    from .invert import Request
+
+    geom = Geometry(nt=nt)
+    dev = torch.device(device)
+    if start is None:
+        v0 = torch.tensor(synth.to_vec(synth.depth_linear_start(NZ, NX, 1500.0, 4500.0), min_vel=1500.0),
+                          dtype=torch.float64, device=dev)
+        start = lambda s, seis: v0.clone()
+    results: list = [None] * len(samples)
+    fams: list = [None] * len(samples)
+    todo = []
+    for i, s in enumerate(samples):
+        hit = cache.get(s) if cache is not None else None
+        if hit is not None:
+            results[i] = torch.as_tensor(hit, device=dev)
+        else:
+            todo.append(i)
+    for b in Loader([samples[i] for i in todo], batch=batch, nt=nt, device=dev, velocities=False):
+        idx = todo[:len(b.samples)]
+        todo = todo[len(b.samples):]
+        reqs = [Request(geom=geom, target=b.seis[j], prior=None, start=start(s, b.seis[j]), rid=j)
+                for j, s in enumerate(b.samples)]
+        vecs, f = schedule.infer(reqs, test_mode=test_mode, batched=True,
+                                 families=None if families is None else [families[i] for i in idx],
+                                 ckpt_every=ckpt_every)
+        for i, v, fam in zip(idx, vecs, f):
+            results[i], fams[i] = v, fam
+            if cache is not None:
+                cache.put(samples[i], v)
+    return results, fams
+
+
+__all__ = ["Sample", "load_train", "load_test", "read_seismograms", "read_velocities", "Loader", "Batch",
+           "ResultCache", "write_submission", "score_metric", "infer_dataset"]
