@@ -46,18 +46,21 @@ def test_dispatch_symmetric_and_faulted():
 
 
 def test_batched_chains_equal_plain():
-    """TV2D (three stages, the last on RestrictFlatAreas), FlatVel and
-    Style_A (SE modes, then a dense Gauss-Newton pass) chains in test mode,
-    batched together and run one at a time: identical results, lower
-    misfit than the start (for Style_A, after its LBFGS stage)."""
+    """TV2D (three stages, the last on RestrictFlatAreas), FlatVel, Style_A
+    (SE modes, then a dense Gauss-Newton pass), Style_B (two SE stages, the
+    second stopping on the misfit) and TV2Deasy chains in test mode, batched
+    together and run one at a time: identical results, lower misfit than the
+    start (for Style_A, after its LBFGS stage)."""
     kw = dict(nz=20, nx=24, nbc=16, nt=300)
     g, og = Geometry(**kw), fd2d.Geometry(**kw)
     reqs = []
-    fams = ["TV2D", "FlatVel", "Style_A", "TV2D"]
+    fams = ["TV2D", "FlatVel", "Style_A", "TV2D", "Style_B", "TV2Deasy"]
     for i, fam in enumerate(fams):
-        # each family on a model of its kind: layered for FlatVel, faulted for TV2D, smooth for Style_A
+        # each family on a model of its kind: layered for FlatVel, smooth for the SE
+        # families, faulted for the TV ones
         true = {"FlatVel": lambda: synth.make_model("flat", g.nz, g.nx, 20 + i),
-                "Style_A": lambda: synth.make_model("grf", g.nz, g.nx, 20 + i)}.get(
+                "Style_A": lambda: synth.make_model("grf", g.nz, g.nx, 20 + i),
+                "Style_B": lambda: synth.make_model("grf", g.nz, g.nx, 20 + i)}.get(
                     fam, lambda: synth.curve_fault_model(g.nz, g.nx, 20 + i))()
         start = synth.to_vec(np.round(true * 0.96), min_vel=1500.0)
         reqs.append(_request(true, start, g, og))
