@@ -276,20 +276,27 @@ def lbfgs_machine(req: Request, max_iter: int = 30, tolerance_grad: float = 1e-7
 
 # --------------------------------------------------------------------- Gauss-Newton
 def conjugate_gradient_machine(apply_A, b: torch.Tensor, n_iter: int):
-    """CG from 0 on an SPD operator given as a machine ``apply_A(p)``."""
+    """CG from 0 on an SPD operator given as a machine ``apply_A(p)``.
+
+    The scalars stay 0-d device tensors, so a batch of machines resumed
+    after one shared product queues its updates without a host round trip
+    each.  An exactly converged residual (rr = 0) makes every later step
+    zero (the guarded a and beta), which leaves x as a break would."""
     x = torch.zeros_like(b)
     r = b.clone()
     p = r.clone()
-    rr = float(r.dot(r))
+    rr = r.dot(r)
+    if float(rr) == 0.0:
+        return x
+    zero = torch.zeros((), dtype=b.dtype, device=b.device)
     for _ in range(n_iter):
-        if rr == 0.0:
-            break
         Ap = yield from apply_A(p)
-        a = rr / float(p.dot(Ap))
-        x.add_(p, alpha=a)
-        r.add_(Ap, alpha=-a)
-        rr_new = float(r.dot(r))
-        p = r + (rr_new / rr) * p
+        pAp = p.dot(Ap)
+        a = torch.where(pAp != 0, rr / pAp, zero)
+        x.add_(a * p)
+        r.sub_(a * Ap)
+        rr_new = r.dot(r)
+        p = r + torch.where(rr != 0, rr_new / rr, zero) * p
         rr = rr_new
     return x
 
