@@ -20,6 +20,7 @@ set for 70x70 grids with 5 shots; they are kept as they are.
 """
 from __future__ import annotations
 
+import copy
 import math
 from dataclasses import dataclass, field
 from typing import Callable
@@ -41,6 +42,8 @@ class Stage:
     seis_error_tolerance: float = 0.0
     n_cg: int | None = None    # GN: CG steps; None = dense solve as :89-92
     n_scales: int = 50         # GN line search points (:96)
+    scaling: float = 1e15      # LBFGS cost scaling (:55)
+    round_results: bool = False  # round this stage's result (Model.infer, core/kaggle_support.py:472-474)
 
 
 @dataclass
@@ -63,8 +66,10 @@ def chain_machine(req: Request, schedule: Schedule):
             vec = yield from gn_machine(sub, n_outer=1, n_cg=st.n_cg, n_scales=st.n_scales)
         else:
             vec = yield from lbfgs_machine(sub, max_iter=st.maxiter, tolerance_grad=st.tolerance_grad,
-                                           tolerance_change=st.tolerance_change,
+                                           tolerance_change=st.tolerance_change, scaling=st.scaling,
                                            seis_error_tolerance=st.seis_error_tolerance)
+        if st.round_results:
+            vec = torch.round(vec)
         stats.append(sub.meta.get("stats", {}))
     if schedule.round_results:
         vec = torch.round(vec)
@@ -189,14 +194,11 @@ def dispatch_machines(requests, test_mode=False, families=None):
     return machines, fams
 
 
-def infer(requests, test_mode=False, batched=True, families=None, ckpt_every=0):
-    """Classify and invert every request with its schedule; batched runs all
-    chains as one set of machines over one executor."""
+def _run(requests, machines, batched, ckpt_every):
     from .invert import Executor, run_machines
     ex = Executor([r.geom for r in requests], ckpt_every=ckpt_every, max_batch=None if batched else 1)
     for i, r in enumerate(requests):
         ex.set_target(i, r.target)
-    machines, fams = dispatch_machines(requests, test_mode, families)
     if batched:
         res = run_machines(machines, ex)
     else:
@@ -204,8 +206,78 @@ def infer(requests, test_mode=False, batched=True, families=None, ckpt_every=0):
         for i in machines:
             res.update(run_machines({i: machines[i]}, ex))
     ex.close()
-    return [res[i] for i in range(len(requests))], fams
+    return [res[i] for i in range(len(requests))]
+
+
+def infer(requests, test_mode=False, batched=True, families=None, ckpt_every=0):
+    """Classify and invert every request with its schedule; batched runs all
+    chains as one set of machines over one executor."""
+    machines, fams = dispatch_machines(requests, test_mode, families)
+    return _run(requests, machines, batched, ckpt_every), fams
+
+
+# --------------------------------------------------------------------- the reference's model classes
+class InversionModel:
+    """InversionModel (core/seis_invert.py:52-166) with the reference's
+    attribute names: ``prior`` (a prior instance, copied for each request
+    and adapted to its guess, :155-157; its grid must match the requests'),
+    ``maxiter`` (0: one Gauss-Newton pass, else LBFGS), ``scaling``,
+    ``lbfgs_tolerance_grad``, ``lbfgs_tolerance_change``,
+    ``seis_error_tolerance`` and ``round_results`` (Model,
+    core/kaggle_support.py:384).  ``infer(requests)`` inverts every request
+    from its ``start``; ``n_cg`` (not in the reference) switches the
+    Gauss-Newton pass from the dense solve to CG."""
+
+    def __init__(self, prior=None, maxiter: int = 0, scaling: float = 1e15, lbfgs_tolerance_grad: float = 1e-7,
+                 lbfgs_tolerance_change: float = 0.0, seis_error_tolerance: float = 0.0,
+                 round_results: bool = False, n_cg: int | None = None):
+        self.prior = prior
+        self.maxiter = maxiter
+        self.scaling = scaling
+        self.lbfgs_tolerance_grad = lbfgs_tolerance_grad
+        self.lbfgs_tolerance_change = lbfgs_tolerance_change
+        self.seis_error_tolerance = seis_error_tolerance
+        self.round_results = round_results
+        self.n_cg = n_cg
+
+    def stages(self) -> list:
+        if self.prior is None:
+            raise ValueError("InversionModel.prior is not set")
+        tmpl = self.prior
+
+        def make(nz, nx, device):
+            if (tmpl.nz, tmpl.nx) != (nz, nx):
+                raise ValueError(f"the prior is for a {tmpl.nz}x{tmpl.nx} grid, the request is {nz}x{nx}")
+            return copy.copy(tmpl)    # adapt() sets per-request state on the copy only
+        return [Stage(make, maxiter=self.maxiter, tolerance_grad=self.lbfgs_tolerance_grad,
+                      tolerance_change=self.lbfgs_tolerance_change, seis_error_tolerance=self.seis_error_tolerance,
+                      n_cg=self.n_cg, scaling=self.scaling, round_results=self.round_results)]
+
+    def infer(self, requests, batched: bool = True, ckpt_every: int = 0):
+        sched = Schedule(self.stages())
+        return _run(requests, {i: chain_machine(r, sched) for i, r in enumerate(requests)}, batched, ckpt_every)
+
+
+class ChainedModel:
+    """kgs.ChainedModel (core/kaggle_support.py:512-523): ``models`` run in
+    sequence, each from the previous one's result; ``round_results`` rounds
+    the chain's output."""
+
+    def __init__(self, models=None, round_results: bool = False):
+        self.models = list(models or [])
+        self.round_results = round_results
+
+    def stages(self) -> list:
+        out = [st for m in self.models for st in m.stages()]
+        if self.round_results and out:
+            out[-1] = copy.copy(out[-1])
+            out[-1].round_results = True
+        return out
+
+    def infer(self, requests, batched: bool = True, ckpt_every: int = 0):
+        sched = Schedule(self.stages())
+        return _run(requests, {i: chain_machine(r, sched) for i, r in enumerate(requests)}, batched, ckpt_every)
 
 
 __all__ = ["Stage", "Schedule", "chain_machine", "SCHEDULES", "classify", "dispatch_machines", "infer",
-           "kpi_flat_vel", "kpi_style_a", "kpi_style_b", "kpi_fault_a"]
+           "InversionModel", "ChainedModel", "kpi_flat_vel", "kpi_style_a", "kpi_style_b", "kpi_fault_a"]

@@ -84,3 +84,23 @@ def test_batched_chains_equal_plain():
         m1 = m(a)
         print(fam, "misfit", m0, "->", m1, flush=True)
         assert np.isfinite(m1) and m1 < m0
+
+
+def test_reference_model_classes_on_gpu():
+    """An InversionModel set up as the reference's FlatVel model
+    (core/seis_model.py:15-27, test-mode length) and wrapped in a rounding
+    ChainedModel gives the FlatVel chain of schedule.infer bit for bit."""
+    kw = dict(nz=20, nx=24, nbc=16, nt=300)
+    g, og = Geometry(**kw), fd2d.Geometry(**kw)
+    reqs = []
+    for i in range(3):
+        true = synth.make_model("flat", g.nz, g.nx, 60 + i)
+        reqs.append(_request(true, synth.to_vec(np.round(true * 0.96), min_vel=1500.0), g, og))
+    clone = lambda: [invert.Request(geom=r.geom, target=r.target, prior=None, start=r.start.clone()) for r in reqs]
+    from seismic_amd import priors
+    m = schedule.InversionModel(priors.RowTotalVariation(g.nz, g.nx, lam=1e-8, epsilon=0.1, device="cuda"),
+                                maxiter=30, lbfgs_tolerance_change=1e-9, lbfgs_tolerance_grad=1e-7)
+    out = schedule.ChainedModel([m], round_results=True).infer(clone())
+    ref, _ = schedule.infer(clone(), test_mode=True, families=["FlatVel"] * 3)
+    for a, b in zip(out, ref):
+        assert torch.equal(a, b)

@@ -179,3 +179,34 @@ def test_schedule_kpis_match_oracle():
     x = rng.normal(2500, 300, 6 * 8 + 1)
     # K^-1 of the SE kernel is ill-conditioned (sigma_mean^2 offset): agreement to 1e-7
     assert schedule.kpi_style_a(torch.tensor(x), se.P) == pytest.approx(float(x[:-1] @ o.P @ x[:-1]), rel=1e-7)
+
+
+def test_reference_model_classes_map_to_stages():
+    """InversionModel / ChainedModel with the reference's attribute names
+    (core/seis_invert.py:52-58, core/kaggle_support.py:384, 512-523) run the
+    same chain as the explicit stages; per-model rounding rounds that
+    model's output; a prior for another grid is refused."""
+    from seismic_amd import schedule
+    g, og, target, start = _setup(6)
+    tv = priors.TotalVariation(g.nz, g.nx, lam=1e-8, device="cpu")
+    rtv = priors.RowTotalVariation(g.nz, g.nx, lam=1e-8, device="cpu")
+    chain = schedule.ChainedModel([schedule.InversionModel(tv, maxiter=3, lbfgs_tolerance_grad=1e-7),
+                                   schedule.InversionModel(rtv, maxiter=2, lbfgs_tolerance_change=1e-9,
+                                                           round_results=True)])
+    sts = chain.stages()
+    assert [st.maxiter for st in sts] == [3, 2] and [st.round_results for st in sts] == [False, True]
+    explicit = [schedule.Stage(lambda nz, nx, d: priors.TotalVariation(nz, nx, lam=1e-8, device="cpu"), maxiter=3),
+                schedule.Stage(lambda nz, nx, d: priors.RowTotalVariation(nz, nx, lam=1e-8, device="cpu"), maxiter=2,
+                               tolerance_change=1e-9, round_results=True)]
+    outs = []
+    for stages in (sts, explicit):
+        r = invert.Request(geom=g, target=torch.tensor(target), prior=None, start=torch.tensor(start))
+        m = schedule.chain_machine(r, schedule.Schedule(stages))
+        outs.append(invert.run_machines({0: m}, OracleExecutor(og, target))[0])
+    assert torch.equal(outs[0], outs[1])
+    assert torch.equal(outs[0], torch.round(outs[0]))
+    wrong = schedule.InversionModel(priors.TotalVariation(g.nz + 1, g.nx, device="cpu"), maxiter=1)
+    with pytest.raises(ValueError):
+        wrong.stages()[0].prior(g.nz, g.nx, "cpu")
+    with pytest.raises(ValueError):
+        schedule.InversionModel().stages()
