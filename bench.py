@@ -412,6 +412,44 @@ def schedule_run(device, n_each=8):
                                     for f in sorted(set(fams))}, "stage_lengths": "test mode"}
 
 
+def served_schedule_run(device, n_each=8, slots=12, rate=50.0):
+    """The classified chains of schedule_run served with continuous batching:
+    2 n_each requests arriving as a Poisson process at `rate` on `slots`
+    70x70 slots (later arrivals wait for a free slot), each classified on
+    admission and run through its family's chain (test-mode stage lengths)."""
+    import torch
+    from seismic_amd import invert, schedule, serve, synth
+    from seismic_amd.geometry import Geometry
+    from seismic_amd.propagator import Propagator
+    g = Geometry(nt=1000)
+    trues = [synth.make_model("flat", 70, 70, 1300 + i) for i in range(n_each)] + \
+            [synth.curve_fault_model(70, 70, 1350 + i) for i in range(n_each)]
+    order = np.random.default_rng(5).permutation(len(trues))
+    trues = [trues[i] for i in order]
+    P = Propagator([g] * len(trues), caps=1, device=device)
+    obs = P.forward(torch.tensor(np.concatenate([synth.to_vec(t) for t in trues]), device=device)).clone()
+    P.close()
+    reqs = [invert.Request(geom=g, target=obs[i * g.n_data:(i + 1) * g.n_data].clone(), prior=None,
+                           start=torch.tensor(synth.to_vec(np.round(t * 0.97), min_vel=1500.0),
+                                              dtype=torch.float64, device=device), rid=i)
+            for i, t in enumerate(trues)]
+    arrivals = serve.poisson_arrivals(len(reqs), rate, seed=7)
+    srv = serve.InversionServer([g] * slots, hbm_budget=200 << 30, ckpt_every=50, history_size=100,
+                                machine=schedule.serving_machine(test_mode=True), device=device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = srv.run(reqs, arrivals)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fams = [r.stats.get("family") for r in recs]
+    lat = [r.latency for r in recs]
+    return {"requests": len(reqs), "slots": slots, "rate_per_s": rate, "wall_s": wall,
+            "inversions_per_s": len(reqs) / wall, "latency_s": serve.percentiles(lat),
+            "families": {f: fams.count(f) for f in sorted(set(fams))},
+            "max_requests_per_launch": max(n for _, n in srv.log), "failed": sum(r.failed for r in recs),
+            "stage_lengths": "test mode"}
+
+
 def formats_run(device, n_files=4, per_file=32, batch=32, reps=3):
     """Competition-shaped files (core/kaggle_support.py:192-345, 554-587):
     n_files x per_file training samples ((N, 5, 1000, 70) float32 seismograms
@@ -551,6 +589,11 @@ def main():
         extras["schedules_16"] = schedule_run(dev)
         log(f"[bench] schedules: {extras['schedules_16']['inversions_per_s']:.2f} inversions/s "
             f"{extras['schedules_16']['families']}")
+        log("[bench] served classified chains, 16 requests on 12 slots")
+        extras["served_schedules_16"] = served_schedule_run(dev)
+        log(f"[bench] served chains: {extras['served_schedules_16']['inversions_per_s']:.2f} inversions/s, "
+            f"p50 {extras['served_schedules_16']['latency_s']['p50']:.2f} s, "
+            f"p99 {extras['served_schedules_16']['latency_s']['p99']:.2f} s")
         log("[bench] on-disk formats, 128 samples")
         extras["formats_128"] = formats_run(dev)
         log(f"[bench] formats: loader {extras['formats_128']['loader_gb_per_s']:.2f} GB/s, submission "

@@ -194,6 +194,26 @@ def dispatch_machines(requests, test_mode=False, families=None):
     return machines, fams
 
 
+def serving_machine(test_mode=False, families=None):
+    """A per-request machine factory for ``serve.InversionServer``: each
+    admitted request is classified (ModelSplit._infer_single) on admission,
+    its family recorded in ``req.meta["family"]``, and its family's chain
+    run; ``families`` maps request ids to fixed families instead."""
+    Ps: dict = {}
+
+    def make(req: Request):
+        if families is not None:
+            fam = families[req.rid]
+        else:
+            key = (req.geom.nz, req.geom.nx, str(req.start.device))
+            if key not in Ps:
+                Ps[key] = style_a_precision(req.geom.nz, req.geom.nx, req.start.device)
+            fam = classify(req, Ps[key])
+        req.meta["family"] = fam
+        return chain_machine(req, SCHEDULES[fam](test_mode))
+    return make
+
+
 def _run(requests, machines, batched, ckpt_every):
     from .invert import Executor, run_machines
     ex = Executor([r.geom for r in requests], ckpt_every=ckpt_every, max_batch=None if batched else 1)
@@ -280,4 +300,5 @@ class ChainedModel:
 
 
 __all__ = ["Stage", "Schedule", "chain_machine", "SCHEDULES", "classify", "dispatch_machines", "infer",
+           "serving_machine",
            "InversionModel", "ChainedModel", "kpi_flat_vel", "kpi_style_a", "kpi_style_b", "kpi_fault_a"]

@@ -54,15 +54,21 @@ def request_footprint(g: Geometry, ckpt_every: int, history: int) -> int:
 
 
 class InversionServer:
-    """Admits LBFGS inversion requests into fixed slots and batches their fevs."""
+    """Admits inversion requests into fixed slots and batches their
+    propagator ops.  Each admitted request runs ``machine(req)``: by default
+    LBFGS with the given settings; ``schedule.serving_machine()`` runs the
+    reference's classified multi-stage chains instead (then
+    ``history_size`` should bound the longest LBFGS stage's pairs, which
+    the footprint reserves)."""
 
     def __init__(self, slots: Sequence[Geometry], hbm_budget: int, ckpt_every: int = 50,
                  max_iter: int = 30, history_size: int = 10, tolerance_grad: float = 1e-7,
-                 tolerance_change: float = 0.0, device="cuda"):
+                 tolerance_change: float = 0.0, device="cuda", machine: Callable | None = None):
         self.slots = list(slots)
         self.ckpt_every = ckpt_every
         self.kw = dict(max_iter=max_iter, history_size=history_size, tolerance_grad=tolerance_grad,
                        tolerance_change=tolerance_change)
+        self.machine = machine if machine is not None else (lambda req: lbfgs_machine(req, **self.kw))
         self.budget = int(hbm_budget)
         self.ex = Executor(self.slots, ckpt_every=ckpt_every, device=device)
         self._fp = {g: request_footprint(g, ckpt_every, history_size) for g in set(self.slots)}
@@ -94,6 +100,9 @@ class InversionServer:
         rec.finished = t
         rec.result = value
         rec.stats = dict(req.meta.get("stats", {}))
+        if "family" in req.meta:
+            rec.stats["family"] = req.meta["family"]
+            rec.stats["stages"] = list(req.meta.get("stages", []))
         rec.failed = bool(torch.isnan(value).any())
 
     def run(self, requests: Sequence[Request], arrivals: Sequence[float],
@@ -122,7 +131,7 @@ class InversionServer:
                     blocked.append((req, arr, idx))
                     continue
                 self.ex.set_target(slot, req.target)
-                m = lbfgs_machine(req, **self.kw)
+                m = self.machine(req)
                 out[idx].admitted = now()
                 try:
                     active[slot] = [req, idx, m, next(m)]

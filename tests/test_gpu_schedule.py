@@ -104,3 +104,29 @@ def test_reference_model_classes_on_gpu():
     ref, _ = schedule.infer(clone(), test_mode=True, families=["FlatVel"] * 3)
     for a, b in zip(out, ref):
         assert torch.equal(a, b)
+
+
+def test_served_chains_equal_batched():
+    """Classified chains served with continuous batching (Poisson arrivals,
+    4 slots for 6 requests, checkpointed plans) give the families and the
+    results of schedule.infer on the same requests, bit for bit."""
+    from seismic_amd import serve
+    kw = dict(nz=20, nx=24, nbc=16, nt=300)
+    g, og = Geometry(**kw), fd2d.Geometry(**kw)
+    reqs = []
+    for i in range(6):
+        true = synth.make_model("flat", g.nz, g.nx, 70 + i) if i % 2 == 0 else synth.curve_fault_model(g.nz, g.nx, 70 + i)
+        r = _request(true, synth.to_vec(np.round(true * 0.96), min_vel=1500.0), g, og)
+        r.rid = i
+        reqs.append(r)
+    clone = lambda: [invert.Request(geom=r.geom, target=r.target, prior=None, start=r.start.clone(), rid=r.rid)
+                     for r in reqs]
+    ref, fams = schedule.infer(clone(), test_mode=True)
+    srv = serve.InversionServer([g] * 4, hbm_budget=1 << 40, ckpt_every=50, history_size=100,
+                                machine=schedule.serving_machine(test_mode=True))
+    recs = srv.run(clone(), serve.poisson_arrivals(6, 200.0, seed=3))
+    print("families", fams, [r.stats.get("family") for r in recs], flush=True)
+    assert [r.stats["family"] for r in recs] == fams
+    for rec, b in zip(recs, ref):
+        assert not rec.failed and torch.equal(rec.result, b)
+    assert max(n for _, n in srv.log) > 1          # requests shared launches
