@@ -74,6 +74,7 @@ def _server(slots, budget, fake, footprint=100):
     srv.slots = list(slots)
     srv.ckpt_every = 0
     srv.kw = dict(max_iter=5, history_size=3, tolerance_grad=1e-12, tolerance_change=0.0)
+    srv.machine = lambda req: invert.lbfgs_machine(req, **srv.kw)     # the default machine
     srv.budget = budget
     srv.ex = fake
     srv._fp = {g: footprint for g in set(slots)}
