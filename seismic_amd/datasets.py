@@ -292,11 +292,17 @@ def score_metric(true_vel: torch.Tensor, guess: torch.Tensor, families: Sequence
 # --------------------------------------------------------------------- the pipeline
 def infer_dataset(samples: Sequence[Sample], start: Callable | None = None, cache: ResultCache | None = None,
                   batch: int = 32, test_mode: bool = False, families: Sequence[str] | None = None,
-                  nt: int = 999, device="cuda", ckpt_every: int = 0):
+                  nt: int = 999, device="cuda", ckpt_every: int = 0, served: bool = False,
+                  hbm_budget: int = 200 << 30):
     """Model.infer with ModelSplit (:414-476, core/seis_model.py:129-168)
     over samples on disk: cached results are read back, the rest stream
-    through ``Loader`` and each batch is classified and inverted as one set
-    of chains; new results go to the cache.  ``start(sample, seis_row)``
+    through ``Loader`` and are classified and inverted by their family's
+    chain; new results go to the cache.  By default each loaded batch is
+    one set of chains that finishes before the next starts.  ``served``
+    instead feeds the samples to a continuous-batching server with ``batch``
+    slots (checkpoints every ``ckpt_every`` or 50 steps): a slot takes the
+    next sample as soon as its chain ends, so one long chain holds no batch
+    back; results are the same bit for bit.  ``start(sample, seis_row)``
     gives each starting parameter vector (float64, 4901) -- the reference
     takes it from its network; the default is 1500 -> 4500 m/s linear in
     depth.  Returns (parameter vectors on the device, families; None for
@@ -320,7 +326,30 @@ def infer_dataset(samples: Sequence[Sample], start: Callable | None = None, cach
             results[i] = torch.as_tensor(hit, device=dev)
         else:
             todo.append(i)
-    for b in Loader([samples[i] for i in todo], batch=batch, nt=nt, device=dev, velocities=False):
+    loader = Loader([samples[i] for i in todo], batch=batch, nt=nt, device=dev, velocities=False)
+
+    def done(i, v, fam):
+        results[i], fams[i] = v, fam
+        if cache is not None:
+            cache.put(samples[i], v)
+
+    if served:
+        from . import serve
+
+        def source():        # requests in sample order, read ahead by the loader
+            pos = 0
+            for b in loader:
+                for j, s in enumerate(b.samples):
+                    i = todo[pos + j]
+                    yield Request(geom=geom, target=b.seis[j], prior=None, start=start(s, b.seis[j]), rid=i)
+                pos += len(b.samples)
+        fixed = None if families is None else {i: families[i] for i in todo}
+        srv = serve.InversionServer([geom] * batch, hbm_budget=hbm_budget, ckpt_every=ckpt_every or 50,
+                                    history_size=100, device=dev,
+                                    machine=schedule.serving_machine(test_mode, families=fixed))
+        srv.run_stream(source(), on_event=lambda rec: done(rec.rid, rec.result, rec.stats.get("family")))
+        return results, fams
+    for b in loader:
         idx = todo[:len(b.samples)]
         todo = todo[len(b.samples):]
         reqs = [Request(geom=geom, target=b.seis[j], prior=None, start=start(s, b.seis[j]), rid=j)
@@ -329,9 +358,7 @@ def infer_dataset(samples: Sequence[Sample], start: Callable | None = None, cach
                                  families=None if families is None else [families[i] for i in idx],
                                  ckpt_every=ckpt_every)
         for i, v, fam in zip(idx, vecs, f):
-            results[i], fams[i] = v, fam
-            if cache is not None:
-                cache.put(samples[i], v)
+            done(i, v, fam)
     return results, fams
 
 

@@ -162,6 +162,56 @@ class InversionServer:
         return out
 
 
+    def run_stream(self, source, on_event: Callable | None = None) -> list:
+        """Serve requests pulled from the iterable ``source`` as slots free
+        up: the next request is pulled once the previous one is admitted, so
+        a long request never holds a batch back (a data set streamed from
+        disk keeps every slot busy).  Returns the Served records in
+        admission order; ``on_event`` sees each one as it finishes."""
+        it = iter(source)
+        nxt = next(it, None)
+        out: list = []
+        active: dict = {}
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while nxt is not None or active:
+            while nxt is not None:
+                if nxt.geom not in self.free:
+                    raise ValueError(f"no slot of this server fits a {nxt.geom.nz}x{nxt.geom.nx} request")
+                slot = self._try_admit(nxt)
+                if slot is None:
+                    break
+                rec = Served(rid=nxt.rid, arrival=now(), admitted=now())
+                out.append(rec)
+                self.ex.set_target(slot, nxt.target)
+                m = self.machine(nxt)
+                try:
+                    active[slot] = [nxt, rec, m, next(m)]
+                except StopIteration as e:
+                    self._finish(rec, nxt, e.value, now())
+                    self._release(slot, nxt)
+                    if on_event:
+                        on_event(rec)
+                nxt = next(it, None)
+            if not active:
+                if nxt is not None:
+                    raise ValueError("a request's footprint exceeds the HBM budget of an idle server")
+                break
+            results = self.ex.execute({s: v[3] for s, v in active.items()})
+            self.log.append((now(), len(results)))
+            for s, res in results.items():
+                req, rec, m, _ = active[s]
+                try:
+                    active[s][3] = m.send(res)
+                except StopIteration as e:
+                    self._finish(rec, req, e.value, now())
+                    del active[s]
+                    self._release(s, req)
+                    if on_event:
+                        on_event(rec)
+        return out
+
+
 def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
     """Arrival times of a Poisson process with `rate` requests per second."""
     rng = np.random.default_rng(seed)

@@ -50,7 +50,8 @@ def test_loader_rows_reach_hbm(tmp_path, batch, depth):
 def test_infer_dataset_with_cache(tmp_path):
     """Four layered 70x70 models as test files (nt 300 of the 1000 rows
     used), classified and inverted in batches of 3 (a ragged last batch);
-    equal to schedule.infer on the same requests at once; the second call
+    equal to schedule.infer on the same requests at once and to the served
+    pipeline (3 slots, samples streamed in as slots free up); the second call
     reads every result from the cache; submission and score follow (the
     score against the oracle's)."""
     nt = 300
@@ -74,6 +75,11 @@ def test_infer_dataset_with_cache(tmp_path):
     ref, rfams = schedule.infer(reqs, test_mode=True, batched=True)
     assert rfams == fams
     for a, b in zip(out, ref):
+        assert torch.equal(a, b)
+    served, fams_s = ds.infer_dataset(samples, start=lambda s, seis: start.clone(), batch=3, test_mode=True, nt=nt,
+                                      served=True)
+    assert fams_s == fams
+    for a, b in zip(out, served):
         assert torch.equal(a, b)
     again, fams2 = ds.infer_dataset(samples, cache=cache, batch=3, test_mode=True, nt=nt)
     assert fams2 == [None] * 4
