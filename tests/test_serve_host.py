@@ -144,6 +144,37 @@ def test_nan_request_falls_back_to_start():
     assert not torch.isnan(recs[0].result).any()
 
 
+def test_run_stream_refills_slots_and_matches_alone():
+    """run_stream pulls the next request whenever a slot frees: 6 requests on
+    2 slots all finish in pull order, at most 2 run at once, every slot and
+    byte is released at the end, and each result equals that request run
+    alone; a request with no slot of its size, or over the budget of an
+    idle server, is refused."""
+    g = Geometry(nz=4, nx=4, nbc=2, nt=150)
+    n = 7
+    centers = {i: torch.linspace(0, 1, n, dtype=torch.float64) * (i + 2) for i in range(6)}
+    fake = FakeExecutor()
+    srv = _server([g, g], budget=10 ** 9, fake=fake)
+    pulled = []
+
+    def source():
+        for i in range(6):
+            pulled.append(i)
+            yield _req(g, i, n, centers[i])
+    recs = srv.run_stream(source())
+    assert [r.rid for r in recs] == list(range(6)) and pulled == list(range(6))
+    assert all(not r.failed for r in recs) and max(len(c) for c in fake.calls) <= 2
+    assert srv.used == 0 and sorted(srv.free[g]) == [0, 1]
+    for r in recs:
+        r1 = _server([g], budget=10 ** 9, fake=FakeExecutor()).run([_req(g, r.rid, n, centers[r.rid])], [0.0])[0]
+        assert torch.equal(r.result, r1.result)
+    other = Geometry(nz=5, nx=5, nbc=2, nt=150)
+    with pytest.raises(ValueError):
+        _server([g], budget=10 ** 9, fake=FakeExecutor()).run_stream([_req(other, 0)])
+    with pytest.raises(ValueError):
+        _server([g], budget=10, fake=FakeExecutor()).run_stream([_req(g, 0)])
+
+
 def test_poisson_arrivals_and_percentiles():
     a = serve.poisson_arrivals(2000, 100.0, seed=1)
     assert np.all(np.diff(a) > 0)
