@@ -1326,15 +1326,30 @@ __global__ void k_params(const ModelDev* models, const WaveDev* waves, int n_wav
 
 constexpr int kRedBlocks = 32;  // partial-sum blocks per model (misfit)
 
-// Block-wide fixed-order tree sum of NTHR doubles (deterministic).
+// Fixed-order tree sum over the 64 lanes of a wavefront (cross-lane moves,
+// no LDS); lane 0 ends with the sum.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of NTHR doubles: each wavefront reduces its lanes by
+// shuffles, then the first wavefront reduces the NTHR/64 partials.  The
+// order is fixed, so results are deterministic and independent of what
+// else the launch processes.
 __device__ __forceinline__ double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
+  static_assert(NTHR % 64 == 0 && NTHR <= 64 * 64, "block_sum: whole wavefronts, one partial per lane");
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int k = NTHR / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
+  if (threadIdx.x < 64) {
+    double w = (int)threadIdx.x < NTHR / 64 ? red[threadIdx.x] : 0.0;
+    w = wave_sum(w);
+    if (threadIdx.x == 0) red[NTHR / 64] = w;
   }
-  const double r = red[0];
+  __syncthreads();
+  const double r = red[NTHR / 64];
   __syncthreads();
   return r;
 }
