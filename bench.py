@@ -34,6 +34,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12                     # MI355X HBM3E bytes/s
+# L2 (TCC): 8 XCDs x 16 channels x 128 B/clk x 2.4 GHz (the CDNA3 L2
+# organisation, kept by CDNA4; AMD publishes no MI355X L2 figure)
+L2_PEAK = 8 * 16 * 128 * 2.4e9
 # Algorithmic HBM bytes per cell-update (float32, one padded cell advanced one
 # step for one shot): forward reads p[t-1], p[t], alpha, kappa and writes
 # p[t+1]; the fused tangent also reads dp[t-1], dp[t], dalpha, dkappa and
@@ -140,11 +143,11 @@ def kernel_roofline(ex, B, g, reps=3):
     return out
 
 
-def measured_traffic(name, kd):
-    """HBM bytes per launch of this kernel and config from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary in profiles/, or None when the committed
-    measurement was taken on another config."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+def measured_traffic(name, kd, table="hbm_traffic.json", field="bytes_per_launch"):
+    """HBM (or, with table="l2_traffic.json", L2) bytes per launch of this
+    kernel and config from the rocprofv3 summaries in profiles/, or None
+    when the committed measurement was taken on another config."""
+    path = os.path.join(ROOT, "profiles", table)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -153,7 +156,19 @@ def measured_traffic(name, kd):
     if not e or e.get("window") != kd["window"] or e.get("k") != kd["k"] or e.get("strips") != kd["strips"] \
             or e.get("models") != kd["models"]:
         return None
-    return e["bytes_per_launch"]
+    return e[field] if field in e else None
+
+
+def l2_info(name, kd):
+    """L2 bytes per launch (rocprofv3 TCC_READ_sum / TCC_WRITE_sum, profiles/)
+    over this run's launch time, against L2_PEAK; None on another config."""
+    b = measured_traffic(name, kd, "l2_traffic.json", "l2_bytes_per_launch")
+    if b is None:
+        return None
+    hit = measured_traffic(name, kd, "l2_traffic.json", "l2_hit_rate")
+    achieved = b / kd["launch_s"]
+    return {"traffic": b, "achieved": achieved / 1e9, "peak": L2_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / L2_PEAK, "hit_rate": hit}
 
 
 def cpu_baseline(g, seconds_hint=25.0):
@@ -638,12 +653,13 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "launch_us": kd["launch_s"] * 1e6, "alg_bytes_per_cell_update": ALG_BYTES[dom],
                      "cell_updates_per_launch": kd["cell_updates_per_launch"], "k": kd["k"],
-                     "window": kd["window"], "strips": kd["strips"], "models": kd["models"]},
+                     "window": kd["window"], "strips": kd["strips"], "models": kd["models"],
+                     "l2": l2_info(dom, kd)},
         "kernels": {k: {"kernel": v["kernel"], "launch_us": v["launch_s"] * 1e6, "k": v["k"], "window": v["window"],
                         "strips": v["strips"], "models": v["models"],
                         "gcell_updates_per_s": v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
                         "alg_GBps": ALG_BYTES[k] * v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
-                        "share_of_gn_pass_s": share[k]}
+                        "share_of_gn_pass_s": share[k], "l2": l2_info(k, v)}
                     for k, v in kr.items()},
         "cpu_baseline": cpu,
     }

@@ -4,6 +4,8 @@
     python -m seismic_amd.profsum pmc     <counter_collection.csv> NAME per (kernel, grid) counter means
     python -m seismic_amd.profsum traffic <probe.log> <FETCH_SIZE csv> <WRITE_SIZE csv>
                                           HBM bytes per launch of each kernel_probe pass (JSON)
+    python -m seismic_amd.profsum l2      <probe.log> <TCC_REQ_sum+TCC_HIT_sum csv> <TCC_READ_sum+TCC_WRITE_sum csv>
+                                          L2 bytes and hit rate per launch of each pass (JSON)
 """
 from __future__ import annotations
 
@@ -39,6 +41,27 @@ def trace(path):
               f"{sum(v)/1e6:.2f} | {sum(v)/total:.1%} |")
 
 
+ALG = {"linesearch_forward": 20, "saving_forward": 28, "tangent": 40, "born": 36, "adjoint": 28}
+
+
+def _pattern(name, e):
+    """Kernel-name pattern of a kernel_probe pass with config entry e."""
+    rx, rz = e["window"]
+    if e["strips"] == 2:   # row-streamed kernels: k_fws / k_bns<threads, levels>
+        return rf"k_bns<{rx}, {e['k']}>" if name == "born" else rf"k_fws<{rx}, {e['k']}>"
+    st = "true" if e["strips"] else "false"
+    return {"linesearch_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, false>",
+            "saving_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, true>",
+            "tangent": rf"k_fwd<{rx}, {rz}, \d+, true, {st}, false>",
+            "born": rf"k_born<{rx}, {rz}, \d+, {st}>",
+            "adjoint": rf"k_adj<{rx}, {rz}, \d+, {st}>"}[name]
+
+
+def _probe_info(probe_log):
+    import json
+    return json.loads([line for line in open(probe_log) if line.startswith("{")][-1])
+
+
 def traffic(probe_log, fetch_csv, write_csv):
     """Per pass of seismic_amd.kernel_probe: mean FETCH_SIZE / WRITE_SIZE per
     dispatch of the pass's kernel (matched by name from the probe's configs),
@@ -46,19 +69,9 @@ def traffic(probe_log, fetch_csv, write_csv):
     copyBuffer dispatch of each run)."""
     import json
     import re
-    info = json.loads([line for line in open(probe_log) if line.startswith("{")][-1])
-    alg = {"linesearch_forward": 20, "saving_forward": 28, "tangent": 40, "born": 36, "adjoint": 28}
-
-    def pattern(name, e):
-        rx, rz = e["window"]
-        if e["strips"] == 2:   # row-streamed kernels: k_fws / k_bns<threads, levels>
-            return rf"k_bns<{rx}, {e['k']}>" if name == "born" else rf"k_fws<{rx}, {e['k']}>"
-        st = "true" if e["strips"] else "false"
-        return {"linesearch_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, false>",
-                "saving_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, true>",
-                "tangent": rf"k_fwd<{rx}, {rz}, \d+, true, {st}, false>",
-                "born": rf"k_born<{rx}, {rz}, \d+, {st}>",
-                "adjoint": rf"k_adj<{rx}, {rz}, \d+, {st}>"}[name]
+    info = _probe_info(probe_log)
+    alg = ALG
+    pattern = _pattern
 
     def load(path):
         rows = list(csv.DictReader(open(path)))
@@ -87,6 +100,47 @@ def traffic(probe_log, fetch_csv, write_csv):
     print(json.dumps(out, indent=1))
 
 
+def l2(probe_log, req_csv, rw_csv):
+    """Per pass of seismic_amd.kernel_probe: L2 (TCC) read and write
+    requests, bytes and hit rate per dispatch.  Bytes per read and per write
+    request come from the 1 GiB calibration copy, whose reads and writes all
+    go through the L2 (TCC_READ_sum and TCC_WRITE_sum of its dispatch); the
+    hit rate is TCC_HIT_sum / TCC_REQ_sum from the other run."""
+    import json
+    import re
+    info = _probe_info(probe_log)
+
+    def load(path):
+        rows = list(csv.DictReader(open(path)))
+        by = defaultdict(list)
+        for r in rows:
+            by[r.get("Counter_Name", "")].append(r)
+        return by
+
+    req, rw = load(req_csv), load(rw_csv)
+    copy = lambda rows: max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+    nbytes = info["calibration_copy_bytes"]
+    rb, wb = nbytes / copy(rw["TCC_READ_sum"]), nbytes / copy(rw["TCC_WRITE_sum"])
+    out = {"bytes_per_read_request": round(rb, 2), "bytes_per_write_request": round(wb, 2)}
+    for name in ALG:
+        if name not in info:
+            continue
+        e = dict(info[name])
+        pat = re.compile(_pattern(name, e))
+        mean = lambda rows: (lambda v: sum(v) / len(v) if v else None)(
+            [float(r["Counter_Value"]) for r in rows if pat.search(r["Kernel_Name"])])
+        rd, wr = mean(rw["TCC_READ_sum"]), mean(rw["TCC_WRITE_sum"])
+        rq, ht = mean(req.get("TCC_REQ_sum", [])), mean(req.get("TCC_HIT_sum", []))
+        if rd is None or wr is None:
+            continue
+        e["l2_read_requests"], e["l2_write_requests"] = rd, wr
+        e["l2_bytes_per_launch"] = int(round(rd * rb + wr * wb))
+        if rq and ht is not None:
+            e["l2_hit_rate"] = round(ht / rq, 4)
+        out[name] = e
+    print(json.dumps(out, indent=1))
+
+
 def pmc(path, counter):
     rows = list(csv.DictReader(open(path)))
     agg = defaultdict(list)
@@ -106,5 +160,7 @@ if __name__ == "__main__":
         trace(sys.argv[2])
     elif sys.argv[1] == "traffic":
         traffic(sys.argv[2], sys.argv[3], sys.argv[4])
+    elif sys.argv[1] == "l2":
+        l2(sys.argv[2], sys.argv[3], sys.argv[4])
     else:
         pmc(sys.argv[2], sys.argv[3])
