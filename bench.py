@@ -220,11 +220,13 @@ def _cpu_fev(a):
     return fd2d.vel_to_seis(vec, og, vec_adjoint=tgt, adjoint_on_residual=True, shots=[s])[2]
 
 
-def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep_requests=36):
+def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep_requests=36, ckpt_every=None):
     """Served LBFGS inversions (30 iterations, memory 10, smoothness prior) of
     mixed grid sizes arriving as a Poisson process: `n_req` requests at
     `rate` on as many slots, then a rate sweep with `sweep_requests` requests
-    on the same slots (requests queue for a free slot of their size)."""
+    on the same slots (requests queue for a free slot of their size).  The
+    server keeps full forward histories when they fit its 200 GiB budget
+    (ckpt_every=None), else checkpoints every 50 steps."""
     import torch
     from seismic_amd import invert, priors, serve, synth
     from seismic_amd.geometry import Geometry
@@ -263,7 +265,7 @@ def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep
         return out
 
     slots = [geoms[sizes[si]] for si in size_mix(n_req)]
-    srv = serve.InversionServer(slots, hbm_budget=200 * 2 ** 30, ckpt_every=50, max_iter=30, history_size=10)
+    srv = serve.InversionServer(slots, hbm_budget=200 * 2 ** 30, ckpt_every=ckpt_every, max_iter=30, history_size=10)
     srv.kw["max_iter"] = 1                      # capture the server's graphs before timing
     srv.run(make_reqs(n_req), [0.0] * n_req)
     srv.kw["max_iter"] = 30
@@ -278,6 +280,7 @@ def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         return {"requests": n, "rate_req_per_s": r, "inversions_per_s": n / wall, "wall_s": wall,
+                "ckpt_every": srv.ckpt_every,
                 "latency_s": serve.percentiles([x.latency for x in recs]),
                 "queue_wait_s": serve.percentiles([x.admitted - x.arrival for x in recs]),
                 "max_in_flight": max(k for _, k in srv.log) if srv.log else 0,
@@ -430,8 +433,9 @@ def schedule_run(device, n_each=8):
 def served_schedule_run(device, n_each=8, slots=12, rate=50.0):
     """The classified chains of schedule_run served with continuous batching:
     2 n_each requests arriving as a Poisson process at `rate` on `slots`
-    70x70 slots (later arrivals wait for a free slot), each classified on
-    admission and run through its family's chain (test-mode stage lengths)."""
+    70x70 slots (later arrivals wait for a free slot; full forward history,
+    which fits), each classified on admission and run through its family's
+    chain (test-mode stage lengths)."""
     import torch
     from seismic_amd import invert, schedule, serve, synth
     from seismic_amd.geometry import Geometry
@@ -449,7 +453,7 @@ def served_schedule_run(device, n_each=8, slots=12, rate=50.0):
                                               dtype=torch.float64, device=device), rid=i)
             for i, t in enumerate(trues)]
     arrivals = serve.poisson_arrivals(len(reqs), rate, seed=7)
-    srv = serve.InversionServer([g] * slots, hbm_budget=200 << 30, ckpt_every=50, history_size=100,
+    srv = serve.InversionServer([g] * slots, hbm_budget=200 << 30, ckpt_every=None, history_size=100,
                                 machine=schedule.serving_machine(test_mode=True), device=device)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -300,9 +300,10 @@ def infer_dataset(samples: Sequence[Sample], start: Callable | None = None, cach
     chain; new results go to the cache.  By default each loaded batch is
     one set of chains that finishes before the next starts.  ``served``
     instead feeds the samples to a continuous-batching server with ``batch``
-    slots (checkpoints every ``ckpt_every`` or 50 steps): a slot takes the
-    next sample as soon as its chain ends, so one long chain holds no batch
-    back; results are the same bit for bit.  ``start(sample, seis_row)``
+    slots (full forward history when the slots fit ``hbm_budget``, else
+    checkpoints; every ``ckpt_every`` steps when that is set): a slot takes
+    the next sample as soon as its chain ends, so one long chain holds no
+    batch back; results are the same bit for bit.  ``start(sample, seis_row)``
     gives each starting parameter vector (float64, 4901) -- the reference
     takes it from its network; the default is 1500 -> 4500 m/s linear in
     depth.  Returns (parameter vectors on the device, families; None for
@@ -344,7 +345,7 @@ def infer_dataset(samples: Sequence[Sample], start: Callable | None = None, cach
                     yield Request(geom=geom, target=b.seis[j], prior=None, start=start(s, b.seis[j]), rid=i)
                 pos += len(b.samples)
         fixed = None if families is None else {i: families[i] for i in todo}
-        srv = serve.InversionServer([geom] * batch, hbm_budget=hbm_budget, ckpt_every=ckpt_every or 50,
+        srv = serve.InversionServer([geom] * batch, hbm_budget=hbm_budget, ckpt_every=ckpt_every or None,
                                     history_size=100, device=dev,
                                     machine=schedule.serving_machine(test_mode, families=fixed))
         srv.run_stream(source(), on_event=lambda rec: done(rec.rid, rec.result, rec.stats.get("family")))

@@ -59,12 +59,22 @@ class InversionServer:
     LBFGS with the given settings; ``schedule.serving_machine()`` runs the
     reference's classified multi-stage chains instead (then
     ``history_size`` should bound the longest LBFGS stage's pairs, which
-    the footprint reserves)."""
+    the footprint reserves).  ``ckpt_every=None`` keeps the full forward
+    history of every slot when all of them fit ``hbm_budget`` and
+    checkpoints every 50 steps otherwise; results are bit-identical either
+    way, the full history saves the adjoint's segment recomputes."""
 
-    def __init__(self, slots: Sequence[Geometry], hbm_budget: int, ckpt_every: int = 50,
+    def __init__(self, slots: Sequence[Geometry], hbm_budget: int, ckpt_every: int | None = 50,
                  max_iter: int = 30, history_size: int = 10, tolerance_grad: float = 1e-7,
                  tolerance_change: float = 0.0, device="cuda", machine: Callable | None = None):
         self.slots = list(slots)
+        if ckpt_every is None:
+            # keep every slot's full forward history when all of them fit the
+            # budget (the adjoint then reads it instead of recomputing
+            # checkpoint segments), else checkpoint every 50 steps
+            caps = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT
+            full = sum(plan_bytes([g], ckpt_every=0, caps=caps) for g in self.slots)
+            ckpt_every = 0 if full <= int(hbm_budget) else 50
         self.ckpt_every = ckpt_every
         self.kw = dict(max_iter=max_iter, history_size=history_size, tolerance_grad=tolerance_grad,
                        tolerance_change=tolerance_change)
@@ -160,7 +170,6 @@ class InversionServer:
                     if on_event:
                         on_event(rec)
         return out
-
 
     def run_stream(self, source, on_event: Callable | None = None) -> list:
         """Serve requests pulled from the iterable ``source`` as slots free

@@ -49,3 +49,24 @@ def test_served_lbfgs_equals_alone():
     recs2 = srv2.run_stream(iter(_clone(reqs)))
     for rec in recs2:
         assert torch.equal(rec.result, alone[rec.rid]), rec.rid
+
+
+def test_server_keeps_full_history_when_it_fits():
+    """ckpt_every=None keeps every slot's full forward history when all of
+    them fit the budget (one byte less and it checkpoints every 50 steps);
+    results are those of the plain run either way."""
+    from seismic_amd._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
+    from seismic_amd.propagator import plan_bytes
+    geoms, reqs = _requests()
+    slots = [geoms[0], geoms[1], geoms[0], geoms[1]]
+    full = sum(plan_bytes([g], ckpt_every=0, caps=CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT) for g in slots)
+    kw = dict(max_iter=8, history_size=5, tolerance_grad=1e-7, tolerance_change=0.0)
+    small = serve.InversionServer(slots, hbm_budget=full - 1, ckpt_every=None, device="cuda", **kw)
+    assert small.ckpt_every == 50
+    small.ex.close()
+    srv = serve.InversionServer(slots, hbm_budget=1 << 40, ckpt_every=None, device="cuda", **kw)
+    assert srv.ckpt_every == 0
+    alone = invert.invert(_clone(reqs), method="lbfgs", batched=False, **kw)
+    recs = srv.run(_clone(reqs), serve.poisson_arrivals(6, 300.0, seed=4))
+    for rec, a in zip(recs, alone):
+        assert torch.equal(rec.result, a), rec.rid
