@@ -6,6 +6,8 @@
                                           HBM bytes per launch of each kernel_probe pass (JSON)
     python -m seismic_amd.profsum l2      <probe.log> <TCC_REQ_sum+TCC_HIT_sum csv> <TCC_READ_sum+TCC_WRITE_sum csv>
                                           L2 bytes and hit rate per launch of each pass (JSON)
+    python -m seismic_amd.profsum counters <probe.log> <counter csv>
+                                          mean of every counter per dispatch of each pass (JSON)
 """
 from __future__ import annotations
 
@@ -141,6 +143,29 @@ def l2(probe_log, req_csv, rw_csv):
     print(json.dumps(out, indent=1))
 
 
+def counters(probe_log, path):
+    """Per pass of seismic_amd.kernel_probe: the mean of every counter in
+    the run per dispatch of the pass's kernel."""
+    import json
+    import re
+    info = _probe_info(probe_log)
+    rows = list(csv.DictReader(open(path)))
+    names = sorted({r["Counter_Name"] for r in rows})
+    out = {}
+    for name in ALG:
+        if name not in info:
+            continue
+        e = dict(info[name])
+        pat = re.compile(_pattern(name, e))
+        for c in names:
+            v = [float(r["Counter_Value"]) for r in rows if r["Counter_Name"] == c and pat.search(r["Kernel_Name"])]
+            if v:
+                e[c] = sum(v) / len(v)
+                e["dispatches"] = len(v)
+        out[name] = e
+    print(json.dumps(out, indent=1))
+
+
 def pmc(path, counter):
     rows = list(csv.DictReader(open(path)))
     agg = defaultdict(list)
@@ -162,5 +187,7 @@ if __name__ == "__main__":
         traffic(sys.argv[2], sys.argv[3], sys.argv[4])
     elif sys.argv[1] == "l2":
         l2(sys.argv[2], sys.argv[3], sys.argv[4])
+    elif sys.argv[1] == "counters":
+        counters(sys.argv[2], sys.argv[3])
     else:
         pmc(sys.argv[2], sys.argv[3])
