@@ -143,6 +143,17 @@ def kernel_roofline(ex, B, g, reps=3):
     return out
 
 
+def per_step_configs(P):
+    """Every kernel kind of plan P at one time step per launch (64x32
+    windows, round-robin rows): the reference's launch pattern, one kernel
+    per step, on this library's kernels."""
+    for kind in ("forward", "record", "tangent", "adjoint", "born"):
+        try:
+            P.set_config(kind, 64, 32, 1, 0)
+        except (ValueError, RuntimeError):
+            pass                          # a kind this plan does not run
+
+
 def measured_traffic(name, kd, table="hbm_traffic.json", field="bytes_per_launch"):
     """HBM (or, with table="l2_traffic.json", L2) bytes per launch of this
     kernel and config from the rocprofv3 summaries in profiles/, or None
@@ -588,9 +599,19 @@ def main():
     t_p, v_p = run_gn(ex_p, preq, args.plain_steps)
     plain = 1 * (args.plain_steps / N_OUTER) / t_p
     log(f"[bench] plain: {t_p:.2f} s, {plain:.4f} inversions/s")
+    ex_p.close()
+    # the same plain inversion shaped like the reference's loop: one time step
+    # per launch (k = 1, 64x32 windows, no temporal blocking) inside the
+    # captured graphs
+    ex_s = invert.Executor([g], ckpt_every=0, max_batch=1, configure=per_step_configs)
+    run_gn(ex_s, preq, 1)
+    log("[bench] timing plain path with one step per launch, 1 GN pass")
+    t_s, _ = run_gn(ex_s, preq, 1)
+    plain_step = (1 / N_OUTER) / t_s
+    log(f"[bench] plain, one step per launch: {t_s:.2f} s per GN pass, {plain_step:.4f} inversions/s")
+    ex_s.close()
 
     ex_b.close()
-    ex_p.close()
     torch.cuda.empty_cache()
     extras = {}
     if not args.no_extras:
@@ -646,6 +667,9 @@ def main():
                    "batch_models": B, "step": "one GN pass over the batch", "ckpt_every": 0},
         "plain": {"inversions_per_s": plain, "seconds_per_gn_pass": t_p / args.plain_steps},
         "speedup_vs_plain": value / plain,
+        "plain_one_step_per_launch": {"inversions_per_s": plain_step, "seconds_per_gn_pass": t_s,
+                                      "speedup": value / plain_step,
+                                      "configs": "k=1, 64x32 windows, every kernel kind"},
         "misfit": {"batched_mean_final": float(np.mean(mis_b)), "after_gn_passes": args.steps,
                    "model0_batched": hist_b0[:args.plain_steps],
                    "model0_plain": preq[0].meta["stats"]["misfit_hist"],

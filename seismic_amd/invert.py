@@ -366,8 +366,10 @@ class Executor:
     """
 
     def __init__(self, slots: Sequence[Geometry], ckpt_every: int = 0, max_batch: int | None = None,
-                 fwd_fanout: int = 50, reuse_forward: bool = True, device="cuda"):
+                 fwd_fanout: int = 50, reuse_forward: bool = True, device="cuda",
+                 configure: Callable | None = None):
         self.slots = list(slots)
+        self.configure = configure    # called on each new plan after the tuned configs are applied
         self.n = len(self.slots)
         self.max_batch = self.n if max_batch is None else max_batch
         self.ckpt_every = ckpt_every
@@ -388,6 +390,8 @@ class Executor:
         if key not in self._plans:
             P = Propagator(geoms, ckpt_every=self.ckpt_every, caps=caps, device=self.device)
             autotune.apply(P)
+            if self.configure is not None:
+                self.configure(P)
             self._plans[key] = P
         return self._plans[key]
 
