@@ -46,7 +46,6 @@ CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WI
 # whole columns), k levels
 CANDIDATES["forward"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 40, 80, 160) for k in (3, 4)]
 CANDIDATES["born"] += [(rx, rz, k, 2) for rx in (64, 128) for rz in (0, 80, 160) for k in (3, 4)]
-CANDIDATES["record"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 80, 160) for k in (3, 4)]
 # tuning order: the adjoint and Born passes run after the forward that
 # records their history
 KINDS = ("forward", "record", "tangent", "adjoint", "born")

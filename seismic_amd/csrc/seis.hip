@@ -484,9 +484,7 @@ __device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float ksca
 // work; receiver samples are kept in registers and stored after the march.  Same stencil order and
 // fused multiply-adds as k_fwd, so wavefields and seismograms are
 // bit-identical to it.  Launches always advance KL steps (a shorter last
-// launch runs k_fwd).  With HIST (the saving forward) level l also stores
-// the pair (Lf(p[n]), q[n]), n = n0 + l - 1, of every committed cell into
-// R_HIST slot hist_slot0 + l - 1, the values k_fwd stores.
+// launch runs k_fwd).
 // Row pointer base + ro (bytes), kept in scalar registers and opaque to the
 // compiler, and element of a row at a 32-bit lane byte offset: every access
 // of the streamed kernel is then one scalar-based address (saddr form)
@@ -501,7 +499,7 @@ __device__ __forceinline__ gfloat* lane(gfloat* row, unsigned off) {
   return reinterpret_cast<gfloat*>(reinterpret_cast<__attribute__((address_space(1))) char*>(row) + off);
 }
 
-template <int NT, int KL, bool HIST = false>
+template <int NT, int KL>
 __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
   static_assert(2 * KL + SPD < SUN && SUN % 6 == 0 && SUN % SPD == 0, "pipeline rings too short");
   constexpr int H = 2 * KL;
@@ -530,8 +528,6 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
     float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
     float* seis = a.write_seis ? a.seis + w.seis_off : nullptr;
-    // R_HIST slot of the pairs of n0 (float2 per cell; unused without HIST)
-    const float* hist0 = HIST ? a.arena + w.off[R_HIST] + 2 * (int64_t)a.hist_slot0 * Pn : nullptr;
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float amp = a.wparam[te.w].amp;
@@ -609,15 +605,6 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
         t2 = t2 + (pr[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
         const float L = fmaf(kC2, t1, kC3 * t2);
         const float2 ak = AK[sr][c];
-        if constexpr (HIST) {  // pair of n0 + l - 1 of a committed cell
-          if (u >= H && u < H + IZ) {
-            gfloat* const hr = rowp(hist0 + 2 * (int64_t)(l - 1) * Pn, (int64_t)(oz + u) * rowb * 2);
-            if (colint) {
-              *lane(hr, 2 * gx4) = L;
-              *lane(hr, 2 * gx4 + 4) = qr[l - 1][s3];
-            }
-          }
-        }
         float qn = fmaf(ak.y, qr[l - 1][s3], ak.x * L);
         if constexpr (SPECIAL) {
           if (u == su0 || u == su1) qn = srcc ? qn + srcv[l - 1] : qn;
@@ -1605,7 +1592,7 @@ bool stream_cfg(int kind, const KCfg& c) {
   bool nt_ok = false;
   for (int nt : kStreamNT) nt_ok = nt_ok || c.rx == nt;
   if (!nt_ok || !(c.rz == 0 || c.rz >= 8)) return false;
-  if (kind == SEIS_KIND_FORWARD || kind == SEIS_KIND_RECORD) return c.k >= kStreamKMin && c.k <= kStreamKMax;
+  if (kind == SEIS_KIND_FORWARD) return c.k >= kStreamKMin && c.k <= kStreamKMax;
   if (kind == SEIS_KIND_BORN) return c.k >= kStreamBornKMin && c.k <= kStreamBornKMax;
   return false;
 }
@@ -1808,16 +1795,15 @@ const void* fwd_ptr(const KCfg& c) {
 }
 
 template <int NT>
-const void* fws_ptr_nt(int k, bool hist) {
-  if (hist) return k == 3 ? (const void*)k_fws<NT, 3, true> : (const void*)k_fws<NT, 4, true>;
+const void* fws_ptr_nt(int k) {
   return k == 3 ? (const void*)k_fws<NT, 3> : (const void*)k_fws<NT, 4>;
 }
 
-const void* fws_ptr(const KCfg& c, bool hist) {
-  if (c.rx == 64) return fws_ptr_nt<64>(c.k, hist);
-  if (c.rx == 128) return fws_ptr_nt<128>(c.k, hist);
-  if (c.rx == 192) return fws_ptr_nt<192>(c.k, hist);
-  return fws_ptr_nt<256>(c.k, hist);
+const void* fws_ptr(const KCfg& c) {
+  if (c.rx == 64) return fws_ptr_nt<64>(c.k);
+  if (c.rx == 128) return fws_ptr_nt<128>(c.k);
+  if (c.rx == 192) return fws_ptr_nt<192>(c.k);
+  return fws_ptr_nt<256>(c.k);
 }
 
 template <int NT>
@@ -1833,7 +1819,7 @@ const void* bns_ptr(const KCfg& c) {
 }
 
 const void* kernel_ptr(int kind, const KCfg& c) {
-  if (c.strips == 2) return kind == SEIS_KIND_BORN ? bns_ptr(c) : fws_ptr(c, kind == SEIS_KIND_RECORD);
+  if (c.strips == 2) return kind == SEIS_KIND_BORN ? bns_ptr(c) : fws_ptr(c);
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
 #define PTR_R(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, true>(c);
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
@@ -1868,14 +1854,8 @@ void launch_fwd_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
 
 template <int NT>
 void launch_fws_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
-  const bool hist = a.hist_slot0 >= 0;
-  if (c.k == 3) {
-    if (hist) hipLaunchKernelGGL((k_fws<NT, 3, true>), dim3(grid), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_fws<NT, 3>), dim3(grid), dim3(NT), 0, s, a);
-  } else {
-    if (hist) hipLaunchKernelGGL((k_fws<NT, 4, true>), dim3(grid), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_fws<NT, 4>), dim3(grid), dim3(NT), 0, s, a);
-  }
+  if (c.k == 3) hipLaunchKernelGGL((k_fws<NT, 3>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_fws<NT, 4>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
@@ -2011,14 +1991,6 @@ FwdArgs fwd_args(seis_plan* p, int kind) {
   return a;
 }
 
-// Kind whose (tiled) window and tile list run the launches shorter than a
-// streamed config's k: the record kind's, or the adjoint kind's when the
-// record kind streams too (the adjoint windows are forward windows, and a
-// tile list built for k steps serves any K <= k).
-int short_kind(const seis_plan* p) {
-  return p->cfg[SEIS_KIND_RECORD].strips == 2 ? SEIS_KIND_ADJOINT : SEIS_KIND_RECORD;
-}
-
 // Forward time loop from the zero state at n = 0 through n = nt (p[1] .. p[nt+1],
 // core/seis_forward2.py:211-247), fused with the tangent when `tan`
 // (:255-283).  A saving forward (`save`) on a full-history plan stores the
@@ -2044,10 +2016,9 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   FwdArgs a = fwd_args(p, kind);
   a.write_seis = 1;
   // a streamed forward always advances c.k steps; shorter launches (the last
-  // one, or up to a checkpoint) run k_fwd with a tiled kind's window
-  const int sk = short_kind(p);
-  const KCfg& cr = p->cfg[sk];
-  FwdArgs ar = fwd_args(p, sk);
+  // one, or up to a checkpoint) run k_fwd with the record kind's window
+  const KCfg& cr = p->cfg[SEIS_KIND_RECORD];
+  FwdArgs ar = fwd_args(p, SEIS_KIND_RECORD);
   int n = 0, li = 0;
   while (n < p->nt + 1) {
     int K = std::min(c.k, p->nt + 1 - n);
@@ -2062,7 +2033,7 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
     f.p_out_slot = f.q_out_slot = f.dp_out_slot = f.dq_out_slot = (li + 1) & 1;
     f.hist_slot0 = hist ? n : -1;
     f.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
-    if (tail) launch_fwd(cr, false, p->tl[sk].grid, s, f);
+    if (tail) launch_fwd(cr, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
     else launch_fwd(c, tan, p->tl[kind].grid, s, f);
     n += K;
     ++li;
@@ -2149,12 +2120,6 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
   const KCfg& cf = p->cfg[SEIS_KIND_RECORD];
   FwdArgs f = fwd_args(p, SEIS_KIND_RECORD);
   f.write_seis = 0;
-  // a streamed record config advances cf.k steps per launch; shorter ones
-  // (a segment's end) run k_fwd with a tiled kind's window
-  const int sk = short_kind(p);
-  const KCfg& cs = p->cfg[sk];
-  FwdArgs fs = fwd_args(p, sk);
-  fs.write_seis = 0;
   for (int j = L.J - 1; j >= 0; --j) {
     const int lo = j * L.Kc;
     const int hi = std::min(lo + L.Kc, p->nt);  // largest n whose pair the segment needs
@@ -2162,17 +2127,13 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
                        p->d_models, p->n_waves, p->d_arena, j);
     int n = lo, fi = 0;
     while (n <= hi) {
-      int K = std::min(cf.k, hi + 1 - n);
-      const bool tail = cf.strips == 2 && K < cf.k;
-      if (tail) K = std::min(K, cs.k);
-      FwdArgs& fa = tail ? fs : f;
-      fa.n0 = n;
-      fa.K = K;
-      fa.p_in_slot = fa.q_in_slot = fi & 1;
-      fa.p_out_slot = fa.q_out_slot = (fi + 1) & 1;
-      fa.hist_slot0 = n - lo;
-      if (tail) launch_fwd(cs, false, p->tl[sk].grid, s, fa);
-      else launch_fwd(cf, false, p->tl[SEIS_KIND_RECORD].grid, s, fa);
+      const int K = std::min(cf.k, hi + 1 - n);
+      f.n0 = n;
+      f.K = K;
+      f.p_in_slot = f.q_in_slot = fi & 1;
+      f.p_out_slot = f.q_out_slot = (fi + 1) & 1;
+      f.hist_slot0 = n - lo;
+      launch_fwd(cf, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
       n += K;
       ++fi;
     }

@@ -139,9 +139,9 @@ class Propagator:
     def set_config(self, kind: str, rx: int, rz: int, k: int, strips: int | None = None):
         """Launch config of one kernel kind: rx x rz LDS window, k steps per
         launch, row ownership (1 = consecutive rows per wavefront, 0 =
-        round-robin; None keeps the current one).  strips = 2 (forward,
-        record and Born kinds) streams rx-column strips down rz-row segments
-        (rz = 0: whole columns) with k pipelined time levels."""
+        round-robin; None keeps the current one).  strips = 2 (forward and
+        Born kinds) streams rx-column strips down rz-row segments (rz = 0:
+        whole columns) with k pipelined time levels."""
         if strips is None:
             strips = self.get_config(kind)[3]
         _lib.check(_lib.lib().seis_plan_set_config(self._h, self.KINDS[kind], rx, rz, k, int(strips)),

@@ -194,35 +194,6 @@ def test_streamed_forward_bit_identical(small_geoms, cfg):
     assert rel(out, ref_o) < SEIS_TOL
 
 
-@pytest.mark.parametrize("cfg", [(128, 0, 4, 2), (128, 0, 3, 2), (192, 23, 3, 2), (64, 13, 4, 2)])
-def test_streamed_record_bit_identical(small_geoms, cfg):
-    """The streamed saving forward (the streamed kernel stores the history
-    pairs; launches shorter than k run the adjoint kind's tiled window) gives
-    the misfits and gradients of the tiled saving forward, with full history
-    and with checkpoint-segment recomputes, also with the forward kind
-    streamed too; the Born pass on its saved history equals the fused
-    tangent."""
-    g, og = small_geoms
-    v, vec = _case(g, "curved", 25)
-    target = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 1.01), og)[0])
-    rng = np.random.default_rng(7)
-    dv = torch.tensor(np.concatenate([rng.normal(0, 10, g.nz * g.nx), [1.0]]))
-    for ckpt in (0, 17):
-        P = Propagator([g], ckpt_every=ckpt)
-        _, m0, gref = P.gradient(torch.tensor(vec), target)
-        dref = _np(P.tangent(torch.tensor(vec), dv)[1]) if ckpt == 0 else None
-        P.set_config("record", *cfg)
-        assert P.get_config("record") == cfg
-        for fwd in (None, (128, 0, 4, 2)):
-            if fwd:
-                P.set_config("forward", *fwd)
-            _, m1, gr = P.gradient(torch.tensor(vec), target)
-            assert np.array_equal(_np(gr), _np(gref)), (cfg, ckpt, fwd)
-            assert float(m1[0]) == float(m0[0])
-            if ckpt == 0:
-                assert np.array_equal(_np(P.tangent_saved(torch.tensor(vec), dv)), dref), (cfg, fwd)
-
-
 def test_streamed_forward_ragged_batch():
     """Mixed grid shapes, shot counts and source / receiver rows (incl. the
     surface row) in one streamed launch, against the tiled kernel; the
@@ -245,9 +216,7 @@ def test_streamed_forward_ragged_batch():
     with pytest.raises(ValueError):
         Ps.set_config("forward", 128, 0, 4, 2)  # 14 < 16
     with pytest.raises(ValueError):
-        P.set_config("record", 128, 0, 5, 2)    # streamed record: 3 or 4 levels
-    with pytest.raises(ValueError):
-        P.set_config("tangent", 128, 0, 4, 2)   # streamed: forward, record and Born kinds only
+        P.set_config("record", 128, 0, 4, 2)    # streamed: plain forward and Born kinds only
     with pytest.raises(ValueError):
         P.set_config("born", 128, 0, 5, 2)      # streamed Born: 3 or 4 levels
 
