@@ -185,3 +185,13 @@ def test_score_matches_oracle():
     oscore, oper, ores = of.score_metric(list(t), list(g), fams)
     assert res == pytest.approx(ores, rel=1e-14) and per == pytest.approx(oper, rel=1e-14)
     assert score == pytest.approx(oscore, rel=1e-14)
+
+
+def test_submission_and_loader_edges(tree, tmp_path):
+    """The writer refuses grids that are not 70x70; the loader reports its
+    batch count and yields nothing for an empty sample list."""
+    with pytest.raises(ValueError):
+        ds.write_submission([ds.Sample("/d/test/f.npy")], [np.zeros((70, 71))], str(tmp_path / "x.csv"))
+    train = ds.load_train(tree)
+    assert len(ds.Loader(train, batch=3, device="cpu")) == (len(train) + 2) // 3
+    assert list(ds.Loader([], batch=3, device="cpu")) == []
