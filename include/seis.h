@@ -219,6 +219,22 @@ int seis_label_flat(const int32_t* nz /* host */, const int32_t* nx /* host */,
                     int32_t n_models, const double* vel, double threshold,
                     int32_t* labels, int32_t* counts, seis_stream_t stream);
 
+/* seis_label_flat without any device allocation: the grid table is staged
+ * in a fixed device array of the library (one per GPU, held until the
+ * process ends) instead of a stream-ordered allocation per call, with the
+ * same arguments, results and errors as seis_label_flat, plus SEIS_EINVAL
+ * for more than SEIS_LABEL_STATIC_MAX_GRIDS grids.  Every call on one GPU
+ * uses the same array: calls that overlap on one GPU (other streams, other
+ * threads) race, so callers serialise them, e.g. on one stream.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (serving loops and harnesses that forbid hipMalloc).  The
+ * table is copied from host memory the call frees on return, so the call is
+ * not meant for stream capture. */
+#define SEIS_LABEL_STATIC_MAX_GRIDS 4096
+int seis_label_flat_static(const int32_t* nz /* host */, const int32_t* nx /* host */, int32_t n_models,
+                           const double* vel, double threshold, int32_t* labels, int32_t* counts,
+                           seis_stream_t stream);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* seis_last_error(void);
 

@@ -12,6 +12,7 @@ CAP_TANGENT = 2
 CAP_ADJOINT = 4
 KIND_FORWARD, KIND_TANGENT, KIND_ADJOINT, KIND_BORN, KIND_RECORD = 0, 1, 2, 3, 4
 MAX_SHOTS = 16
+LABEL_STATIC_MAX_GRIDS = 4096
 
 ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "HIP error",
           -4: "operation not enabled for this plan"}
@@ -50,6 +51,8 @@ EXPORTS = {
                                        POINTER(c_int32)]),
     "seis_label_flat": (c_int32, [POINTER(c_int32), POINTER(c_int32), c_int32, c_void_p, c_double, c_void_p,
                                   c_void_p, c_void_p]),
+    "seis_label_flat_static": (c_int32, [POINTER(c_int32), POINTER(c_int32), c_int32, c_void_p, c_double, c_void_p,
+                                         c_void_p, c_void_p]),
     "seis_last_error": (c_char_p, []),
 }
 

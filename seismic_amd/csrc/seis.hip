@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -2217,6 +2218,73 @@ int copy_out(seis_plan* p, hipStream_t s, const float* src, float* dst) {
     if (rc_ != SEIS_OK) return rc_; \
   } while (0)
 
+// Grid table of a labelling call: validated sizes, per-grid offsets into the
+// concatenated cells (int64) and (nz, nx) pairs, staged on the device as
+// offsets followed by sizes.
+int label_args(const int32_t* nz, const int32_t* nx, int32_t n_models, const double* vel, double threshold,
+               const int32_t* labels, const int32_t* counts, std::vector<int64_t>& offs, std::vector<int>& dims) {
+  if (!nz || !nx || n_models < 1 || !vel || !labels || !counts) return fail(SEIS_EINVAL, "NULL argument or no grids");
+  if (!(threshold >= 0.0)) return fail(SEIS_EINVAL, "threshold must be >= 0");
+  dims.assign(2 * (size_t)n_models, 0);
+  offs.assign(n_models, 0);
+  int64_t off = 0;
+  for (int m = 0; m < n_models; ++m) {
+    if (nz[m] < 1 || nx[m] < 1 || (int64_t)nz[m] * nx[m] > kLabelMax)
+      return fail(SEIS_EINVAL, "grid " + std::to_string(m) + ": need 1 <= nz*nx <= " + std::to_string(kLabelMax));
+    dims[2 * m] = nz[m];
+    dims[2 * m + 1] = nx[m];
+    offs[m] = off;
+    off += (int64_t)nz[m] * nx[m];
+  }
+  return SEIS_OK;
+}
+
+// Stage the grid table at device address d (8-byte aligned, room for
+// 12 bytes per grid) on stream s and launch one labelling workgroup per grid.
+int launch_label(char* d, const std::vector<int64_t>& offs, const std::vector<int>& dims, const double* vel,
+                 double threshold, int32_t* labels, int32_t* counts, hipStream_t s) {
+  const size_t bo = sizeof(int64_t) * offs.size(), bd = sizeof(int) * dims.size();
+  HIP_TRY(hipMemcpyAsync(d, offs.data(), bo, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + bo, dims.data(), bd, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_label, dim3((unsigned)offs.size()), dim3(kLabelThreads), 0, s,
+                     reinterpret_cast<const int*>(d + bo), reinterpret_cast<const int64_t*>(d), vel, threshold,
+                     labels, counts);
+  HIP_TRY(hipGetLastError());
+  return SEIS_OK;
+}
+
+int static_scratch(char** out);
+
+}  // namespace
+
+// ---------------------------------------------------------------- allocation-free labelling
+// Device staging of seis_label_flat_static: the grid table of one call (12
+// bytes per grid).  Each GPU holds its own copy of this array until the
+// process ends, and every call on that GPU stages into it.  Host code never
+// uses the array's name as a device address: static_scratch looks the
+// current device's copy up once (hipGetSymbolAddress) and hands out that.
+static __device__ __align__(256) char scratch[SEIS_LABEL_STATIC_MAX_GRIDS * (sizeof(int64_t) + 2 * sizeof(int))];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SEIS_EINVAL, "device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return SEIS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2291,30 +2359,29 @@ int seis_plan_destroy(seis_plan_t* plan) {
 
 int seis_label_flat(const int32_t* nz, const int32_t* nx, int32_t n_models, const double* vel, double threshold,
                     int32_t* labels, int32_t* counts, seis_stream_t stream) {
-  if (!nz || !nx || n_models < 1 || !vel || !labels || !counts) return fail(SEIS_EINVAL, "NULL argument or no grids");
-  if (!(threshold >= 0.0)) return fail(SEIS_EINVAL, "threshold must be >= 0");
-  std::vector<int> dims(2 * (size_t)n_models);
-  std::vector<int64_t> offs(n_models);
-  int64_t off = 0;
-  for (int m = 0; m < n_models; ++m) {
-    if (nz[m] < 1 || nx[m] < 1 || (int64_t)nz[m] * nx[m] > kLabelMax)
-      return fail(SEIS_EINVAL, "grid " + std::to_string(m) + ": need 1 <= nz*nx <= " + std::to_string(kLabelMax));
-    dims[2 * m] = nz[m];
-    dims[2 * m + 1] = nx[m];
-    offs[m] = off;
-    off += (int64_t)nz[m] * nx[m];
-  }
+  std::vector<int64_t> offs;
+  std::vector<int> dims;
+  TRY(label_args(nz, nx, n_models, vel, threshold, labels, counts, offs, dims));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const size_t bd = sizeof(int) * dims.size(), bo = sizeof(int64_t) * offs.size();
+  const size_t bo = sizeof(int64_t) * offs.size(), bd = sizeof(int) * dims.size();
   char* d = nullptr;
-  HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), bd + bo, s));
-  HIP_TRY(hipMemcpyAsync(d, dims.data(), bd, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(d + bd, offs.data(), bo, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_label, dim3(n_models), dim3(kLabelThreads), 0, s, reinterpret_cast<const int*>(d),
-                     reinterpret_cast<const int64_t*>(d + bd), vel, threshold, labels, counts);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), bo + bd, s));
+  TRY(launch_label(d, offs, dims, vel, threshold, labels, counts, s));
   HIP_TRY(hipFreeAsync(d, s));
   return SEIS_OK;
+}
+
+int seis_label_flat_static(const int32_t* nz, const int32_t* nx, int32_t n_models, const double* vel,
+                           double threshold, int32_t* labels, int32_t* counts, seis_stream_t stream) {
+  if (n_models > SEIS_LABEL_STATIC_MAX_GRIDS)
+    return fail(SEIS_EINVAL, "seis_label_flat_static takes at most " + std::to_string(SEIS_LABEL_STATIC_MAX_GRIDS) +
+                                 " grids per call, got " + std::to_string(n_models));
+  std::vector<int64_t> offs;
+  std::vector<int> dims;
+  TRY(label_args(nz, nx, n_models, vel, threshold, labels, counts, offs, dims));
+  char* d = nullptr;
+  TRY(static_scratch(&d));
+  return launch_label(d, offs, dims, vel, threshold, labels, counts, reinterpret_cast<hipStream_t>(stream));
 }
 
 int64_t seis_plan_device_bytes(const seis_plan_t* plan) { return plan ? plan->L.bytes : 0; }
