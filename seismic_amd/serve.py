@@ -91,6 +91,15 @@ class InversionServer:
     def footprint(self, req: Request) -> int:
         return self._fp[req.geom]
 
+    def _check_admissible(self, req: Request):
+        """Refuse a request no state of the server could ever admit: no slot
+        of its size, or a footprint above the whole budget."""
+        if req.geom not in self.free:
+            raise ValueError(f"no slot of this server fits a {req.geom.nz}x{req.geom.nx} request")
+        if self.footprint(req) > self.budget:
+            raise ValueError(f"a {req.geom.nz}x{req.geom.nx} request's footprint ({self.footprint(req)} bytes) "
+                             f"exceeds the server's HBM budget ({self.budget} bytes)")
+
     def _try_admit(self, req: Request):
         free = self.free.get(req.geom)
         if not free:
@@ -129,6 +138,7 @@ class InversionServer:
         while queue or waiting or active:
             t = now()
             while queue and queue[0][1] <= t:
+                self._check_admissible(queue[0][0])
                 waiting.append(queue.popleft())
             # first fit in arrival order: a request that cannot be admitted
             # (no free slot of its size, or footprint over budget) does not
@@ -152,6 +162,8 @@ class InversionServer:
                         on_event(out[idx])
             waiting = blocked
             if not active:
+                if waiting:        # an idle server admits every admissible request
+                    raise RuntimeError("waiting requests could not be admitted on an idle server")
                 if queue:
                     time.sleep(max(0.0, queue[0][1] - now()))
                 continue
@@ -185,8 +197,7 @@ class InversionServer:
         now = lambda: time.perf_counter() - t0
         while nxt is not None or active:
             while nxt is not None:
-                if nxt.geom not in self.free:
-                    raise ValueError(f"no slot of this server fits a {nxt.geom.nz}x{nxt.geom.nx} request")
+                self._check_admissible(nxt)
                 slot = self._try_admit(nxt)
                 if slot is None:
                     break
@@ -203,8 +214,8 @@ class InversionServer:
                         on_event(rec)
                 nxt = next(it, None)
             if not active:
-                if nxt is not None:
-                    raise ValueError("a request's footprint exceeds the HBM budget of an idle server")
+                if nxt is not None:   # admissible, yet refused by an idle server
+                    raise RuntimeError("a request could not be admitted on an idle server")
                 break
             results = self.ex.execute({s: v[3] for s, v in active.items()})
             self.log.append((now(), len(results)))

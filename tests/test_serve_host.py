@@ -175,6 +175,20 @@ def test_run_stream_refills_slots_and_matches_alone():
         _server([g], budget=10, fake=FakeExecutor()).run_stream([_req(g, 0)])
 
 
+def test_run_refuses_requests_it_can_never_admit():
+    """run() raises instead of waiting forever for a request with no slot of
+    its size, or with a footprint above the whole budget; requests before
+    it are still served (they arrive earlier)."""
+    g, other = Geometry(nz=4, nx=4, nbc=2, nt=150), Geometry(nz=5, nx=5, nbc=2, nt=150)
+    with pytest.raises(ValueError, match="no slot"):
+        _server([g], budget=10 ** 9, fake=FakeExecutor()).run([_req(g, 0), _req(other, 1)], [0.0, 0.01])
+    with pytest.raises(ValueError, match="budget"):
+        _server([g], budget=10, fake=FakeExecutor()).run([_req(g, 0)], [0.0])
+    srv = _server([g, g], budget=250, fake=FakeExecutor())     # two slots, room for two footprints
+    recs = srv.run([_req(g, i) for i in range(3)], [0.0, 0.0, 0.0])
+    assert all(not r.failed for r in recs) and srv.used == 0
+
+
 def test_poisson_arrivals_and_percentiles():
     a = serve.poisson_arrivals(2000, 100.0, seed=1)
     assert np.all(np.diff(a) > 0)
