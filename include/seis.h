@@ -205,6 +205,38 @@ int seis_plan_get_config(const seis_plan_t* plan, int32_t kind, int32_t* rx,
 /* Device bytes held by the plan. */
 int64_t seis_plan_device_bytes(const seis_plan_t* plan);
 
+/* One LBFGS direction request: the history update and two-loop recursion
+ * of bfgs (core/seis_numerics.py:350-392), whose loops the reference runs
+ * as the lbfgs_backward / lbfgs_forward kernels (:450-639).  All pointers
+ * are device pointers, float64 vectors of n entries.  The (s, y) pairs live
+ * in ring buffers S and Y of cap + 1 rows (cap = the history size; the
+ * reference's unbounded history is cap = max_iter), with rho and al of
+ * cap + 1 entries and ring = {row of the oldest pair, number of pairs}, all
+ * owned by the caller and carried from call to call.  first != 0 starts a
+ * new history (d = -g, H_diag = 1, :350-352); otherwise y = g - g_prev and
+ * s = t * d_prev join the history when y.s > 1e-10 (dropping the oldest pair
+ * when cap pairs are held), H_diag = y.s / y.y, and d = the two-loop
+ * product.  scal receives {g.d, max|d|, sum|g|, max|g|}. */
+typedef struct {
+  double* S;             /* (cap + 1) x n steps s_i                          */
+  double* Y;             /* (cap + 1) x n gradient changes y_i               */
+  double* rho;           /* cap + 1: 1 / (y_i . s_i)                         */
+  double* al;            /* cap + 1: two-loop coefficients (workspace)       */
+  int32_t* ring;         /* 2: row of the oldest pair, number of pairs       */
+  double* hdiag;         /* 1: H_diag                                        */
+  const double* g;       /* current gradient                                 */
+  const double* g_prev;  /* gradient at the previous iterate                 */
+  const double* d_prev;  /* previous direction                               */
+  double t;              /* previous step length                             */
+  double* d;             /* out: new direction                               */
+  double* scal;          /* out: 4 doubles                                   */
+  int32_t n, cap, first, pad;
+} seis_lbfgs_req;
+
+/* Directions of n_req independent LBFGS requests (any sizes n <= 32768) in
+ * one launch per 16 requests, one workgroup each.  reqs is a host array. */
+int seis_lbfgs_direction(const seis_lbfgs_req* reqs /* host */, int32_t n_req, seis_stream_t stream);
+
 /* Flat areas of n_models velocity grids (no plan needed).  Grid m has
  * nz[m] x nx[m] float64 velocities (host arrays of sizes; row-major, depth =
  * row), concatenated on the device in vel.  Neighbouring cells (4-

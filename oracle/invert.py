@@ -8,6 +8,8 @@ TEST INFRASTRUCTURE ONLY (parity oracle); the product never imports it.
                       conjugate gradients on J^T J products (new; pinned by the
                       dense solve at small size)
   bfgs                core/seis_numerics.py:307-448 (torch 2.7 LBFGS port)
+  lbfgs_history_update, lbfgs_two_loop
+                      its history update and two-loop kernels (:355-392, :450-565)
   strong_wolfe        core/seis_numerics.py:138-277
   cubic_interpolate   core/seis_numerics.py:279-304
   lbfgs_invert        core/seis_invert.py:109-152 (InversionModel.seis_to_vel_lbfgs)
@@ -232,6 +234,44 @@ def strong_wolfe(obj_func, x, t, d, f, g, gtd, c1=1e-4, c2=0.9, tolerance_change
     return f_new, g_new, t, ls_func_evals
 
 
+def lbfgs_history_update(old_dirs, old_stps, ro, H_diag, flat_grad, prev_flat_grad, d, t, history_size=None):
+    """core/seis_numerics.py:355-368: y = g - g_prev, s = t d joins the
+    history (lists, oldest first) when y.s > 1e-10, and H_diag becomes
+    y.s / y.y; otherwise both stay.  history_size drops the oldest pair when
+    that many are held (the reference's history is unbounded).  Returns
+    H_diag."""
+    y = flat_grad - prev_flat_grad
+    s = d * t
+    ys = y @ s
+    if ys > 1e-10:
+        if history_size is not None and len(old_dirs) == history_size:
+            old_dirs.pop(0)
+            old_stps.pop(0)
+            ro.pop(0)
+        old_dirs.append(y)
+        old_stps.append(s)
+        ro.append(1.0 / ys)
+        H_diag = ys / (y @ y)
+    return H_diag
+
+
+def lbfgs_two_loop(old_dirs, old_stps, ro, H_diag, flat_grad):
+    """The two-loop recursion of the lbfgs_backward / lbfgs_forward kernels
+    (core/seis_numerics.py:450-565, called at :376-392): newest pair first,
+    then H_diag scaling, then oldest first.  Returns the direction."""
+    num_old = len(old_dirs)
+    q = -flat_grad
+    al = [0.0] * num_old
+    for i in range(num_old - 1, -1, -1):  # lbfgs_backward kernel (:451-501)
+        al[i] = (old_stps[i] @ q) * ro[i]
+        q = q - al[i] * old_dirs[i]
+    r = q * H_diag
+    for i in range(num_old):  # lbfgs_forward kernel (:511-559)
+        be_i = (old_dirs[i] @ r) * ro[i]
+        r = r + (al[i] - be_i) * old_stps[i]
+    return r
+
+
 def bfgs(cost_and_gradient_func, x0, max_iter, tolerance_grad, tolerance_change,
          history_size=None, torch_compat=False):
     """core/seis_numerics.py:307-448.  history_size=None keeps every pair
@@ -260,29 +300,9 @@ def bfgs(cost_and_gradient_func, x0, max_iter, tolerance_grad, tolerance_change,
             d = -flat_grad
             H_diag = 1
         else:
-            y = flat_grad - prev_flat_grad
-            s = d * t
-            ys = y @ s
-            if ys > 1e-10:
-                if history_size is not None and len(old_dirs) == history_size:
-                    old_dirs.pop(0)
-                    old_stps.pop(0)
-                    ro.pop(0)
-                old_dirs.append(y)
-                old_stps.append(s)
-                ro.append(1.0 / ys)
-                H_diag = ys / (y @ y)
-            num_old = len(old_dirs)
-            q = -flat_grad
-            al = [0.0] * num_old
-            for i in range(num_old - 1, -1, -1):  # lbfgs_backward kernel (:451-501)
-                al[i] = (old_stps[i] @ q) * ro[i]
-                q = q - al[i] * old_dirs[i]
-            r = q * H_diag
-            for i in range(num_old):  # lbfgs_forward kernel (:511-559)
-                be_i = (old_dirs[i] @ r) * ro[i]
-                r = r + (al[i] - be_i) * old_stps[i]
-            d = r
+            H_diag = lbfgs_history_update(old_dirs, old_stps, ro, H_diag, flat_grad, prev_flat_grad, d, t,
+                                          history_size)
+            d = lbfgs_two_loop(old_dirs, old_stps, ro, H_diag, flat_grad)
         prev_flat_grad = flat_grad.copy()
         prev_loss = loss
         if n_iter == 1:

@@ -28,6 +28,15 @@ class TimeDesc(ctypes.Structure):
     _fields_ = [("nt", c_int32), ("dt", c_float), ("nbc", c_int32), ("wavelet", POINTER(c_double))]
 
 
+class LbfgsReq(ctypes.Structure):
+    _fields_ = [("S", c_void_p), ("Y", c_void_p), ("rho", c_void_p), ("al", c_void_p), ("ring", c_void_p),
+                ("hdiag", c_void_p), ("g", c_void_p), ("g_prev", c_void_p), ("d_prev", c_void_p), ("t", c_double),
+                ("d", c_void_p), ("scal", c_void_p), ("n", c_int32), ("cap", c_int32), ("first", c_int32),
+                ("pad", c_int32)]
+
+
+LBFGS_MAX_N = 32768
+
 EXPORTS = {
     "seis_plan_create": (c_int32, [POINTER(ModelDesc), c_int32, POINTER(TimeDesc), c_int32, c_int32,
                                    POINTER(c_void_p)]),
@@ -51,6 +60,7 @@ EXPORTS = {
                                        POINTER(c_int32)]),
     "seis_label_flat": (c_int32, [POINTER(c_int32), POINTER(c_int32), c_int32, c_void_p, c_double, c_void_p,
                                   c_void_p, c_void_p]),
+    "seis_lbfgs_direction": (c_int32, [POINTER(LbfgsReq), c_int32, c_void_p]),
     "seis_label_flat_static": (c_int32, [POINTER(c_int32), POINTER(c_int32), c_int32, c_void_p, c_double, c_void_p,
                                          c_void_p, c_void_p]),
     "seis_last_error": (c_char_p, []),

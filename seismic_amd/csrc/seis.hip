@@ -1447,6 +1447,171 @@ __global__ __launch_bounds__(NTHR) void k_misfit_part(const ModelDev* models, co
   if (threadIdx.x == 0) part[(int64_t)blockIdx.y * kRedBlocks + blockIdx.x] = acc;
 }
 
+// ---------------------------------------------------------------- LBFGS direction
+constexpr int kLbfgsThreads = 1024;
+constexpr int kLbfgsChunk = 16;       // requests per launch (kernel-argument descriptors)
+constexpr int kLbfgsMaxPer = 32;      // entries per thread: n <= 32768
+
+struct LbfgsBatch {
+  seis_lbfgs_req r[kLbfgsChunk];
+};
+
+// One LBFGS direction per workgroup (bfgs, core/seis_numerics.py:350-392):
+// the history update (y = g - g_prev, s = t d_prev, kept when y.s > 1e-10,
+// H_diag = y.s / y.y) and the two-loop recursion of the lbfgs_backward /
+// lbfgs_forward kernels (:450-565), newest pair first, then oldest first.
+// The pairs live in a ring of cap + 1 rows: the new pair is written into the
+// free row and joins the ring when accepted; a full ring (history_size
+// reached) drops its oldest pair.  Each thread keeps its E entries of the
+// vector q / r in registers and only reads back history entries it wrote or
+// that earlier launches wrote, so the one cross-thread exchange is the
+// fixed-order block sum: results do not depend on what else the launch
+// carries.  Also returns g.d, max|d|, sum|g| and max|g| for the line search
+// and stopping tests.
+template <int E>
+__global__ __launch_bounds__(kLbfgsThreads) void k_lbfgs(LbfgsBatch b) {
+  constexpr int NWV = kLbfgsThreads / 64;
+  __shared__ double red[2][NWV];
+  const seis_lbfgs_req& q = b.r[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int phase = 0;
+  // Block reduction with alternating partial buffers: one barrier each (a
+  // buffer is rewritten two reductions later, after every thread has passed
+  // the barrier of the reduction in between, i.e. finished reading it).
+  // max that propagates NaN, as torch's max does
+  auto nmax = [](double a, double b) { return (a > b || a != a) ? a : b; };
+  auto reduce = [&](double v, bool is_max) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off, 64);
+      v = is_max ? nmax(v, o) : v + o;
+    }
+    double* r = red[phase];
+    phase ^= 1;
+    if (lane == 0) r[wv] = v;
+    __syncthreads();
+    double acc = r[0];
+    for (int w = 1; w < NWV; ++w) acc = is_max ? nmax(acc, r[w]) : acc + r[w];
+    return acc;
+  };
+  const int n = q.n, R = q.cap + 1;
+  double x[E];
+  int head = 0, cnt = 0;
+  double hd = 1.0;
+  if (q.first) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * kLbfgsThreads;
+      x[e] = i < n ? -q.g[i] : 0.0;
+    }
+  } else {
+    head = q.ring[0];
+    cnt = q.ring[1];
+    hd = q.hdiag[0];
+    const int fr = (head + cnt) % R;   // the free row
+    double* Sf = q.S + (int64_t)fr * n;
+    double* Yf = q.Y + (int64_t)fr * n;
+    double ys = 0.0, yy = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * kLbfgsThreads;
+      if (i < n) {
+        const double y = q.g[i] - q.g_prev[i];
+        const double sv = q.d_prev[i] * q.t;
+        Yf[i] = y;
+        Sf[i] = sv;
+        ys += y * sv;
+        yy += y * y;
+      }
+    }
+    ys = reduce(ys, false);
+    yy = reduce(yy, false);
+    double rho_new = 0.0;
+    const bool accept = ys > 1e-10;
+    if (accept) {
+      rho_new = 1.0 / ys;
+      if (tid == 0) q.rho[fr] = rho_new;
+      if (cnt == q.cap) head = (head + 1) % R;
+      else ++cnt;
+      hd = ys / yy;
+    }
+    // rho of the ring's entry i: the new pair's from the registers (its
+    // stored copy is for later launches)
+    auto rho_of = [&](int i, int slot) { return (accept && i == cnt - 1) ? rho_new : q.rho[slot]; };
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * kLbfgsThreads;
+      x[e] = i < n ? -q.g[i] : 0.0;
+    }
+    for (int i = cnt - 1; i >= 0; --i) {          // lbfgs_backward (:451-501)
+      const int slot = (head + i) % R;
+      const double* Si = q.S + (int64_t)slot * n;
+      const double* Yi = q.Y + (int64_t)slot * n;
+      double p = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int j = tid + e * kLbfgsThreads;
+        if (j < n) p += Si[j] * x[e];
+      }
+      const double a = reduce(p, false) * rho_of(i, slot);
+      if (tid == 0) q.al[slot] = a;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int j = tid + e * kLbfgsThreads;
+        if (j < n) x[e] -= a * Yi[j];
+      }
+    }
+    __threadfence_block();
+    __syncthreads();                               // al[] written by thread 0 is read by all below
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] *= hd;
+    for (int i = 0; i < cnt; ++i) {                // lbfgs_forward (:511-559)
+      const int slot = (head + i) % R;
+      const double* Si = q.S + (int64_t)slot * n;
+      const double* Yi = q.Y + (int64_t)slot * n;
+      double p = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int j = tid + e * kLbfgsThreads;
+        if (j < n) p += Yi[j] * x[e];
+      }
+      const double be = reduce(p, false) * rho_of(i, slot);
+      const double c = q.al[slot] - be;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int j = tid + e * kLbfgsThreads;
+        if (j < n) x[e] += c * Si[j];
+      }
+    }
+  }
+  double gd = 0.0, dmax = 0.0, gsum = 0.0, gmax = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * kLbfgsThreads;
+    if (i < n) {
+      const double gv = q.g[i];
+      q.d[i] = x[e];
+      gd += gv * x[e];
+      dmax = nmax(dmax, fabs(x[e]));
+      gsum += fabs(gv);
+      gmax = nmax(gmax, fabs(gv));
+    }
+  }
+  gd = reduce(gd, false);
+  dmax = reduce(dmax, true);
+  gsum = reduce(gsum, false);
+  gmax = reduce(gmax, true);
+  if (tid == 0) {
+    q.scal[0] = gd;
+    q.scal[1] = dmax;
+    q.scal[2] = gsum;
+    q.scal[3] = gmax;
+    q.ring[0] = head;
+    q.ring[1] = cnt;
+    q.hdiag[0] = hd;
+  }
+}
+
 // ---------------------------------------------------------------- flat-area labelling
 constexpr int kLabelMax = 24576;    // most cells of one labelled grid (LDS: 5 bytes per cell)
 constexpr int kLabelThreads = 1024;
@@ -2354,6 +2519,41 @@ int seis_plan_destroy(seis_plan_t* plan) {
   if (!plan) return SEIS_OK;
   (void)hipDeviceSynchronize();
   plan_free(plan);
+  return SEIS_OK;
+}
+
+int seis_lbfgs_direction(const seis_lbfgs_req* reqs, int32_t n_req, seis_stream_t stream) {
+  if (n_req < 0 || (n_req > 0 && !reqs)) return fail(SEIS_EINVAL, "bad request list");
+  for (int i = 0; i < n_req; ++i) {
+    const seis_lbfgs_req& r = reqs[i];
+    const std::string who = "request " + std::to_string(i) + ": ";
+    if (r.n < 1 || r.n > kLbfgsThreads * kLbfgsMaxPer)
+      return fail(SEIS_EINVAL, who + "n must be in 1.." + std::to_string(kLbfgsThreads * kLbfgsMaxPer));
+    if (r.cap < 1) return fail(SEIS_EINVAL, who + "cap must be >= 1");
+    if (!r.g || !r.d || !r.scal || !r.ring || !r.hdiag)
+      return fail(SEIS_EINVAL, who + "NULL argument");
+    if (!r.first && (!r.S || !r.Y || !r.rho || !r.al || !r.g_prev || !r.d_prev))
+      return fail(SEIS_EINVAL, who + "NULL history argument");
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (int c0 = 0; c0 < n_req; c0 += kLbfgsChunk) {
+    const int nc = std::min(kLbfgsChunk, n_req - c0);
+    LbfgsBatch b{};
+    int nmax = 1;
+    for (int i = 0; i < nc; ++i) {
+      b.r[i] = reqs[c0 + i];
+      nmax = std::max(nmax, (int)reqs[c0 + i].n);
+    }
+    const int per = (nmax + kLbfgsThreads - 1) / kLbfgsThreads;
+    const dim3 grid(nc), block(kLbfgsThreads);
+    if (per <= 1) hipLaunchKernelGGL(k_lbfgs<1>, grid, block, 0, s, b);
+    else if (per <= 2) hipLaunchKernelGGL(k_lbfgs<2>, grid, block, 0, s, b);
+    else if (per <= 4) hipLaunchKernelGGL(k_lbfgs<4>, grid, block, 0, s, b);
+    else if (per <= 8) hipLaunchKernelGGL(k_lbfgs<8>, grid, block, 0, s, b);
+    else if (per <= 16) hipLaunchKernelGGL(k_lbfgs<16>, grid, block, 0, s, b);
+    else hipLaunchKernelGGL(k_lbfgs<32>, grid, block, 0, s, b);
+    HIP_TRY(hipGetLastError());
+  }
   return SEIS_OK;
 }
 

@@ -38,12 +38,15 @@ class Op:
                 g = J(vec)^T (F(vec) - target)                     (float64)
     kind 'gnp': (vec, dvec) -> J(vec)^T J(vec) dvec                 (float64)
     kind 'fwd': vecs (list) -> [sum((F(v) - target)^2) for v in vecs]
+    kind 'dir': step (LbfgsStep) -> (d, (g.d, max|d|, sum|g|, max|g|)): the
+                LBFGS history update and two-loop direction
     """
 
     kind: str
     vec: Any = None
     dvec: Any = None
     vecs: Any = None
+    step: Any = None
 
 
 @dataclass
@@ -63,6 +66,86 @@ class Request:
 
 
 # --------------------------------------------------------------------- LBFGS
+class LbfgsHistory:
+    """Device-resident (s, y) history of one LBFGS run: ring buffers of
+    cap + 1 rows (libseis ``seis_lbfgs_direction``).  ``cap`` is the history
+    size; the reference keeps every pair (cap = max_iter,
+    core/seis_numerics.py:336-340).  Buffers are allocated on first use."""
+
+    def __init__(self, n: int, cap: int, device):
+        self.n, self.cap, self.device = int(n), max(1, int(cap)), torch.device(device)
+        self.S = self.Y = self.rho = self.al = self.ring = self.hdiag = None
+
+    def allocate(self):
+        if self.S is None:
+            R = self.cap + 1
+            f64 = dict(dtype=torch.float64, device=self.device)
+            self.S = torch.zeros(R, self.n, **f64)
+            self.Y = torch.zeros(R, self.n, **f64)
+            self.rho = torch.zeros(R, **f64)
+            self.al = torch.zeros(R, **f64)
+            self.ring = torch.zeros(2, dtype=torch.int32, device=self.device)
+            self.hdiag = torch.ones(1, **f64)
+
+    def nbytes(self) -> int:
+        return 8 * ((self.cap + 1) * (2 * self.n + 2) + 1) + 8
+
+
+@dataclass
+class LbfgsStep:
+    """One direction request: gradient g, the previous iterate's gradient,
+    direction and step (ignored when ``first``)."""
+
+    hist: LbfgsHistory
+    g: torch.Tensor
+    g_prev: torch.Tensor | None
+    d_prev: torch.Tensor | None
+    t: float
+    first: bool
+
+
+def lbfgs_directions(steps: Sequence[LbfgsStep]) -> list:
+    """LBFGS directions of many requests in one launch per 16
+    (``seis_lbfgs_direction``): returns [(d, (g.d, max|d|, sum|g|, max|g|))]
+    with one device-to-host read for all the scalars."""
+    import ctypes
+
+    from . import _lib
+    if not steps:
+        return []
+    dev = steps[0].g.device
+    if dev.type != "cuda":
+        raise RuntimeError("lbfgs_directions needs a ROCm GPU (torch.cuda)")
+    scal = torch.empty(len(steps), 4, dtype=torch.float64, device=dev)
+    reqs = (_lib.LbfgsReq * len(steps))()
+    keep = []
+    outs = []
+    ptr = lambda t: None if t is None else t.data_ptr()
+    for i, st in enumerate(steps):
+        h = st.hist
+        if h.n > _lib.LBFGS_MAX_N:
+            raise ValueError(f"LBFGS direction supports at most {_lib.LBFGS_MAX_N} parameters, got {h.n}")
+        h.allocate()
+        g = st.g.to(torch.float64).contiguous()
+        gp = None if st.first else st.g_prev.to(torch.float64).contiguous()
+        dp = None if st.first else st.d_prev.to(torch.float64).contiguous()
+        d = torch.empty(h.n, dtype=torch.float64, device=dev)
+        keep += [g, gp, dp]
+        outs.append(d)
+        r = reqs[i]
+        r.S, r.Y, r.rho, r.al = ptr(h.S), ptr(h.Y), ptr(h.rho), ptr(h.al)
+        r.ring, r.hdiag = ptr(h.ring), ptr(h.hdiag)
+        r.g, r.g_prev, r.d_prev = ptr(g), ptr(gp), ptr(dp)
+        r.t = 0.0 if st.first else float(st.t)
+        r.d, r.scal = ptr(d), scal[i].data_ptr()
+        r.n, r.cap, r.first = h.n, h.cap, int(bool(st.first))
+    with torch.cuda.device(dev):
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().seis_lbfgs_direction(reqs, len(steps), stream), "seis_lbfgs_direction")
+    sc = scal.cpu().tolist()
+    return [(d, tuple(v)) for d, v in zip(outs, sc)]
+
+
 def _cubic_interpolate(x1, f1, g1, x2, f2, g2, bounds=None):
     """core/seis_numerics.py:279-304 (scalars as Python floats)."""
     if bounds is not None:
@@ -81,10 +164,12 @@ def _cubic_interpolate(x1, f1, g1, x2, f2, g2, bounds=None):
     return (xmin_bound + xmax_bound) / 2.0
 
 
-def _strong_wolfe(obj, x, t, d, f, g, gtd, c1=1e-4, c2=0.9, tolerance_change=1e-9, max_ls=25):
+def _strong_wolfe(obj, x, t, d, f, g, gtd, c1=1e-4, c2=0.9, tolerance_change=1e-9, max_ls=25, d_norm=None):
     """core/seis_numerics.py:138-277 as a sub-machine; ``obj(x, t, d)`` is a
-    machine returning (f: float, g: tensor)."""
-    d_norm = float(d.abs().max())
+    machine returning (f: float, g: tensor).  ``d_norm`` = max|d| when the
+    caller has it."""
+    if d_norm is None:
+        d_norm = float(d.abs().max())
     g = g.clone()
     f_new, g_new = yield from obj(x, t, d)
     ls_func_evals = 1
@@ -165,8 +250,10 @@ def bfgs_machine(func, x0: torch.Tensor, max_iter: int, tolerance_grad: float,
                  tolerance_change: float, history_size: int | None = None):
     """core/seis_numerics.py:307-448 as a machine.  ``func(x)`` is a machine
     returning (loss: float, grad: tensor).  history_size=None keeps every
-    (s, y) pair like the reference; an int keeps the most recent ones.
-    Returns (x, n_evals, n_iter)."""
+    (s, y) pair like the reference; an int keeps the most recent ones.  The
+    history update and two-loop recursion (the reference's lbfgs_backward /
+    lbfgs_forward kernels) run on the device as a ``dir`` op, batched over
+    requests by the executor.  Returns (x, n_evals, n_iter)."""
     lr = 1
     max_eval = max_iter * 5 // 4
     loss, flat_grad = yield from func(x0)
@@ -174,55 +261,27 @@ def bfgs_machine(func, x0: torch.Tensor, max_iter: int, tolerance_grad: float,
     current_evals = 1
     if float(flat_grad.abs().max()) <= tolerance_grad:
         return x, current_evals, 0
+    hist = LbfgsHistory(x0.numel(), history_size if history_size is not None else max_iter, x0.device)
     n_iter = 0
-    old_dirs: list = []
-    old_stps: list = []
-    ro: list = []
     prev_flat_grad = None
-    H_diag = 1.0
     d = t = None
     while n_iter < max_iter:
         n_iter += 1
-        if n_iter == 1:
-            d = flat_grad.neg()
-            H_diag = 1.0
-        else:
-            y = flat_grad.sub(prev_flat_grad)
-            s = d.mul(t)
-            ys = float(y.dot(s))
-            if ys > 1e-10:
-                if history_size is not None and len(old_dirs) == history_size:
-                    old_dirs.pop(0)
-                    old_stps.pop(0)
-                    ro.pop(0)
-                old_dirs.append(y)
-                old_stps.append(s)
-                ro.append(1.0 / ys)
-                H_diag = ys / float(y.dot(y))
-            num_old = len(old_dirs)
-            q = flat_grad.neg()
-            al = [0.0] * num_old
-            for i in range(num_old - 1, -1, -1):        # lbfgs_backward (:451-501)
-                al[i] = float(old_stps[i].dot(q)) * ro[i]
-                q.add_(old_dirs[i], alpha=-al[i])
-            r = torch.mul(q, H_diag)
-            for i in range(num_old):                     # lbfgs_forward (:511-559)
-                be_i = float(old_dirs[i].dot(r)) * ro[i]
-                r.add_(old_stps[i], alpha=al[i] - be_i)
-            d = r
-        prev_flat_grad = flat_grad.clone()
+        d, (gtd, d_max, g_sum, _) = yield Op("dir", step=LbfgsStep(hist, flat_grad, prev_flat_grad, d, t,
+                                                                   n_iter == 1))
+        prev_flat_grad = flat_grad
         prev_loss = loss
         if n_iter == 1:
-            t = min(1.0, 1.0 / float(flat_grad.abs().sum())) * lr
+            t = min(1.0, 1.0 / g_sum) * lr
         else:
             t = lr
-        gtd = float(flat_grad.dot(d))
         x_init = x.clone()
 
         def obj(xx, tt, dd):
             return (yield from func(xx + tt * dd))
 
-        loss, flat_grad, t, ls_func_evals = yield from _strong_wolfe(obj, x_init, t, d, loss, flat_grad, gtd)
+        loss, flat_grad, t, ls_func_evals = yield from _strong_wolfe(obj, x_init, t, d, loss, flat_grad, gtd,
+                                                                     d_norm=d_max)
         x.add_(d, alpha=t)
         opt_cond = float(flat_grad.abs().max()) <= tolerance_grad
         current_evals += ls_func_evals
@@ -232,7 +291,7 @@ def bfgs_machine(func, x0: torch.Tensor, max_iter: int, tolerance_grad: float,
             break
         if opt_cond:
             break
-        if float(d.mul(t).abs().max()) <= tolerance_change:
+        if d_max * t <= tolerance_change:      # max|d t| (t > 0)
             break
         if abs(loss - prev_loss) < tolerance_change:
             break
@@ -428,6 +487,8 @@ class Executor:
                     out.update(self._gnp(ch, pending))
                 elif kind == "fwd":
                     out.update(self._fwd(ch, pending))
+                elif kind == "dir":
+                    out.update(zip(ch, lbfgs_directions([pending[i].step for i in ch])))
                 else:
                     raise ValueError(kind)
         return out
@@ -559,10 +620,13 @@ def bfgs(cost_and_gradient_func, x0, max_iter, tolerance_grad, tolerance_change,
 
     m = bfgs_machine(func, torch.as_tensor(x0), max_iter, tolerance_grad, tolerance_change, history_size)
     try:
-        next(m)
+        op = next(m)
+        while True:
+            if op.kind != "dir":
+                raise RuntimeError("bfgs machine yielded a propagator op")
+            op = m.send(lbfgs_directions([op.step])[0])
     except StopIteration as e:
         return e.value[0]
-    raise RuntimeError("bfgs machine yielded an op without a propagator")
 
 
 def invert(requests: Sequence[Request], method: str = "lbfgs", batched: bool = True,

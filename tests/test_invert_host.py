@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import torch
 
+import host_lbfgs
 from oracle import fd2d
 from oracle import invert as oinv
 from oracle import priors as opri
@@ -28,6 +29,8 @@ class OracleExecutor:
                 seis, _, gr = fd2d.vel_to_seis(op.vec.numpy(), self.og, vec_adjoint=self.t,
                                                adjoint_on_residual=True)
                 out[s] = (float(np.sum((seis - self.t) ** 2)), torch.tensor(gr))
+            elif op.kind == "dir":
+                out[s] = host_lbfgs.direction(op.step)
             elif op.kind == "gnp":
                 _, jdv, _ = fd2d.vel_to_seis(op.vec.numpy(), self.og, vec_diff=op.dvec.numpy())
                 out[s] = torch.tensor(fd2d.vel_to_seis(op.vec.numpy(), self.og, vec_adjoint=jdv)[2])

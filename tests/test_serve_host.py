@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 import torch
 
+import host_lbfgs
 from seismic_amd import invert, serve
 from seismic_amd.geometry import Geometry
 
@@ -57,6 +58,9 @@ class FakeExecutor:
         time.sleep(self.seconds)
         out = {}
         for s, op in pending.items():
+            if op.kind == "dir":
+                out[s] = host_lbfgs.direction(op.step)
+                continue
             assert op.kind == "fev"
             c = self.centers[s]
             r = op.vec - c
