@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY (parity oracle); the product never imports it.
 
   cost_and_gradient   core/seis_invert.py:19-49
   gauss_newton_dense  core/seis_invert.py:63-107  (InversionModel.seis_to_vel_gn)
+  gn_rhs              its right-hand side (:77-78)
   gauss_newton_cg     the same pass with the dense J^T J solve replaced by
                       conjugate gradients on J^T J products (new; pinned by the
                       dense solve at small size)
@@ -47,6 +48,16 @@ def _line_search_scales(n=50):
     return np.linspace(-0.1, 1.1, n)
 
 
+def gn_rhs(x_guess, target, prior, v2s):
+    """Right-hand side of the GN system, core/seis_invert.py:77-78:
+    -B^T J^T (s(B x) - t) / N - lam P x (velocity part; 0 for min_vel)."""
+    B = prior.basis
+    N = len(target)
+    grad_res = v2s(B @ x_guess, vec_adjoint=target, adjoint_on_residual=True)[2]
+    rhs = -B.T @ grad_res / N
+    return rhs - np.concatenate([prior.lam * prior.P @ x_guess[:-1], [0.0]])
+
+
 def gauss_newton_dense(vec_guess, target, prior, g, v2s=None, n_scales=50):
     """One Gauss-Newton pass, core/seis_invert.py:63-107 (prior must have P)."""
     v2s = v2s or (lambda vec, **kw: fd2d.vel_to_seis(vec, g, **kw))
@@ -54,9 +65,7 @@ def gauss_newton_dense(vec_guess, target, prior, g, v2s=None, n_scales=50):
     vec_guess = np.asarray(vec_guess, dtype=np.float64).reshape(-1)
     x_guess = np.linalg.solve(B.T @ B, B.T @ vec_guess)
     N = len(target)
-    grad_res = v2s(B @ x_guess, vec_adjoint=target, adjoint_on_residual=True)[2]
-    rhs = -B.T @ grad_res / N
-    rhs = rhs - np.concatenate([prior.lam * prior.P @ x_guess[:-1], [0.0]])
+    rhs = gn_rhs(x_guess, target, prior, v2s)
     n = B.shape[1]
     P = np.zeros((n, n))
     P[:-1, :-1] = prior.lam * prior.P
@@ -88,9 +97,7 @@ def gauss_newton_cg(vec_guess, target, prior, g, n_cg=20, v2s=None, n_scales=50)
     vec_guess = np.asarray(vec_guess, dtype=np.float64).reshape(-1)
     x_guess = np.linalg.solve(B.T @ B, B.T @ vec_guess)
     N = len(target)
-    grad_res = v2s(B @ x_guess, vec_adjoint=target, adjoint_on_residual=True)[2]
-    rhs = -B.T @ grad_res / N
-    rhs = rhs - np.concatenate([prior.lam * prior.P @ x_guess[:-1], [0.0]])
+    rhs = gn_rhs(x_guess, target, prior, v2s)
 
     def A(v):
         jv = v2s(vec_guess, vec_diff=B @ v)[1]

@@ -10,7 +10,15 @@ to float32 (the precision observed data is handed over in).
   ref70_grad.npz  the float32 target (349650), J^T (F(v) - target) for all 5
                   shots (float64, 4901) and the misfit sum((F(v) - target)^2)
 
-Run from the repo root: python tests/golden/make_golden.py  (~3 minutes).
+The grid sizes of the batched and served workloads (100x100, 70x140 (z x x),
+140x140; nbc 120, nt 999, shots at round(linspace(0, nx-1, 5)), receivers on
+every surface column), one curved-fault model each (seed 20 + size index):
+
+  ref{nz}x{nx}.npz  seismogram of shot 0 (float32, (999, nx)) and J^T dd for
+                    the float32 data vector dd = adjoint_probe(...) (float64)
+
+Run from the repo root: python tests/golden/make_golden.py [ref70] [sizes]
+(~5 minutes).
 """
 import os
 import sys
@@ -24,9 +32,28 @@ from oracle import fd2d  # noqa: E402
 from seismic_amd import synth  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+SIZES = [(100, 100), (70, 140), (140, 140)]
 
 
-def main():
+def size_model(i, nz, nx):
+    """The velocity model of the i-th fixture size (float64 (nz, nx))."""
+    return synth.curve_fault_model(nz, nx, seed=20 + i)
+
+
+def adjoint_probe(n_shots, nt, n_rcv, seed=5):
+    """Data-space vector the size fixtures back-propagate: seeded standard
+    normal samples in float32 (the GPU tests regenerate it)."""
+    return np.random.default_rng(seed).standard_normal(n_shots * nt * n_rcv).astype(np.float32)
+
+
+def main(parts=("ref70", "sizes")):
+    if "ref70" in parts:
+        write_ref70()
+    if "sizes" in parts:
+        write_sizes()
+
+
+def write_ref70():
     g = fd2d.Geometry()
     v = synth.curve_fault_model(70, 70, seed=7)
     vec = synth.to_vec(v)
@@ -41,5 +68,16 @@ def main():
     print("written", np.abs(s3).max(), np.linalg.norm(grad))
 
 
+def write_sizes():
+    for i, (nz, nx) in enumerate(SIZES):
+        gs = fd2d.Geometry(nz=nz, nx=nx)
+        vs = synth.to_vec(size_model(i, nz, nx))
+        dd = adjoint_probe(gs.n_shots, gs.nt, gs.n_rcv).astype(np.float64)
+        seis, _, grad = fd2d.vel_to_seis(vs, gs, vec_adjoint=dd, adjoint_on_residual=False)
+        s0 = seis.reshape(gs.n_shots, gs.nt, gs.n_rcv)[0]
+        np.savez_compressed(os.path.join(HERE, f"ref{nz}x{nx}.npz"), seis0=s0.astype(np.float32), grad=grad)
+        print("written", nz, nx, np.abs(s0).max(), np.linalg.norm(grad))
+
+
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]) or ("ref70", "sizes"))

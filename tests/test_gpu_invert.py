@@ -86,6 +86,35 @@ def test_gn_cg_matches_oracle_inversion():
     assert mae < 1.0
 
 
+def test_dense_gn_matches_oracle_on_row_basis():
+    """The reference's Gauss-Newton pass with its dense solve
+    (core/seis_invert.py:63-107: J column by column, solve, 50-point line
+    search), gn_machine(n_cg=None) here (one J^T J product per basis vector,
+    the small system solved on the host), against the oracle's
+    gauss_newton_dense on a row basis: RestrictFlatAreas over L2 smoothness,
+    adapted to the depth-linear start, whose rows differ by more than the
+    threshold (one area per row, plus min_vel).  Two passes."""
+    g, og, true, target, start = _problem(3)
+    lam = 1e-9
+    v_ref = start.copy()
+    v = torch.tensor(start, dtype=torch.float64, device="cuda")
+    for k in range(2):
+        op = opri.RestrictFlatAreas(opri.L2Smoothness(g.nz, g.nx, lam=lam))
+        op.adapt(v_ref)
+        pp = priors.RestrictFlatAreas(priors.L2Smoothness(g.nz, g.nx, lam=lam))
+        pp.adapt(v)
+        assert pp.N == op.N and (k > 0 or op.N == g.nz + 1)
+        v_ref, s_ref, _ = oinv.gauss_newton_dense(v_ref, target, op, og)
+        v = invert.invert([_req(g, target, v.cpu().numpy(), pp)], method="gn", batched=False, n_outer=1,
+                          n_cg=None)[0]
+    v = v.cpu().numpy()
+    mae = np.abs(v[:-1] - v_ref[:-1]).mean()
+    print("dense GN MAE vs oracle", mae, "moved", np.abs(v_ref - start)[:-1].mean(), "min_vel", v[-1], v_ref[-1])
+    assert np.abs(v_ref - start)[:-1].mean() > 50.0
+    assert mae < 1.0
+    assert abs(v[-1] - v_ref[-1]) < 1.0
+
+
 @pytest.mark.parametrize("method", ["lbfgs", "gn"])
 def test_batched_equals_plain(method):
     """Each request's result is the same alone (plain path) and in the batch."""

@@ -70,3 +70,36 @@ def test_server_keeps_full_history_when_it_fits():
     recs = srv.run(_clone(reqs), serve.poisson_arrivals(6, 300.0, seed=4))
     for rec, a in zip(recs, alone):
         assert torch.equal(rec.result, a), rec.rid
+
+
+def test_served_lbfgs_at_config5_sizes():
+    """The serving workload at its sizes: LBFGS requests (30 iterations,
+    memory 10, L2 smoothness prior) on 70x70, 100x100 and 140x140 grids
+    (nbc 120, nt 1000, 5 shots, a receiver on every surface column),
+    curved-fault models, 6 requests arriving as a Poisson process on 3 slots
+    (one per size, full forward histories): every served result equals the
+    same request run alone on the plain path, bit for bit."""
+    from seismic_amd.propagator import Propagator
+    geoms = [Geometry(nz=n, nx=n, nt=1000) for n in (70, 100, 140)]
+    reqs = []
+    for i in range(6):
+        g = geoms[i % 3]
+        P = Propagator([g], caps=1)
+        target = P.forward(torch.tensor(synth.to_vec(synth.curve_fault_model(g.nz, g.nx, 200 + i)))).clone()
+        P.close()
+        start = synth.to_vec(synth.depth_linear_start(g.nz, g.nx, 1500.0, 4000.0), min_vel=1500.0)
+        reqs.append(invert.Request(geom=g, target=target, prior=priors.L2Smoothness(g.nz, g.nx, lam=1e-8),
+                                   start=torch.tensor(start, dtype=torch.float64, device="cuda"), rid=i))
+    kw = dict(max_iter=30, history_size=10, tolerance_grad=1e-7, tolerance_change=0.0)
+    alone = invert.invert(_clone(reqs), method="lbfgs", batched=False, **kw)
+    srv = serve.InversionServer(geoms, hbm_budget=1 << 40, ckpt_every=None, device="cuda", **kw)
+    assert srv.ckpt_every == 0
+    recs = srv.run(_clone(reqs), serve.poisson_arrivals(6, 100.0, seed=3))
+    print("served at config-5 sizes: in flight", max(n for _, n in srv.log),
+          "latency p50/p99", serve.percentiles([r.latency for r in recs]), flush=True)
+    assert all(not r.failed for r in recs)
+    assert max(n for _, n in srv.log) == 3                     # all three slots shared launches
+    for rec, a in zip(recs, alone):
+        assert rec.stats["n_iter"] >= 10, rec.stats
+        assert torch.equal(rec.result, a), rec.rid
+    srv.ex.close()

@@ -213,3 +213,73 @@ def test_reference_model_classes_map_to_stages():
         wrong.stages()[0].prior(g.nz, g.nx, "cpu")
     with pytest.raises(ValueError):
         schedule.InversionModel().stages()
+
+
+class LinearExecutor:
+    """Machine ops on a linear model s = J vec (exact Gauss-Newton model)."""
+
+    def __init__(self, J, t):
+        self.J, self.t = J, t
+
+    def execute(self, pending):
+        out = {}
+        J, t = self.J, self.t
+        for s, op in pending.items():
+            if op.kind == "fev":
+                r = J @ op.vec.numpy() - t
+                out[s] = (float(r @ r), torch.tensor(J.T @ r))
+            elif op.kind == "gnp":
+                out[s] = torch.tensor(J.T @ (J @ op.dvec.numpy()))
+            elif op.kind == "dir":
+                out[s] = host_lbfgs.direction(op.step)
+            else:
+                out[s] = [float(np.sum((J @ v.numpy() - t) ** 2)) for v in op.vecs]
+        return out
+
+
+@pytest.mark.parametrize("n_cg", [None, 30])
+def test_gn_machine_reaches_closed_form_minimiser(n_cg):
+    """The product's GN pass (dense solve, or CG with more steps than
+    parameters) on a linear model lands on the minimiser of
+    lam x^T P x + |J x - t|^2 / N, computed by least squares on the stacked
+    system; pins the machine's rhs and system scaling (core/seis_invert.py:77-92)."""
+    g = Geometry(nz=4, nx=5, nbc=2, nt=150)
+    rng = np.random.default_rng(6)
+    n, N, lam = g.n_params, g.n_data, 0.37
+    J = rng.normal(size=(N, n))
+    t = rng.normal(size=N) * 10
+    D = np.hstack([opri.L2Smoothness(g.nz, g.nx).D, np.zeros((opri.L2Smoothness(g.nz, g.nx).D.shape[0], 1))])
+    M = np.vstack([np.sqrt(lam) * D, J / np.sqrt(N)])
+    ref = np.linalg.lstsq(M, np.concatenate([np.zeros(D.shape[0]), t / np.sqrt(N)]), rcond=None)[0]
+    r = invert.Request(geom=g, target=torch.tensor(t), prior=priors.L2Smoothness(g.nz, g.nx, lam=lam, device="cpu"),
+                       start=torch.tensor(rng.normal(size=n)))
+    out = invert.run_machines({0: invert.gn_machine(r, n_outer=1, n_cg=n_cg, n_scales=13)}, LinearExecutor(J, t))[0]
+    np.testing.assert_allclose(out.numpy(), ref, rtol=1e-7, atol=1e-7 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("scaling,tol", [(1e15, 0.0), (1.0, 0.0), (1e15, 0.02)])
+def test_lbfgs_closure_matches_oracle_on_linear_model(scaling, tol):
+    """The product's LBFGS closure (scaling, seis_error_tolerance zeroing,
+    core/seis_invert.py:125-138) against the oracle's (pinned by
+    test_lbfgs_closure_scaling_and_error_tolerance) on a linear model: same
+    iterate, same number of evaluations, also when the misfit tolerance
+    stops the run early."""
+    from test_oracle import LinearV2S
+    g = Geometry(nz=6, nx=5, nbc=2, nt=150)
+    og = fd2d.Geometry(nz=6, nx=5, nbc=2, nt=150)
+    rng = np.random.default_rng(9)
+    J = rng.normal(size=(g.n_data, g.n_params)) / 50
+    op = opri.RowTotalVariation(g.nz, g.nx, lam=1e-4)
+    true = op.basis @ np.concatenate([np.linspace(1800, 2600, g.nz), [1700.0]])
+    t = J @ true
+    start = op.basis @ np.concatenate([np.full(g.nz, 2200.0), [1700.0]])
+    ref, nfev = oinv.lbfgs_invert(start, t, op, og, max_iter=12, tolerance_change=1e-9, scaling=scaling,
+                                  seis_error_tolerance=tol, v2s=LinearV2S(J))
+    r = invert.Request(geom=g, target=torch.tensor(t), prior=priors.RowTotalVariation(g.nz, g.nx, lam=1e-4, device="cpu"),
+                       start=torch.tensor(start))
+    m = invert.lbfgs_machine(r, max_iter=12, tolerance_change=1e-9, scaling=scaling, seis_error_tolerance=tol)
+    out = invert.run_machines({0: m}, LinearExecutor(J, t))[0].numpy()
+    assert r.meta["stats"]["fev"] == nfev
+    if tol > 0:
+        assert nfev < 12 and r.meta["stats"]["misfit"] < tol    # stopped by the misfit tolerance
+    np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-7)

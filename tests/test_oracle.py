@@ -464,3 +464,149 @@ def test_dispatch_kpis_on_constructed_cases():
     assert invert.kpi_fault_a(g) == 6 * 7 + 6 * 3
     assert invert.kpi_fault_a(np.full((4, 4), 2000.0)) == 16    # one area: every cell
     assert invert.kpi_fault_a(np.arange(12.0).reshape(3, 4) * 11) == 2    # all singletons
+
+
+# ------------------------------------------------------------ GN, priors and the LBFGS closure by closed forms
+class LinearV2S:
+    """s(vec) = J vec with the vel_to_seis call convention (J dvec, J^T dd,
+    J^T (s - dd)): a propagator whose Gauss-Newton model is exact."""
+
+    def __init__(self, J):
+        self.J = J
+
+    def __call__(self, vec, vec_diff=None, vec_adjoint=None, adjoint_on_residual=False):
+        s = self.J @ vec
+        ds = None if vec_diff is None else self.J @ vec_diff
+        gr = None
+        if vec_adjoint is not None:
+            gr = self.J.T @ ((s - vec_adjoint) if adjoint_on_residual else vec_adjoint)
+        return s, ds, gr
+
+
+def _row_l2(nz, nx, lam):
+    """L2 smoothness restricted to the row basis (rows constant, min_vel)."""
+    p = priors.L2Smoothness(nz, nx, lam=lam)
+    B = priors.RowTotalVariation(nz, nx).basis
+
+    class RowL2:
+        basis = B
+        P = B[:-1, :-1].T @ p.P @ B[:-1, :-1]
+        D = p.D @ B[:-1, :-1]
+
+        def cost_grad(self, x):
+            xv = x[:-1]
+            return self.lam * xv @ self.P @ xv, np.concatenate([2 * self.lam * self.P @ xv, [0.0]])
+
+    RowL2.lam = lam
+    return RowL2()
+
+
+def _gn_closed_form(prior, J, t):
+    """argmin_x lam |D x_v|^2 + |J B x - t|^2 / N (x_v = velocity part),
+    from a least-squares solve of the stacked system: the minimiser one
+    Gauss-Newton step reaches on a linear model."""
+    N = len(t)
+    B = prior.basis
+    D = np.hstack([prior.D, np.zeros((prior.D.shape[0], 1))])
+    M = np.vstack([np.sqrt(prior.lam) * D, J @ B / np.sqrt(N)])
+    rhs = np.concatenate([np.zeros(D.shape[0]), t / np.sqrt(N)])
+    return B @ np.linalg.lstsq(M, rhs, rcond=None)[0]
+
+
+@pytest.mark.parametrize("row_basis", [False, True])
+def test_gauss_newton_reaches_closed_form_minimiser(row_basis):
+    """On a linear model s = J vec a GN pass (core/seis_invert.py:63-107)
+    lands on the minimiser of lam x^T P x + |J B x - t|^2 / N: with 13 line
+    search points the grid holds scale 1 (to 2e-16).  A slip in the rhs or A
+    scaling (a factor N, lam, 2 or a sign) moves the result."""
+    rng = np.random.default_rng(5)
+    nz, nx, lam = 4, 5, 0.37
+    prior = _row_l2(nz, nx, lam) if row_basis else priors.L2Smoothness(nz, nx, lam=lam)
+    n = nz * nx + 1
+    J = rng.normal(size=(90, n))
+    t = rng.normal(size=90) * 10
+    start = prior.basis @ rng.normal(size=prior.basis.shape[1])     # inside the basis' span
+    ref = _gn_closed_form(prior, J, t)
+    v2s = LinearV2S(J)
+    g = fd2d.Geometry(**SMALL)
+    vd, sd, _ = invert.gauss_newton_dense(start, t, prior, g, v2s=v2s, n_scales=13)
+    vc, sc, _ = invert.gauss_newton_cg(start, t, prior, g, n_cg=prior.basis.shape[1] + 5, v2s=v2s, n_scales=13)
+    assert abs(sd - 1.0) < 1e-15 and abs(sc - 1.0) < 1e-15
+    np.testing.assert_allclose(vd, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    np.testing.assert_allclose(vc, ref, rtol=1e-7, atol=1e-7 * np.abs(ref).max())
+
+
+def test_gn_rhs_is_minus_half_cost_gradient():
+    """The GN right-hand side (:77-78) is -1/2 the gradient of the cost
+    lam x^T P x + mean((s - t)^2) (:19-49), whose gradient is pinned by finite
+    differences here (propagator: the oracle on a small grid)."""
+    g = fd2d.Geometry(**SMALL)
+    p = priors.L2Smoothness(g.nz, g.nx, lam=3e-10)
+    target = fd2d.vel_to_seis(_layered(g, 11), g)[0]
+    x = _layered(g, 4) + np.concatenate([20 * np.random.default_rng(3).normal(size=g.nz * g.nx), [0.0]])
+    c, gr, _, _ = invert.cost_and_gradient(x, target, p, p.basis, g, compute_gradient=True)
+    off = 1e-3 * np.random.default_rng(7).normal(0, 1, p.N)
+    c2 = invert.cost_and_gradient(x + off, target, p, p.basis, g)[0]
+    c1 = invert.cost_and_gradient(x - off, target, p, p.basis, g)[0]
+    assert abs(gr @ off - (c2 - c1) / 2) / abs((c2 - c1) / 2) < 1e-6
+    rhs = invert.gn_rhs(x, target, p, lambda vec, **kw: fd2d.vel_to_seis(vec, g, **kw))
+    np.testing.assert_allclose(rhs, -0.5 * gr, rtol=1e-11, atol=1e-12 * np.abs(gr).max())
+
+
+def test_tv_priors_closed_forms():
+    """Costs of core/seis_prior.py:67-177 on fields whose value is known: a
+    constant field costs lam*eps (every smoothed |difference| is eps); a ramp
+    of step h along z costs lam*sqrt(h^2 + eps^2) for the row prior (mean over
+    the nz-1 row differences), and for the 2-D prior the mean over all
+    (nz-1)*nx vertical and nz*(nx-1) horizontal differences."""
+    nz, nx, lam, eps, h = 7, 9, 2.5, 0.1, 3.0
+    rt, tv = priors.RowTotalVariation(nz, nx, lam=lam, epsilon=eps), priors.TotalVariation(nz, nx, lam=lam, epsilon=eps)
+    const = np.full(nz + 1, 1234.0)
+    assert rt.cost_grad(const)[0] == pytest.approx(lam * eps, rel=1e-14)
+    assert tv.cost_grad(np.full(nz * nx + 1, 1234.0))[0] == pytest.approx(lam * eps, rel=1e-14)
+    ramp_rows = np.concatenate([1500.0 + h * np.arange(nz), [1500.0]])
+    assert rt.cost_grad(ramp_rows)[0] == pytest.approx(lam * np.sqrt(h * h + eps * eps), rel=1e-14)
+    ramp = np.concatenate([np.repeat(1500.0 + h * np.arange(nz), nx), [1500.0]])
+    n1, n2 = (nz - 1) * nx, nz * (nx - 1)
+    ref = lam * (n1 * np.sqrt(h * h + eps * eps) + n2 * eps) / (n1 + n2)
+    assert tv.cost_grad(ramp)[0] == pytest.approx(ref, rel=1e-14)
+    # the product's priors are the same functions
+    from seismic_amd import priors as ppri
+    prt = ppri.RowTotalVariation(nz, nx, lam=lam, epsilon=eps, device="cpu")
+    ptv = ppri.TotalVariation(nz, nx, lam=lam, epsilon=eps, device="cpu")
+    assert float(prt.cost_grad(torch.tensor(ramp_rows))[0]) == pytest.approx(lam * np.sqrt(h * h + eps * eps), rel=1e-14)
+    assert float(ptv.cost_grad(torch.tensor(ramp))[0]) == pytest.approx(ref, rel=1e-14)
+
+
+def test_lbfgs_closure_scaling_and_error_tolerance(monkeypatch):
+    """The closure of seis_to_vel_lbfgs (core/seis_invert.py:125-138): cost and
+    gradient of cost_and_gradient scaled by 1e15, and both zeroed when the
+    data misfit mean((s - t)^2) is below seis_error_tolerance."""
+    rng = np.random.default_rng(8)
+    nz, nx = 3, 4
+    p = priors.RowTotalVariation(nz, nx, lam=1e-3)
+    J = rng.normal(size=(50, nz * nx + 1))
+    v2s = LinearV2S(J)
+    x = np.concatenate([rng.normal(2000, 100, nz), [1500.0]])
+    t_exact = J @ (p.basis @ x)
+    seen = {}
+
+    def fake_bfgs(func, x0, *a, **k):
+        seen["func"] = func
+        return x0, 1
+
+    monkeypatch.setattr(invert, "bfgs", fake_bfgs)
+    g = fd2d.Geometry(**SMALL)
+    t_off = t_exact + 0.5
+    invert.lbfgs_invert(p.basis @ x, t_off, p, g, max_iter=1, v2s=v2s, seis_error_tolerance=0.2)
+    f, gr = seen["func"](x)
+    cost_prior = p.cost_grad(x)[0]
+    assert f == pytest.approx(1e15 * (cost_prior + 0.25), rel=1e-13)       # mean residual^2 = 0.25 > 0.2
+    ref_g = 1e15 * (p.cost_grad(x)[1] + (2.0 / 50) * p.basis.T @ (J.T @ np.full(50, -0.5)))   # s - t = -0.5
+    np.testing.assert_allclose(gr, ref_g, rtol=1e-12)
+    invert.lbfgs_invert(p.basis @ x, t_off, p, g, max_iter=1, v2s=v2s, seis_error_tolerance=0.3, scaling=2.0)
+    f, gr = seen["func"](x)
+    assert f == 0.0 and not np.any(gr)                                     # 0.25 < 0.3: zeroed
+    invert.lbfgs_invert(p.basis @ x, t_exact, p, g, max_iter=1, v2s=v2s, scaling=2.0)
+    f, gr = seen["func"](x)
+    assert f == pytest.approx(2.0 * cost_prior, rel=1e-13)                 # tolerance 0: never zeroed
