@@ -72,6 +72,11 @@ def make_requests(B, g, seed0, device, lam):
     return reqs, trues
 
 
+def _copy_request(r):
+    from seismic_amd import invert
+    return invert.Request(geom=r.geom, target=r.target, prior=r.prior, start=r.start.clone(), rid=r.rid)
+
+
 def run_gn(executor, reqs, n_outer):
     """Run n_outer GN passes of every request through `executor`; returns (seconds, models)."""
     import torch
@@ -170,6 +175,19 @@ def measured_traffic(name, kd, table="hbm_traffic.json", field="bytes_per_launch
     return e[field] if field in e else None
 
 
+def hbm_info(traffic, kd):
+    """Counter-measured HBM bytes per launch (profiles/, rocprofv3 FETCH_SIZE /
+    WRITE_SIZE) over this run's launch time, against the 8 TB/s peak: the
+    fraction of HBM bandwidth the kernel really uses, beside the algorithmic
+    roofline fraction (temporal blocking moves fewer bytes than the
+    per-step algorithmic count)."""
+    if traffic is None:
+        return None
+    achieved = traffic / kd["launch_s"]
+    return {"bytes_per_launch": traffic, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK}
+
+
 def l2_info(name, kd):
     """L2 bytes per launch (rocprofv3 TCC_READ_sum / TCC_WRITE_sum, profiles/)
     over this run's launch time, against L2_PEAK; None on another config."""
@@ -182,40 +200,75 @@ def l2_info(name, kd):
             "frac": achieved / L2_PEAK, "hit_rate": hit}
 
 
-def cpu_baseline(g, seconds_hint=25.0):
-    """Oracle (NumPy float64 restatement of the reference propagator) on the
-    host: one forward pass and one forward+adjoint gradient pass over all
-    shots, one process per shot.  Scaled to GN inversions/s with the pass
-    counts of the GPU workload (tangent pass priced at forward+adjoint)."""
+def cpu_baseline(g, lbfgs_iters=30):
+    """The FP64 NumPy oracle (the reference algorithm restated) on the host
+    cores, on bounded samples of the GPU workloads: one forward, one fev
+    (forward + adjoint gradient), one Gauss-Newton product (the oracle's
+    tangent then adjoint) and one FP32 forward, for as many 70x70 models at
+    once as fill the cores at one process per shot; GN inversions/s from those
+    measured pass times and the workload's pass counts.  Also one whole
+    30-iteration LBFGS inversion (memory 10, L2 smoothness prior, the
+    serving workload's), measured."""
     from concurrent.futures import ProcessPoolExecutor
     from oracle import fd2d
+    from oracle import invert as oinv
+    from oracle import priors as opri
     from seismic_amd import synth
     og = fd2d.Geometry(nz=g.nz, nx=g.nx, nbc=g.nbc, nt=g.nt, dt=g.dt, dx=g.dx, freq=g.freq)
-    vec = synth.to_vec(synth.curve_fault_model(g.nz, g.nx, 0))
-    target = fd2d.forward_only(synth.to_vec(synth.depth_linear_start(g.nz, g.nx)), og).reshape(-1)
     n = og.n_shots
-    workers = min(n, len(os.sched_getaffinity(0)))
+    cores = len(os.sched_getaffinity(0))
+    n_models = max(1, cores // n)
+    workers = min(cores, n * n_models)
+    vecs = [synth.to_vec(synth.curve_fault_model(g.nz, g.nx, m)) for m in range(n_models)]
+    target = fd2d.forward_only(synth.to_vec(synth.depth_linear_start(g.nz, g.nx)), og).reshape(-1)
+    dv = np.concatenate([np.random.default_rng(1).normal(0, 20, g.nz * g.nx), [3.0]])
+    units = [(vecs[m], og, s) for m in range(n_models) for s in range(n)]
+
+    def timed(pool, fn, args):
+        t0 = time.perf_counter()
+        list(pool.map(fn, args))
+        return time.perf_counter() - t0
+
     with ProcessPoolExecutor(workers) as ex:
+        list(ex.map(_cpu_fwd, units[:workers]))            # start the workers
+        t_fwd = timed(ex, _cpu_fwd, units)
+        t_fev = timed(ex, _cpu_fev, [u + (target,) for u in units])
+        t_gnp = timed(ex, _cpu_gnp, [u + (dv,) for u in units])
+        t_f32 = timed(ex, _cpu_fwd32, units)
+        # one LBFGS inversion: its fevs run one process per shot
+        true = synth.curve_fault_model(g.nz, g.nx, 100)
+        data = np.concatenate(list(ex.map(_cpu_fwd, [(synth.to_vec(true), og, s) for s in range(n)]))).reshape(-1)
+        start = synth.to_vec(synth.depth_linear_start(g.nz, g.nx), min_vel=1500.0)
+
+        def v2s(vec, vec_adjoint=None, adjoint_on_residual=False):
+            outs = list(ex.map(_cpu_shot, [(vec, og, s, vec_adjoint, adjoint_on_residual) for s in range(n)]))
+            seis = np.concatenate([o[0] for o in outs])
+            return seis, None, (None if vec_adjoint is None else np.sum([o[2] for o in outs], axis=0))
+
         t0 = time.perf_counter()
-        list(ex.map(_cpu_fwd, [(vec, og, s) for s in range(n)]))
-        t_fwd = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        list(ex.map(_cpu_fev, [(vec, og, s, target) for s in range(n)]))
-        t_fev = time.perf_counter() - t0
-    t_gnp = t_fev + t_fev - t_fwd           # tangent ~ forward + adjoint cost, then adjoint
-    t_inv = N_OUTER * (t_fev + N_CG * t_gnp + N_SCALES * t_fwd)
-    cells = n * (g.nt + 1) * og.NZ * og.NX
-    from oracle import priors as opri
+        _, nfev = oinv.lbfgs_invert(start, data, opri.L2Smoothness(g.nz, g.nx, lam=1e-8), og, lbfgs_iters,
+                                    history_size=10, v2s=v2s)
+        t_lbfgs = time.perf_counter() - t0
+    t_inv = N_OUTER * (t_fev + N_CG * t_gnp + N_SCALES * t_fwd) / n_models
+    cells = n * (g.nt + 1) * og.NZ * og.NX * n_models
     t0 = time.perf_counter()
-    opri.label_thresholded_components(_flat_grid(0), 1.0)
+    from oracle import priors as opri2
+    opri2.label_thresholded_components(_flat_grid(0), 1.0)
     t_label = time.perf_counter() - t0
     return {"value": 1.0 / t_inv, "unit": "inversions/s", "cores": workers, "kind": "port",
+            "s_per_forward": t_fwd / n_models, "s_per_fev": t_fev / n_models, "s_per_gn_product": t_gnp / n_models,
+            "s_per_forward_fp32": t_f32 / n_models, "fevs_per_s": n_models / t_fev,
+            "forward_cell_updates_per_s": cells / t_fwd, "forward_fp32_cell_updates_per_s": cells / t_f32,
+            "lbfgs_inversion": {"iterations": lbfgs_iters, "history": 10, "fevs": nfev, "seconds": t_lbfgs,
+                                "inversions_per_s": 1.0 / t_lbfgs, "processes": n},
             "label_flat_s_per_grid": t_label,
-            "s_per_forward": t_fwd, "s_per_fev": t_fev, "fevs_per_s": 1.0 / t_fev,
-            "forward_cell_updates_per_s": cells / t_fwd,
-            "sample": f"NumPy float64 oracle, {g.nz}x{g.nx} nt={g.nt}, {n} shots in {workers} processes: "
-                      f"forward {t_fwd:.2f} s, forward+adjoint gradient {t_fev:.2f} s; "
-                      f"scaled to {N_OUTER} GN passes x ({N_CG} CG + {N_SCALES} line-search forwards)"}
+            "sample": f"NumPy oracle, {g.nz}x{g.nx} nt={g.nt}, {n_models} models x {n} shots in {workers} "
+                      f"processes on {cores} host cores: per model a forward {t_fwd / n_models:.2f} s, "
+                      f"a forward+adjoint gradient {t_fev / n_models:.2f} s, a GN product (tangent then adjoint) "
+                      f"{t_gnp / n_models:.2f} s, all measured (float64; float32 forward "
+                      f"{t_f32 / n_models:.2f} s); value = measured pass times x {N_OUTER} GN passes x "
+                      f"(1 fev + {N_CG} products + {N_SCALES} line-search forwards); one measured "
+                      f"{lbfgs_iters}-iteration LBFGS inversion: {t_lbfgs:.1f} s ({nfev} fevs)"}
 
 
 def _cpu_fwd(a):
@@ -224,11 +277,33 @@ def _cpu_fwd(a):
     return fd2d.forward_only(vec, og, shots=[s])
 
 
+def _cpu_fwd32(a):
+    from oracle import fd2d
+    vec, og, s = a
+    return fd2d.forward_only(vec, og, dtype=np.float32, shots=[s])
+
+
 def _cpu_fev(a):
     from oracle import fd2d
     vec, og, s, target = a
     tgt = target.reshape(og.n_shots, og.nt, og.n_rcv)[s].reshape(-1)
     return fd2d.vel_to_seis(vec, og, vec_adjoint=tgt, adjoint_on_residual=True, shots=[s])[2]
+
+
+def _cpu_gnp(a):
+    """One shot's share of the oracle's Gauss-Newton product J^T J dv."""
+    from oracle import fd2d
+    vec, og, s, dv = a
+    jdv = fd2d.vel_to_seis(vec, og, vec_diff=dv, shots=[s])[1]
+    return fd2d.vel_to_seis(vec, og, vec_adjoint=jdv, shots=[s])[2]
+
+
+def _cpu_shot(a):
+    from oracle import fd2d
+    vec, og, s, dd, resid = a
+    if dd is not None:
+        dd = dd.reshape(og.n_shots, og.nt, og.n_rcv)[s].reshape(-1)
+    return fd2d.vel_to_seis(vec, og, vec_adjoint=dd, adjoint_on_residual=resid, shots=[s])
 
 
 def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep_requests=36, ckpt_every=None):
@@ -295,26 +370,40 @@ def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep
                 "latency_s": serve.percentiles([x.latency for x in recs]),
                 "queue_wait_s": serve.percentiles([x.admitted - x.arrival for x in recs]),
                 "max_in_flight": max(k for _, k in srv.log) if srv.log else 0,
-                "failed": int(sum(x.failed for x in recs))}, reqs
+                "failed": int(sum(x.failed for x in recs)), "records": recs}, reqs
 
     main_run, reqs = timed_run(n_req, rate, seed)
+    served_results = [rec.result for rec in main_run.pop("records")]
     sweep_runs = [timed_run(sweep_requests, r, seed + 7)[0] for r in sweep]
-    # plain path: the same LBFGS inversion, one 70x70 request alone
-    g = geoms[(70, 70)]
-    ex = invert.Executor([g], ckpt_every=0, max_batch=1)
-    r0 = reqs[0]
-    ex.set_target(0, r0.target)
-    r0.meta.clear()
+    for sw in sweep_runs:
+        sw.pop("records")
+    srv.ex.close()
+    # plain path, like for like: every request of the same mix inverted
+    # alone, one after the other (same kernels, one model per plan)
+    ex = invert.Executor([geoms[sz] for sz in sizes], ckpt_every=0, max_batch=1)
+    for j, sz in enumerate(sizes):          # capture each size's graphs before timing
+        r = make_reqs(n_req)[[si for si in size_mix(n_req)].index(j)]
+        ex.set_target(j, r.target)
+        invert.run_machines({j: invert.lbfgs_machine(r, max_iter=1, history_size=10)}, ex)
+    plain_reqs = make_reqs(n_req)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    invert.run_machines({0: invert.lbfgs_machine(r0, max_iter=30, history_size=10)}, ex)
+    plain_results = []
+    for r in plain_reqs:
+        j = sizes.index((r.geom.nz, r.geom.nx))
+        ex.set_target(j, r.target)
+        plain_results.append(invert.run_machines({j: invert.lbfgs_machine(r, max_iter=30, history_size=10)},
+                                                 ex)[j])
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     ex.close()
-    srv.ex.close()
+    same = all(torch.equal(a, b) for a, b in zip(served_results, plain_results))
     out = dict(main_run)
     out.update({"slots": n_req, "sizes": [f"{a}x{b}" for a, b in sizes],
-                "plain_70x70_inversions_per_s": 1.0 / t_plain, "rate_sweep": sweep_runs})
+                "plain_same_mix": {"inversions_per_s": n_req / t_plain, "seconds": t_plain,
+                                   "results_equal_served": same},
+                "speedup_vs_plain_same_mix": out["inversions_per_s"] / (n_req / t_plain),
+                "rate_sweep": sweep_runs})
     return out
 
 
@@ -591,12 +680,14 @@ def main():
     traffic = measured_traffic(dom, kd)
     log(f"[bench] kernels: " + ", ".join(f"{k} {v['launch_s']*1e6:.1f} us/launch" for k, v in kr.items()))
 
-    # ---- plain path: same kernels, one model at a time
+    # ---- plain path: same kernels, one model at a time (its own copy of
+    # request 0, so the batched run's misfit history stays intact)
     ex_p = invert.Executor([g], ckpt_every=0, max_batch=1)
-    preq = [reqs[0]]
+    preq = [_copy_request(reqs[0])]
     run_gn(ex_p, preq, 1)
     log(f"[bench] timing plain path, {args.plain_steps} GN pass(es)")
     t_p, v_p = run_gn(ex_p, preq, args.plain_steps)
+    hist_p0 = list(preq[0].meta["stats"]["misfit_hist"])
     plain = 1 * (args.plain_steps / N_OUTER) / t_p
     log(f"[bench] plain: {t_p:.2f} s, {plain:.4f} inversions/s")
     ex_p.close()
@@ -604,9 +695,11 @@ def main():
     # per launch (k = 1, 64x32 windows, no temporal blocking) inside the
     # captured graphs
     ex_s = invert.Executor([g], ckpt_every=0, max_batch=1, configure=per_step_configs)
-    run_gn(ex_s, preq, 1)
+    sreq = [_copy_request(reqs[0])]
+    run_gn(ex_s, sreq, 1)
     log("[bench] timing plain path with one step per launch, 1 GN pass")
-    t_s, _ = run_gn(ex_s, preq, 1)
+    t_s, _ = run_gn(ex_s, sreq, 1)
+    hist_s0 = list(sreq[0].meta["stats"]["misfit_hist"])
     plain_step = (1 / N_OUTER) / t_s
     log(f"[bench] plain, one step per launch: {t_s:.2f} s per GN pass, {plain_step:.4f} inversions/s")
     ex_s.close()
@@ -671,9 +764,11 @@ def main():
                                       "speedup": value / plain_step,
                                       "configs": "k=1, 64x32 windows, every kernel kind"},
         "misfit": {"batched_mean_final": float(np.mean(mis_b)), "after_gn_passes": args.steps,
-                   "model0_batched": hist_b0[:args.plain_steps],
-                   "model0_plain": preq[0].meta["stats"]["misfit_hist"],
-                   "equal": hist_b0[:args.plain_steps] == preq[0].meta["stats"]["misfit_hist"]},
+                   "model0_batched": hist_b0,
+                   "model0_plain": hist_p0,
+                   "model0_plain_one_step_per_launch": hist_s0,
+                   "equal": hist_b0[:len(hist_p0)] == hist_p0 and hist_b0[:len(hist_s0)] == hist_s0,
+                   "passes_compared": len(hist_p0)},
         "serving": serving,
         "extras": extras,
         "roofline": {"bound": "hbm", "kernel": kd["kernel"], "pass": dom,
@@ -682,12 +777,14 @@ def main():
                      "launch_us": kd["launch_s"] * 1e6, "alg_bytes_per_cell_update": ALG_BYTES[dom],
                      "cell_updates_per_launch": kd["cell_updates_per_launch"], "k": kd["k"],
                      "window": kd["window"], "strips": kd["strips"], "models": kd["models"],
+                     "hbm_measured": hbm_info(traffic, kd),
                      "l2": l2_info(dom, kd)},
         "kernels": {k: {"kernel": v["kernel"], "launch_us": v["launch_s"] * 1e6, "k": v["k"], "window": v["window"],
                         "strips": v["strips"], "models": v["models"],
                         "gcell_updates_per_s": v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
                         "alg_GBps": ALG_BYTES[k] * v["cell_updates_per_launch"] / v["launch_s"] / 1e9,
-                        "share_of_gn_pass_s": share[k], "l2": l2_info(k, v)}
+                        "share_of_gn_pass_s": share[k],
+                        "hbm_measured": hbm_info(measured_traffic(k, v), v), "l2": l2_info(k, v)}
                     for k, v in kr.items()},
         "cpu_baseline": cpu,
     }

@@ -463,6 +463,7 @@ class Executor:
     def set_target(self, slot: int, target: torch.Tensor):
         self.targets[slot] = target.to(self.device, torch.float32).reshape(-1)
         self._dd = None
+        self._dd_fan = None
 
     def _targets_cat(self):
         if getattr(self, "_dd", None) is None:
@@ -534,9 +535,25 @@ class Executor:
             _, grad = P.gn_product(v, dv)
         return {i: grad[int(P.vec_off[l]):int(P.vec_off[l + 1])].clone() for i, l in zip(ids, loc)}
 
+    def _fanout_plan(self):
+        """One forward-only plan of ``fwd_fanout`` models per slot (slot i
+        owns models i*F .. i*F+F-1): every line search of every slot runs on
+        it, the models in use activated per launch, so its size is fixed."""
+        F = self.fwd_fanout
+        P = self._plan(("fB",), [g for g in self.slots for _ in range(F)], CAP_FORWARD)
+        return P
+
+    def _fan_targets(self, P):
+        if getattr(self, "_dd_fan", None) is None:
+            F = self.fwd_fanout
+            self._dd_fan = torch.cat([(t if t is not None else torch.zeros(g.n_data, device=self.device))
+                                      for t, g in zip(self.targets, self.slots) for _ in range(F)])
+        return self._dd_fan
+
     def _fwd(self, ids, pending):
         """Line-search forwards: every trial model of every op in one launch
-        (batched) or one model per launch (plain)."""
+        (batched; rounds of ``fwd_fanout`` trial points per slot) or one
+        model per launch (plain)."""
         out = {}
         if self.max_batch == 1:
             for i in ids:
@@ -547,19 +564,32 @@ class Executor:
                     ms.append(P.forward_misfit(P._vec(v), self.targets[i]))
                 out[i] = [float(m[0]) for m in torch.stack(ms).cpu()]
             return out
-        groups = [(i, len(pending[i].vecs)) for i in ids]
-        geoms = [self.slots[i] for i, k in groups for _ in range(k)]
-        key = ("fB", tuple(geoms))
-        P = self._plan(key, geoms, CAP_FORWARD)
-        vecs = [v for i, _ in groups for v in pending[i].vecs]
-        v = self._pack(P, list(range(len(vecs))), vecs)
-        dd = torch.cat([self.targets[i] for i, k in groups for _ in range(k)])
-        mis = P.forward_misfit(v, dd).cpu().tolist()
-        pos = 0
-        for i, k in groups:
-            out[i] = mis[pos:pos + k]
-            pos += k
-        return out
+        F = self.fwd_fanout
+        P = self._fanout_plan()
+        dd = self._fan_targets(P)
+        res = {i: [] for i in ids}
+        n_rounds = max((len(pending[i].vecs) + F - 1) // F for i in ids)
+        for r in range(n_rounds):
+            part = {i: pending[i].vecs[r * F:(r + 1) * F] for i in ids}
+            models = [i * F + k for i in ids for k in range(len(part[i]))]
+            if not models:
+                continue
+            live = set(models)
+            mask = tuple(m in live for m in range(P.n_models))
+            if getattr(P, "_mask", None) != mask:
+                P.set_active(mask)
+                P._mask = mask
+            v = self._pack(P, models, [x for i in ids for x in part[i]])
+            mis = P.forward_misfit(v, dd).cpu()
+            for i in ids:
+                res[i] += mis[[i * F + k for k in range(len(part[i]))]].tolist()
+        return res
+
+
+def fanout_plan_bytes(slots: Sequence[Geometry], fanout: int = 50) -> int:
+    """Device bytes of an executor's line-search fan-out plan over ``slots``."""
+    from .propagator import plan_bytes
+    return plan_bytes([g for g in slots for _ in range(fanout)], ckpt_every=0, caps=CAP_FORWARD)
 
 
 def run_machines(machines: dict, executor: Executor, on_finish: Callable | None = None) -> dict:

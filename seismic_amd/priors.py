@@ -10,6 +10,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 import torch
 
 
@@ -47,6 +49,47 @@ class Prior:
         """Prior cost (lam included) of every row of X, as a tensor; the
         Gauss-Newton line search evaluates all its trial points with it."""
         return torch.stack([torch.as_tensor(self.cost_grad(x)[0], dtype=X.dtype, device=X.device) for x in X])
+
+    # ---- the reference's Prior interface (core/seis_prior.py:17-65), so code
+    # written against it (model.prior.λ = ..., prior.basis_vectors,
+    # prior.compute_cost_and_gradient(x, True)) runs unchanged
+    prepped: bool = False
+
+    @property
+    def λ(self) -> float:
+        return self.lam
+
+    @λ.setter
+    def λ(self, value: float):
+        self.lam = value
+
+    @property
+    def basis_vectors(self) -> torch.Tensor:
+        """Dense basis, (nz*nx + 1) x N float64 (the identity for identity-basis
+        priors, built on first use)."""
+        if self.basis is not None:
+            return self.basis
+        if getattr(self, "_eye", None) is None:
+            self._eye = torch.eye(self.nz * self.nx + 1, dtype=torch.float64, device=self.device)
+        return self._eye
+
+    def prep(self):
+        """Prior.prep (:27-35): the basis is built at construction here."""
+        self.prepped = True
+
+    def adapt(self, velocity_guess):
+        """Prior.adapt (:37-45); priors without an adaptive basis keep theirs."""
+
+    def compute_cost_and_gradient(self, x, compute_gradient: bool = False):
+        """Prior.compute_cost_and_gradient (:47-65): x of shape (N, 1);
+        returns the 0-d cost (lam applied), with the (N, 1) gradient when
+        compute_gradient."""
+        x = torch.as_tensor(x)
+        if tuple(x.shape) != (self.N, 1):
+            raise ValueError(f"x must have shape ({self.N}, 1), got {tuple(x.shape)}")
+        cost, grad = self.cost_grad(x.reshape(-1).to(device=self.device, dtype=torch.float64))
+        cost = torch.as_tensor(cost, dtype=torch.float64, device=self.device)
+        return (cost, grad.reshape(-1, 1)) if compute_gradient else cost
 
 
 class RowTotalVariation(Prior):
@@ -269,8 +312,17 @@ class RestrictFlatAreas(Prior):
     def N(self) -> int:
         return self.nz * self.nx + 1 if self.labels is None else self.count + 1
 
-    def adapt(self, vec_guess: torch.Tensor):
+    @property
+    def underlying_prior(self):
+        """The reference's attribute name (core/seis_prior.py:247)."""
+        return self.underlying
+
+    def adapt(self, vec_guess):
+        """_adapt (:256-272) on a parameter vector, or on the reference's
+        Velocity object (``.data`` the (nz, nx) grid, ``.min_vel``)."""
         n = self.nz * self.nx
+        if hasattr(vec_guess, "data") and hasattr(vec_guess, "min_vel") and not torch.is_tensor(vec_guess):
+            vec_guess = torch.as_tensor(np.asarray(vec_guess.data, dtype=np.float64)).reshape(-1)
         v = torch.as_tensor(vec_guess).reshape(-1)[:n].to(self.device, torch.float64).reshape(self.nz, self.nx)
         (lab,), (count,) = label_flat([v], self.diff_threshold)
         self.labels = lab.reshape(-1)

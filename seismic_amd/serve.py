@@ -1,10 +1,12 @@
 """Continuous-batching inversion server on one GPU.
 
 Requests (``invert.Request``) arrive over time.  The server owns one plan
-whose slots cover a fixed mix of grid sizes; a request is admitted into a
-free slot of its size when its HBM footprint (its share of the plan:
-wavefields, checkpoints, coefficient fields, data; plus its LBFGS history)
-fits the budget.  Waiting requests are admitted first-fit in arrival order,
+whose slots cover a fixed mix of grid sizes.  The plan's wavefields,
+checkpoints and coefficient fields (every slot's, allocated together), and
+the line-search fan-out plan when the machines evaluate line searches, are
+reserved against the HBM budget when the server is built; a request is
+admitted into a free slot of its size when its own state (LBFGS history,
+observed data, vector temporaries) fits what is left.  Waiting requests are admitted first-fit in arrival order,
 so one waiting for a busy size does not hold back the others.  Every loop iteration collects the pending propagator op
 of every in-flight request and runs them as one batched launch per op kind;
 requests that arrive meanwhile join at the next iteration (continuous
@@ -26,7 +28,7 @@ import torch
 
 from ._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
 from .geometry import Geometry
-from .invert import Executor, Request, lbfgs_machine
+from .invert import Executor, Request, fanout_plan_bytes, lbfgs_machine
 from .propagator import plan_bytes
 
 
@@ -45,12 +47,14 @@ class Served:
         return self.finished - self.arrival
 
 
-def request_footprint(g: Geometry, ckpt_every: int, history: int) -> int:
-    """Device bytes one request holds: its slot of the plan plus LBFGS pairs."""
-    slot = plan_bytes([g], ckpt_every=ckpt_every, caps=CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT)
+def request_footprint(g: Geometry, history: int) -> int:
+    """Device bytes one request holds beyond the plans: its LBFGS history
+    ring (history + 1 pairs of float64 vectors plus coefficients), about
+    eight float64 vectors of its loop (x, d, g, previous g, trial points) and
+    its float32 observed data."""
     n = g.n_params
-    lbfgs = 8 * n * (2 * history + 8)        # (s, y) pairs + x, d, g, prev g, trial points
-    return slot + lbfgs + 4 * g.n_data
+    lbfgs = 8 * ((history + 1) * (2 * n + 2) + 1) + 8
+    return lbfgs + 8 * n * 8 + 4 * g.n_data
 
 
 class InversionServer:
@@ -66,23 +70,33 @@ class InversionServer:
 
     def __init__(self, slots: Sequence[Geometry], hbm_budget: int, ckpt_every: int | None = 50,
                  max_iter: int = 30, history_size: int = 10, tolerance_grad: float = 1e-7,
-                 tolerance_change: float = 0.0, device="cuda", machine: Callable | None = None):
+                 tolerance_change: float = 0.0, device="cuda", machine: Callable | None = None,
+                 line_search: bool | None = None):
         self.slots = list(slots)
+        self.budget = int(hbm_budget)
+        caps = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT
+        # the default machine (LBFGS) runs no line-search fan-out; a custom
+        # machine (Gauss-Newton passes, chains) may
+        self.line_search = (machine is not None) if line_search is None else bool(line_search)
+        fan = fanout_plan_bytes(self.slots) if self.line_search else 0
         if ckpt_every is None:
-            # keep every slot's full forward history when all of them fit the
+            # keep every slot's full forward history when the plan fits the
             # budget (the adjoint then reads it instead of recomputing
             # checkpoint segments), else checkpoint every 50 steps
-            caps = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT
-            full = sum(plan_bytes([g], ckpt_every=0, caps=caps) for g in self.slots)
-            ckpt_every = 0 if full <= int(hbm_budget) else 50
+            full = plan_bytes(self.slots, ckpt_every=0, caps=caps)
+            ckpt_every = 0 if full + fan <= self.budget else 50
         self.ckpt_every = ckpt_every
+        self.ex = Executor(self.slots, ckpt_every=ckpt_every, device=device)
+        self.plan_bytes = plan_bytes(self.slots, ckpt_every=ckpt_every, caps=caps)
+        self.reserved = self.plan_bytes + fan
+        if self.reserved > self.budget:
+            raise ValueError(f"the plans of these slots need {self.reserved} bytes, over the HBM budget "
+                             f"of {self.budget}")
         self.kw = dict(max_iter=max_iter, history_size=history_size, tolerance_grad=tolerance_grad,
                        tolerance_change=tolerance_change)
         self.machine = machine if machine is not None else (lambda req: lbfgs_machine(req, **self.kw))
-        self.budget = int(hbm_budget)
-        self.ex = Executor(self.slots, ckpt_every=ckpt_every, device=device)
-        self._fp = {g: request_footprint(g, ckpt_every, history_size) for g in set(self.slots)}
-        self.used = 0
+        self._fp = {g: request_footprint(g, history_size) for g in set(self.slots)}
+        self.used = self.reserved
         self.free: dict = {}
         for i, g in enumerate(self.slots):
             self.free.setdefault(g, []).append(i)
@@ -96,9 +110,9 @@ class InversionServer:
         of its size, or a footprint above the whole budget."""
         if req.geom not in self.free:
             raise ValueError(f"no slot of this server fits a {req.geom.nz}x{req.geom.nx} request")
-        if self.footprint(req) > self.budget:
+        if getattr(self, "reserved", 0) + self.footprint(req) > self.budget:
             raise ValueError(f"a {req.geom.nz}x{req.geom.nx} request's footprint ({self.footprint(req)} bytes) "
-                             f"exceeds the server's HBM budget ({self.budget} bytes)")
+                             f"exceeds the server's HBM budget ({self.budget} bytes) beyond its plans")
 
     def _try_admit(self, req: Request):
         free = self.free.get(req.geom)

@@ -17,8 +17,17 @@ every surface column), one curved-fault model each (seed 20 + size index):
   ref{nz}x{nx}.npz  seismogram of shot 0 (float32, (999, nx)) and J^T dd for
                     the float32 data vector dd = adjoint_probe(...) (float64)
 
+An LBFGS inversion as the reference's FlatVel model runs it in test mode
+(core/seis_model.py:15-27: row total variation, lam 1e-8, 30 iterations,
+tolerance_change 1e-9; core/seis_invert.py:109-152): a flat-layered 70x70
+model (seed 3), started from its rows smoothed over 13 rows:
+
+  ref70_lbfgs.npz   float32 observed data (the oracle forward of the true
+                    model), the start vector, the oracle's inverted vector,
+                    its evaluation count and data misfit
+
 Run from the repo root: python tests/golden/make_golden.py [ref70] [sizes]
-(~5 minutes).
+[lbfgs70]  (~5 minutes each part; lbfgs70 uses 5 processes).
 """
 import os
 import sys
@@ -46,11 +55,52 @@ def adjoint_probe(n_shots, nt, n_rcv, seed=5):
     return np.random.default_rng(seed).standard_normal(n_shots * nt * n_rcv).astype(np.float32)
 
 
-def main(parts=("ref70", "sizes")):
+def main(parts=("ref70", "sizes", "lbfgs70")):
     if "ref70" in parts:
         write_ref70()
     if "sizes" in parts:
         write_sizes()
+    if "lbfgs70" in parts:
+        write_lbfgs70()
+
+
+def lbfgs70_case():
+    """(true model (70, 70), start vector) of the LBFGS fixture."""
+    true = synth.make_model("flat", 70, 70, 3)
+    rows = true[:, 0]
+    k = np.ones(13) / 13
+    smooth = np.convolve(np.pad(rows, 6, mode="edge"), k, mode="valid")
+    return true, synth.to_vec(np.repeat(smooth[:, None], 70, 1), min_vel=float(true.min()))
+
+
+def _shot(a):
+    vec, g, s, kw = a
+    kw = dict(kw)
+    if kw.get("vec_adjoint") is not None:
+        kw["vec_adjoint"] = kw["vec_adjoint"].reshape(g.n_shots, g.nt, g.n_rcv)[s].reshape(-1)
+    return fd2d.vel_to_seis(vec, g, shots=[s], **kw)
+
+
+def write_lbfgs70():
+    from concurrent.futures import ProcessPoolExecutor
+
+    from oracle import invert as oinv
+    from oracle import priors as opri
+    g = fd2d.Geometry()
+    true, start = lbfgs70_case()
+    with ProcessPoolExecutor(g.n_shots) as pool:
+        def v2s(vec, **kw):
+            outs = list(pool.map(_shot, [(vec, g, s, kw) for s in range(g.n_shots)]))
+            gr = None if outs[0][2] is None else np.sum([o[2] for o in outs], axis=0)
+            return np.concatenate([o[0] for o in outs]), None, gr
+
+        target32 = v2s(synth.to_vec(true))[0].astype(np.float32)
+        v, nfev = oinv.lbfgs_invert(start, target32.astype(np.float64), opri.RowTotalVariation(70, 70, lam=1e-8), g,
+                                    max_iter=30, tolerance_change=1e-9, v2s=v2s)
+        misfit = float(np.mean((v2s(v)[0] - target32.astype(np.float64)) ** 2))
+    np.savez_compressed(os.path.join(HERE, "ref70_lbfgs.npz"), target=target32, start=start, v=v,
+                        nfev=np.array(nfev), misfit=np.array(misfit))
+    print("written lbfgs70", nfev, misfit, np.abs(v - synth.to_vec(true))[:-1].mean())
 
 
 def write_ref70():
@@ -80,4 +130,4 @@ def write_sizes():
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("ref70", "sizes"))
+    main(tuple(sys.argv[1:]) or ("ref70", "sizes", "lbfgs70"))

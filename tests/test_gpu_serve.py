@@ -52,18 +52,24 @@ def test_served_lbfgs_equals_alone():
 
 
 def test_server_keeps_full_history_when_it_fits():
-    """ckpt_every=None keeps every slot's full forward history when all of
-    them fit the budget (one byte less and it checkpoints every 50 steps);
-    results are those of the plain run either way."""
+    """ckpt_every=None keeps every slot's full forward history when the
+    plan fits the budget (one byte less and it checkpoints every 50 steps; a
+    budget below the checkpointed plan is refused); the plan is reserved
+    against the budget when the server is built; results are those of the
+    plain run either way."""
     from seismic_amd._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
     from seismic_amd.propagator import plan_bytes
     geoms, reqs = _requests()
     slots = [geoms[0], geoms[1], geoms[0], geoms[1]]
-    full = sum(plan_bytes([g], ckpt_every=0, caps=CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT) for g in slots)
+    caps = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT
+    full = plan_bytes(slots, ckpt_every=0, caps=caps)
     kw = dict(max_iter=8, history_size=5, tolerance_grad=1e-7, tolerance_change=0.0)
     small = serve.InversionServer(slots, hbm_budget=full - 1, ckpt_every=None, device="cuda", **kw)
-    assert small.ckpt_every == 50
+    assert small.ckpt_every == 50 and small.used == plan_bytes(slots, ckpt_every=50, caps=caps)
     small.ex.close()
+    with pytest.raises(ValueError):        # not even the checkpointed plan fits
+        serve.InversionServer(slots, hbm_budget=plan_bytes(slots, ckpt_every=50, caps=caps) - 1, ckpt_every=None,
+                              device="cuda", **kw)
     srv = serve.InversionServer(slots, hbm_budget=1 << 40, ckpt_every=None, device="cuda", **kw)
     assert srv.ckpt_every == 0
     alone = invert.invert(_clone(reqs), method="lbfgs", batched=False, **kw)

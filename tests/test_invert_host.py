@@ -283,3 +283,29 @@ def test_lbfgs_closure_matches_oracle_on_linear_model(scaling, tol):
     if tol > 0:
         assert nfev < 12 and r.meta["stats"]["misfit"] < tol    # stopped by the misfit tolerance
     np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-7)
+
+
+def test_reference_prior_interface():
+    """The reference's Prior surface (core/seis_prior.py:17-65): λ,
+    basis_vectors, prep / adapt, compute_cost_and_gradient(x (N, 1),
+    compute_gradient), on the product's priors; the same values as their
+    own cost_grad and basis."""
+    rng = np.random.default_rng(4)
+    for p in (priors.RowTotalVariation(7, 9, device="cpu"), priors.TotalVariation(7, 9, device="cpu"),
+              priors.L2Smoothness(7, 9, lam=2e-8, device="cpu")):
+        p.prep()
+        assert p.prepped
+        p.λ = 3e-7
+        assert p.lam == 3e-7 and p.λ == 3e-7
+        B = p.basis_vectors
+        assert tuple(B.shape) == (7 * 9 + 1, p.N) and B.dtype == torch.float64
+        x = torch.tensor(rng.normal(2000, 50, (p.N, 1)))
+        c = p.compute_cost_and_gradient(x)
+        c2, g2 = p.compute_cost_and_gradient(x, True)
+        c3, g3 = p.cost_grad(x.reshape(-1))
+        assert c.shape == () and float(c) == float(c2) == float(c3)
+        assert tuple(g2.shape) == (p.N, 1) and torch.equal(g2.reshape(-1), g3)
+        with pytest.raises(ValueError):
+            p.compute_cost_and_gradient(x.reshape(-1))
+        p.adapt(torch.zeros(7 * 9 + 1))          # no adaptive basis: unchanged
+        assert torch.equal(p.basis_vectors, B)

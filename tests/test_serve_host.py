@@ -199,3 +199,28 @@ def test_poisson_arrivals_and_percentiles():
     assert abs(len(a) / a[-1] - 100.0) < 10.0
     p = serve.percentiles([1.0, 2.0, 3.0, 4.0])
     assert p["p50"] == pytest.approx(2.5) and p["max"] == 4.0
+
+
+def test_server_reserves_its_plans_against_the_budget():
+    """The plans (all slots' wavefields, and the line-search fan-out plan when
+    the machines run line searches) are reserved at construction; requests
+    are admitted against what is left, by their own state."""
+    from seismic_amd._lib import CAP_ADJOINT, CAP_FORWARD, CAP_TANGENT
+    from seismic_amd.propagator import plan_bytes
+    g = Geometry(nz=8, nx=9, nbc=4, nt=150)
+    caps = CAP_FORWARD | CAP_TANGENT | CAP_ADJOINT
+    plan = plan_bytes([g, g], ckpt_every=0, caps=caps)
+    fp = serve.request_footprint(g, 10)
+    srv = serve.InversionServer([g, g], hbm_budget=plan + 2 * fp, ckpt_every=0, device="cpu")
+    assert srv.used == plan and srv.reserved == plan
+    a, b = _req(g, 0), _req(g, 1)
+    assert srv._try_admit(a) is not None and srv._try_admit(b) is not None
+    assert srv.used == plan + 2 * fp
+    tight = serve.InversionServer([g, g], hbm_budget=plan + fp, ckpt_every=0, device="cpu")
+    assert tight._try_admit(a) is not None and tight._try_admit(b) is None     # one request's state fits
+    fan = invert.fanout_plan_bytes([g, g])
+    gn = serve.InversionServer([g, g], hbm_budget=plan + fan + fp, ckpt_every=0, device="cpu",
+                               machine=lambda r: invert.gn_machine(r))
+    assert gn.reserved == plan + fan
+    with pytest.raises(ValueError):
+        serve.InversionServer([g, g], hbm_budget=plan - 1, ckpt_every=0, device="cpu")
