@@ -106,6 +106,15 @@ int seis_plan_layout(const seis_plan_t* plan, int64_t* vec_off /* host */,
  * All models are active after creation. */
 int seis_set_active(seis_plan_t* plan, const int32_t* active /* host */);
 
+/* seis_set_active ordered on `stream` instead of the whole device: the work
+ * lists are compacted to the active models by a kernel on the stream (no
+ * device synchronisation, no blocking copy); later calls on the same stream
+ * see the new mask.  Plans of more than 24576 models fall back to
+ * synchronising the stream and seis_set_active.  No reference counterpart:
+ * the serving loop's per-launch active set (core/seis_forward2.py runs one
+ * model at a time). */
+int seis_set_active_async(seis_plan_t* plan, const int32_t* active /* host */, seis_stream_t stream);
+
 /* Forward: seis = F(vec).  (vel_to_seis with vec_diff = vec_adjoint = None) */
 int seis_forward(seis_plan_t* plan, const float* vec, float* seis,
                  seis_stream_t stream);

@@ -44,6 +44,7 @@ EXPORTS = {
     "seis_plan_destroy": (c_int32, [c_void_p]),
     "seis_plan_layout": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "seis_set_active": (c_int32, [c_void_p, POINTER(c_int32)]),
+    "seis_set_active_async": (c_int32, [c_void_p, POINTER(c_int32), c_void_p]),
     "seis_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_forward_misfit": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "seis_tangent": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -1718,6 +1718,63 @@ __global__ __launch_bounds__(NTHR) void k_copy_active(const ModelDev* models, co
     dst[m.seis_off + i] = src[m.seis_off + i];
 }
 
+// Activity mask for k_compact, passed by value: one bit per model.
+constexpr int kMaskWords = 768;   // 24576 models
+struct CompactArgs {
+  const TileEnt* all[SEIS_KIND_COUNT];
+  TileEnt* tmp[SEIS_KIND_COUNT];
+  TileEnt* out[SEIS_KIND_COUNT];
+  int* count[SEIS_KIND_COUNT];
+  int n_all[SEIS_KIND_COUNT];
+  uint32_t bits[kMaskWords];
+};
+
+// Stream-ordered seis_set_active: workgroup k compacts kind k's work list to
+// the tiles of active models (the sequence order of make_tiles, then its XCD
+// interleave: the block on XCD e % 8 takes a contiguous eighth), and
+// workgroup 0 also writes the models' activity flags.  Replaces the host
+// rebuild and upload, so no device synchronisation is needed.
+__global__ __launch_bounds__(1024) void k_compact(CompactArgs a, const WaveDev* waves, ModelDev* models,
+                                                  int n_models) {
+  __shared__ int scan[1024];
+  __shared__ int base;
+  const int k = blockIdx.x, tid = threadIdx.x;
+  auto active = [&](int m) { return (a.bits[m >> 5] >> (m & 31)) & 1u; };
+  if (k == 0)
+    for (int i = tid; i < n_models; i += 1024) models[i].active = (int)active(i);
+  const int n = a.n_all[k];
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int i = c0 + tid;
+    TileEnt te{-1, 0, 0, 0};
+    int f = 0;
+    if (i < n) {
+      te = a.all[k][i];
+      f = (int)active(waves[te.w].model);
+    }
+    scan[tid] = f;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan
+      const int add = tid >= d ? scan[tid - d] : 0;
+      __syncthreads();
+      scan[tid] += add;
+      __syncthreads();
+    }
+    if (f) a.tmp[k][base + scan[tid] - 1] = te;
+    __syncthreads();
+    if (tid == 1023) base += scan[1023];
+    __syncthreads();
+  }
+  const int total = base;
+  const int per = (total + NXCD - 1) / NXCD;
+  for (int e = tid; e < per * NXCD; e += 1024) {
+    const int lg = (e % NXCD) * per + e / NXCD;
+    a.out[k][e] = lg < total ? a.tmp[k][lg] : TileEnt{-1, 0, 0, 0};
+  }
+  if (tid == 0) *a.count[k] = per * NXCD;
+}
+
 int64_t align64(int64_t n) { return (n + 63) & ~int64_t(63); }
 
 // Kernel launch configuration: LDS window (rx x rz cells, halo included),
@@ -1879,6 +1936,9 @@ struct TileList {
   int* d_count = nullptr;
   int cap = 0;     // entries allocated (all models active)
   int grid = 0;    // persistent grid (multiple of 8), fixed while the config is
+  TileEnt* d_all = nullptr;  // every model's tiles in sequence order (compaction input)
+  TileEnt* d_tmp = nullptr;  // compacted sequence (before the XCD interleave)
+  int n_all = 0;
 };
 
 }  // namespace
@@ -1899,6 +1959,7 @@ struct seis_plan {
   float* d_dd = nullptr;
   double* d_fold = nullptr;
   double* d_part = nullptr;
+  int* d_mask = nullptr;      // activity mask staged for the device-side compaction
   KCfg cfg[SEIS_KIND_COUNT];
   TileList tl[SEIS_KIND_COUNT];
   hipStream_t cap = nullptr;
@@ -1922,12 +1983,15 @@ void drop_graphs(seis_plan* p) {
 void plan_free(seis_plan* p) {
   drop_graphs(p);
   void* bufs[] = {p->d_models, p->d_waves, p->d_params, p->d_wparam, p->d_fold, p->d_part,
-                  p->d_alpha, p->d_dalpha, p->d_wavelet, p->d_arena, p->d_seis, p->d_dseis, p->d_dd};
+                  p->d_alpha, p->d_dalpha, p->d_wavelet, p->d_arena, p->d_seis, p->d_dseis, p->d_dd,
+                  p->d_mask};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& t : p->tl) {
     if (t.d) (void)hipFree(t.d);
     if (t.d_count) (void)hipFree(t.d_count);
+    if (t.d_all) (void)hipFree(t.d_all);
+    if (t.d_tmp) (void)hipFree(t.d_tmp);
   }
   if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
@@ -2081,7 +2145,7 @@ void launch_born(const KCfg& c, int grid, hipStream_t s, const BornArgs& a) {
 // persistent block handling list entry e (on XCD e % 8) receives a contiguous
 // eighth of the tile sequence: neighbouring tiles of a wavefield, which share
 // halo rows and columns, stay in one XCD's L2.
-std::vector<TileEnt> make_tiles(const Layout& L, const KCfg& c, bool all_active) {
+std::vector<TileEnt> tile_sequence(const Layout& L, const KCfg& c, bool all_active) {
   std::vector<TileEnt> seq;
   const bool stream = c.strips == 2;
   const int IX = c.rx - 4 * c.k;
@@ -2094,6 +2158,11 @@ std::vector<TileEnt> make_tiles(const Layout& L, const KCfg& c, bool all_active)
       for (int x0 = 0; x0 < w.NX; x0 += IX)
         seq.push_back(TileEnt{wi, z0, x0, stream ? std::min(IZ, w.NZ - z0) : 0});
   }
+  return seq;
+}
+
+std::vector<TileEnt> make_tiles(const Layout& L, const KCfg& c, bool all_active) {
+  const std::vector<TileEnt> seq = tile_sequence(L, c, all_active);
   const int n = (int)seq.size();
   const int per = (n + NXCD - 1) / NXCD;
   std::vector<TileEnt> out((size_t)per * NXCD, TileEnt{-1, 0, 0, 0});
@@ -2131,6 +2200,18 @@ int setup_kind(seis_plan* p, int kind) {
     int rc = dev_alloc(&t.d_count, 1);
     if (rc) return rc;
   }
+  const std::vector<TileEnt> all = tile_sequence(p->L, p->cfg[kind], true);
+  if ((int)all.size() > t.n_all || !t.d_all) {
+    if (t.d_all) (void)hipFree(t.d_all);
+    if (t.d_tmp) (void)hipFree(t.d_tmp);
+    t.d_all = t.d_tmp = nullptr;
+    t.n_all = 0;
+    int rc = dev_alloc(&t.d_all, std::max<int64_t>(1, (int64_t)all.size()));
+    if (!rc) rc = dev_alloc(&t.d_tmp, std::max<int64_t>(1, (int64_t)all.size()));
+    if (rc) return rc;
+  }
+  t.n_all = (int)all.size();
+  if (t.n_all) HIP_TRY(hipMemcpy(t.d_all, all.data(), sizeof(TileEnt) * t.n_all, hipMemcpyHostToDevice));
   int per_cu = 0;
   const int nthr = p->cfg[kind].strips == 2 ? p->cfg[kind].rx : shape_threads(kind, p->cfg[kind].rx, p->cfg[kind].rz);
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kind, p->cfg[kind]), nthr, 0));
@@ -2626,6 +2707,31 @@ int seis_set_active(seis_plan_t* p, const int32_t* active) {
   HIP_TRY(hipMemcpy(p->d_models, p->L.models.data(), sizeof(ModelDev) * p->n_models,
                     hipMemcpyHostToDevice));
   for (int k = 0; k < SEIS_KIND_COUNT; ++k) TRY(upload_tiles(p, k));
+  return SEIS_OK;
+}
+
+int seis_set_active_async(seis_plan_t* p, const int32_t* active, seis_stream_t stream) {
+  if (!p || !active) return fail(SEIS_EINVAL, "NULL argument");
+  if (p->n_models > kMaskWords * 32) {   // mask too large for kernel arguments
+    HIP_TRY(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+    return seis_set_active(p, active);
+  }
+  CompactArgs a{};
+  for (int i = 0; i < p->n_models; ++i) {
+    p->L.models[i].active = active[i] ? 1 : 0;
+    if (active[i]) a.bits[i >> 5] |= 1u << (i & 31);
+  }
+  for (int k = 0; k < SEIS_KIND_COUNT; ++k) {
+    const TileList& t = p->tl[k];
+    a.all[k] = t.d_all;
+    a.tmp[k] = t.d_tmp;
+    a.out[k] = t.d;
+    a.count[k] = t.d_count;
+    a.n_all[k] = t.n_all;
+  }
+  hipLaunchKernelGGL(k_compact, dim3(SEIS_KIND_COUNT), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), a,
+                     p->d_waves, p->d_models, p->n_models);
+  HIP_TRY(hipGetLastError());
   return SEIS_OK;
 }
 

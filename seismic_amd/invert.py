@@ -379,13 +379,19 @@ def gn_machine(req: Request, n_outer: int = 10, n_cg: int | None = 20, n_scales:
     stats = req.meta.setdefault("stats", {"fev": 0, "gnp": 0, "fwd": 0})
     for _ in range(n_outer):
         x_guess = prior.to_basis(vec)
-        misfit, gres = yield Op("fev", vec=prior.from_basis(x_guess))
+        v_fev = prior.from_basis(x_guess)
+        misfit, gres = yield Op("fev", vec=v_fev)
         stats["fev"] += 1
         rhs = -prior.basis_T(gres) / N
         rhs[:-1] -= prior.lam * prior.apply_P(x_guess[:-1])
+        # J is taken at vec (:86); when the projected fev point is vec itself
+        # the products name that same vector, so the executor reuses the
+        # fev's saved forward (identity basis: always; else one check per pass)
+        identity = prior.is_identity() if hasattr(prior, "is_identity") else prior.basis is None
+        lin = v_fev if (identity or bool(torch.equal(v_fev, vec))) else vec
 
         def apply_A(p):
-            jtj = yield Op("gnp", vec=vec, dvec=prior.from_basis(p))
+            jtj = yield Op("gnp", vec=lin, dvec=prior.from_basis(p))
             stats["gnp"] += 1
             out = prior.basis_T(jtj) / N
             out[:-1] += prior.lam * prior.apply_P(p[:-1])
@@ -515,11 +521,26 @@ class Executor:
             P._mask = mask
         return P, list(ids)
 
+    @staticmethod
+    def _note_saving(P, ids, loc, pending):
+        """Record, per model of plan P, which slot's op vector its saved
+        forward history now belongs to (host bookkeeping: the vector object
+        itself, so the check costs no device round trip)."""
+        rec = P.__dict__.setdefault("_saved_for", {})
+        for i, l in zip(ids, loc):
+            rec[l] = (i, pending[i].vec)
+
+    @staticmethod
+    def _saved_here(P, ids, loc, pending) -> bool:
+        rec = P.__dict__.get("_saved_for", {})
+        return all(l in rec and rec[l][0] == i and rec[l][1] is pending[i].vec for i, l in zip(ids, loc))
+
     def _fev(self, ids, pending):
         P, loc = self._local(ids)
         v = self._pack(P, loc, [pending[i].vec for i in ids])
         dd = self.targets[ids[0]] if self.max_batch == 1 else self._targets_cat()
         _, misfit, grad = P.gradient(v, dd, on_residual=True)
+        self._note_saving(P, ids, loc, pending)
         mis = misfit.cpu().tolist()
         return {i: (mis[l], grad[int(P.vec_off[l]):int(P.vec_off[l + 1])].clone()) for i, l in zip(ids, loc)}
 
@@ -527,12 +548,13 @@ class Executor:
         P, loc = self._local(ids)
         v = self._pack(P, loc, [pending[i].vec for i in ids])
         dv = self._pack(P, loc, [pending[i].dvec for i in ids])
-        if self.reuse_forward and P.saved_matches(v):
+        if self.reuse_forward and P.full_history and self._saved_here(P, ids, loc, pending):
             # every model's last saving forward (its fev or previous product)
-            # ran at this point: tangent from the saved history
+            # ran at this very vector: tangent from the saved history
             _, grad = P.gn_product_saved(v, dv, check=False)
         else:
             _, grad = P.gn_product(v, dv)
+            self._note_saving(P, ids, loc, pending)
         return {i: grad[int(P.vec_off[l]):int(P.vec_off[l + 1])].clone() for i, l in zip(ids, loc)}
 
     def _fanout_plan(self):

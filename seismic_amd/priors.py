@@ -38,6 +38,10 @@ class Prior:
     def from_basis(self, x: torch.Tensor) -> torch.Tensor:
         return x if self.basis is None else self.basis @ x
 
+    def is_identity(self) -> bool:
+        """True when to_basis / from_basis leave vectors unchanged."""
+        return self.basis is None
+
     def basis_T(self, g: torch.Tensor) -> torch.Tensor:
         return g if self.basis is None else self.basis.T @ g
 
@@ -341,6 +345,14 @@ class RestrictFlatAreas(Prior):
             return vec.clone()
         n = self.nz * self.nx
         return torch.cat([self._scatter(vec[:n]), vec[n:]])
+
+    def is_identity(self) -> bool:
+        return self.labels is None
+
+    @property
+    def basis_vectors(self) -> torch.Tensor:
+        """The dense basis of _adapt (:258-272) once adapted."""
+        return self.dense_basis() if self.labels is not None else Prior.basis_vectors.fget(self)
 
     def from_basis(self, x):
         if self.labels is None:

@@ -157,12 +157,20 @@ class Propagator:
     def set_active(self, mask: Sequence[bool]):
         mask = [bool(a) for a in mask]
         m = (ctypes.c_int32 * self.n_models)(*[1 if a else 0 for a in mask])
-        _lib.check(_lib.lib().seis_set_active(self._h, m), "seis_set_active")
+        _lib.check(_lib.lib().seis_set_active_async(self._h, m, _stream()), "seis_set_active_async")
         if all(mask):
             self._act_idx = None
-        else:
+            return
+        # parameter indices of the active models, kept per mask: a serving
+        # loop cycles through few masks, and each upload would wait for the stream
+        key = tuple(mask)
+        cache = self.__dict__.setdefault("_act_cache", {})
+        if key not in cache:
+            if len(cache) >= 64:
+                cache.pop(next(iter(cache)))
             idx = [np.arange(self.vec_off[i], self.vec_off[i + 1]) for i, a in enumerate(mask) if a]
-            self._act_idx = torch.from_numpy(np.concatenate(idx) if idx else np.zeros(0, np.int64)).to(self.device)
+            cache[key] = torch.from_numpy(np.concatenate(idx) if idx else np.zeros(0, np.int64)).to(self.device)
+        self._act_idx = cache[key]
 
     def _note_saved(self, v: torch.Tensor):
         """The active models' saved forward now belongs to v."""

@@ -51,10 +51,12 @@ def test_lbfgs_matches_oracle_inversion():
     """LBFGS (core/seis_numerics.py:307-448) branches on Wolfe tests that sit at
     near-equalities, so its iterates move under any perturbation: the oracle
     itself moves by several m/s when its seismograms carry 3e-7 relative noise.
-    The GPU run must reach the oracle's final misfit and stay within the
-    oracle's own trajectory spread (and within 1 m/s where that spread is
-    below it).  Step-for-step equality of the loop is pinned on the CPU
-    (tests/test_invert_host.py)."""
+    On this small grid the 1 m/s bar cannot be met by any float32 run (the
+    oracle's own spread under 3e-7 noise is ~26 m/s), so this test checks
+    the misfit reached and the spread only; the 1 m/s bar is asserted at the
+    reference size, where the oracle's spread is 7e-5 m/s
+    (test_lbfgs_flatvel_70x70_matches_oracle_fixture).  Step-for-step
+    equality of the loop is pinned on the CPU (tests/test_invert_host.py)."""
     g, og, true, target, start = _problem(1)
     it = 8
     v_ref, nfev = _oracle_lbfgs(g, og, target, start, it)
@@ -150,21 +152,26 @@ def test_gn_reuse_forward_identical():
 
 
 def test_cost_and_gradient_dropin_fd():
-    """core/seis_test.py:71-89 recipe on the GPU path: relative FD error < 1e-4
-    (float32 propagator, float64 cost)."""
+    """The cost-gradient FD check of core/seis_test.py:71-89 on the GPU
+    path, with the reference's 1e-4 bar.  The float32 propagator's cost
+    carries ~3e-7 relative noise, which the reference's one-sided difference
+    over a 1e-4 step cannot resolve; a central difference over 3 m/s in a
+    random direction keeps the truncation error below the bar too (measured
+    2.6e-6; one-sided differences stay near 1e-3 at every step size)."""
     g = Geometry(**KW)
     p = priors.RowTotalVariation(g.nz, g.nx, lam=1e-9)
     true = synth.make_model("flat", g.nz, g.nx, 3)
     from seismic_amd.propagator import default_propagator
     target = default_propagator(g).forward(torch.tensor(synth.to_vec(true))).double().reshape(-1, 1)
     base = torch.tensor(np.concatenate([np.linspace(2000, 3500, g.nz), [1900.0]]), device="cuda")
-    off = 1e-1 * torch.tensor(np.random.default_rng(0).normal(0, 1, p.N), device="cuda")
-    c2 = invert.cost_and_gradient(base + off, target, p, p.basis, False, geometry=g)[0]
+    off = 3.0 * torch.tensor(np.random.default_rng(0).normal(0, 1, p.N), device="cuda")
+    cp = invert.cost_and_gradient(base + off, target, p, p.basis, False, geometry=g)[0]
+    cm = invert.cost_and_gradient(base - off, target, p, p.basis, False, geometry=g)[0]
     c, gr, _, _ = invert.cost_and_gradient(base, target, p, p.basis, True, geometry=g)
     d = float((gr.reshape(-1) * off).sum())
-    rel = abs(d - float(c2 - c)) / abs(float(c2 - c))
-    print("cost FD rel", rel)
-    assert rel < 1e-2  # float32 forward differences of the cost limit this check
+    rel = abs(d - float(cp - cm) / 2) / abs(float(cp - cm) / 2)
+    print("cost FD rel (central, 3 m/s)", rel)
+    assert rel < 1e-4
 
 
 def test_bfgs_dropin_quadratic():
@@ -173,3 +180,31 @@ def test_bfgs_dropin_quadratic():
     f = lambda x: (0.5 * x @ A @ x - b @ x, A @ x - b)
     x = invert.bfgs(f, torch.zeros(6, dtype=torch.float64, device="cuda"), 50, 1e-10, 1e-12)
     torch.testing.assert_close(x, torch.linalg.solve(A, b), rtol=1e-6, atol=1e-9)
+
+
+def test_lbfgs_flatvel_70x70_matches_oracle_fixture():
+    """LBFGS where the 1 m/s bar is meaningful: the reference's FlatVel
+    inversion in test mode at full size (70x70, nbc 120, nt 999; row total
+    variation, 30 iterations, tolerance_change 1e-9) from a smoothed start.
+    The FP64 oracle's own result moves by only 5e-5 to 7e-5 m/s MAE when its
+    seismograms and gradients carry the float32 path's relative error
+    (3e-7 / 1e-6 noise, two seeds), so the GPU run must land within 1 m/s of
+    the oracle's inverted model and reach its misfit (tests/golden/ref70_lbfgs.npz
+    from tests/golden/make_golden.py)."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref70_lbfgs.npz"))
+    g = Geometry()
+    r = _req(g, d["target"], d["start"], priors.RowTotalVariation(g.nz, g.nx, lam=1e-8))
+    v = invert.invert([r], method="lbfgs", batched=False, max_iter=30, tolerance_change=1e-9)[0]
+    from seismic_amd.propagator import Propagator
+    P = Propagator([g], caps=1)
+    misfit = float(P.forward_misfit(v, torch.tensor(d["target"], device="cuda"))[0]) / g.n_data
+    P.close()
+    v = v.cpu().numpy()
+    mae = np.abs(v[:-1] - d["v"][:-1]).mean()
+    print(f"FlatVel 70x70 LBFGS: MAE vs oracle {mae:.4f} m/s (moved {np.abs(d['v'] - d['start'])[:-1].mean():.1f}), "
+          f"fevs {r.meta['stats']['fev']} vs {int(d['nfev'])}, misfit {misfit:.4e} vs {float(d['misfit']):.4e}, "
+          f"min_vel {v[-1]:.3f} vs {d['v'][-1]:.3f}", flush=True)
+    assert mae < 1.0
+    assert abs(v[-1] - d["v"][-1]) < 1.0
+    assert misfit < 1.05 * float(d["misfit"])

@@ -130,3 +130,25 @@ def test_served_chains_equal_batched():
     for rec, b in zip(recs, ref):
         assert not rec.failed and torch.equal(rec.result, b)
     assert max(n for _, n in srv.log) > 1          # requests shared launches
+
+
+def test_flatvel_chain_matches_oracle_chain_at_full_size():
+    """The classified chain at the reference size against the oracle's chain:
+    a flat-layered 70x70 model (nbc 120, nt 999) is dispatched to FlatVel by
+    the mirrored-shot KPI (core/seis_model.py:155-157) and runs the FlatVel
+    model in test mode (:15-27: row TV, 30 LBFGS iterations, rounding); the
+    oracle's chain is its LBFGS inversion of the same data from the same start
+    (tests/golden/ref70_lbfgs.npz), rounded (core/kaggle_support.py:472-474)."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref70_lbfgs.npz"))
+    g = Geometry()
+    req = invert.Request(geom=g, target=torch.tensor(d["target"], device="cuda"), prior=None,
+                         start=torch.tensor(d["start"], dtype=torch.float64, device="cuda"))
+    vecs, fams = schedule.infer([req], test_mode=True, batched=True)
+    assert fams == ["FlatVel"]
+    v = vecs[0].cpu().numpy()
+    ref = np.round(d["v"])
+    mae = np.abs(v[:-1] - ref[:-1]).mean()
+    same = float(np.mean(v[:-1] == ref[:-1]))
+    print(f"FlatVel chain 70x70: MAE vs oracle chain {mae:.4f} m/s, identical cells {same:.4f}", flush=True)
+    assert mae < 1.0 and same > 0.9 and abs(v[-1] - ref[-1]) <= 1.0
