@@ -41,10 +41,11 @@ L2_PEAK = 8 * 16 * 128 * 2.4e9
 # step for one shot): forward reads p[t-1], p[t], alpha, kappa and writes
 # p[t+1]; the fused tangent also reads dp[t-1], dp[t], dalpha, dkappa and
 # writes dp[t+1]; the Born pass reads dp[t-1], dp[t], alpha, kappa, dalpha,
-# dkappa and the saved pair (Lf(p[t]), q[t]) and writes dp[t+1]; the adjoint
-# reads lam[t+1], lam[t+2], alpha, kappa and the pair of p[t] and writes lam[t].
-# The saving forward also writes the pair (Lf(p[t]), q[t]) of every cell.
-ALG_BYTES = {"linesearch_forward": 20, "saving_forward": 28, "tangent": 40, "born": 36, "adjoint": 28}
+# dkappa and the saved q[t+1] and writes dp[t+1]; the adjoint reads lam[t+1],
+# lam[t+2], alpha, kappa and p[t], p[t+1] and writes lam[t] (the per-unit
+# figure of the adjoint + imaging pass; the kernel reads one saved q level per
+# step).  The saving forward also writes q[t+1] of every cell.
+ALG_BYTES = {"linesearch_forward": 20, "saving_forward": 24, "tangent": 40, "born": 32, "adjoint": 28}
 KERNEL = {"linesearch_forward": "k_fwd", "saving_forward": "k_fwd (history)", "tangent": "k_fwd (tangent)",
           "born": "k_born", "adjoint": "k_adj"}
 N_OUTER, N_CG, N_SCALES = 10, 20, 50

@@ -161,8 +161,9 @@ int seis_gn_product(seis_plan_t* plan, const float* vec, const float* dvec,
 /* Tangent and Gauss-Newton product at the linearization point of the last
  * history-saving forward (seis_gradient, seis_forward_save or
  * seis_gn_product) of each active model, without recomputing the forward:
- * that forward stored Lf(p[n]) and q[n] = p[n] - p[n-1] for every step, and
- * the Born step kernel reads them instead of advancing p.  Results are
+ * that forward stored q[n] = p[n] - p[n-1] for every step, and the Born step
+ * kernel reads it (recovering alpha*Lf(p[n]) from q[n] and q[n+1]) instead of
+ * advancing p.  Results are
  * bit-identical to seis_tangent's dseis and seis_gn_product's outputs at the
  * same vec.  vec must be the vector of that saving forward (the caller
  * tracks this, as core/seis_invert.py:84-90 keeps x_guess fixed over the

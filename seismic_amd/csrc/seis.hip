@@ -16,9 +16,13 @@
 // with the imaging terms reduced to two per-cell accumulators
 //   GA = sum lam[m]*Lf(p[m-1])         (alpha_adjoint + 2*c1*temp1_adjoint)
 //   GK = sum lam[m]*(p[m-2] - p[m-1])  (kappa_adjoint = -temp1_adjoint - temp2_adjoint)
-// A saving forward stores the pair (Lf(p[n]), q[n]) per cell and step; the
-// adjoint's imaging terms and the tangent's source terms dalpha*Lf(p) and
-// dkappa*q are pointwise in it, so both run one stencil per step.
+// A saving forward stores q[n] per cell and step.  The adjoint's imaging terms
+// and the tangent's source terms dalpha*Lf(p[n]) and dkappa*q[n] are
+// pointwise in q[n] and in alpha*Lf(p[n]), which the increment form gives
+// back from two consecutive levels: alpha*Lf(p[n]) = q[n+1] - (1-kappa)*q[n]
+// (minus the source term at the source cell), evaluated as one fused
+// multiply-add in every kernel that needs it, so the tangent and the adjoint
+// each run one stencil per step and read 4 bytes of history per cell-update.
 // and prep_run_adjoint (:361-377) + unpad_edge_padded_gradient
 // (core/seis_numerics.py:14-79) folded into one finishing kernel.
 //
@@ -53,8 +57,9 @@ constexpr int kKzPad = 64;       // rows of periodic apron on each side of a mod
 
 // Per-wavefield state regions in the float arena.
 enum Region {
-  R_HIST = 0,  // (Lf(p[n]), q[n]) float pairs: full history n = 0..nt (nt+1
-               // slots) or a recompute segment (Kc+1 slots); 2 floats per cell
+  R_HIST = 0,  // q[n] per cell: full history n = 0..nt+1 (nt+2 slots, slot 0 the
+               // zero initial state), or a recompute segment n = lo..lo+Kc+1
+               // (Kc+2 slots; slot 0 unused); one float per cell
   R_PP,        // forward p ping-pong (2 slots)
   R_Q,         // q = p[n] - p[n-1] ping-pong (2 slots)
   R_DP,        // tangent dp ping-pong (2 slots)
@@ -216,7 +221,7 @@ struct FwdArgs {
   int n0, K;             // advance state n0 -> n0+K; step n produces p[n+1] with s[n-1]
   int Kt;                // steps the tile list was built for (halo 2*Kt); K <= Kt
   int p_in_slot, p_out_slot;  // R_PP slots holding p[n0] and receiving p[n0+K]
-  int hist_slot0;        // >= 0: (Lf(p[n0+s]), q[n0+s]) -> R_HIST slot hist_slot0 + s
+  int hist_slot0;        // >= 0: q[n0+s+1] (the output of step n0+s) -> R_HIST slot hist_slot0 + s
   int q_in_slot, q_out_slot;
   int dp_in_slot, dp_out_slot, dq_in_slot, dq_out_slot;
   int ck_slot;           // >= 0: also store the final (p, q) as checkpoint ck_slot
@@ -247,8 +252,11 @@ __device__ __forceinline__ int rcv_index(int gx, const ModelDev& m) {
 // stored after it.  TAN (two stencils per cell) skips rows outside the band
 // instead, which keeps its register count down.  No global load is issued inside the step loop: the
 // launch's wavelet samples are staged in LDS with the window.  With HIST (a
-// saving forward) the kernel also stores, per interior cell and step, the pair
-// (Lf(p[n]), q[n]) that the tangent and adjoint recursions consume.
+// saving forward) the kernel also stores, per interior cell and step, the q
+// it produced, which the tangent and adjoint recursions consume.  With TAN the
+// source term dalpha*Lf(p[n]) is (dalpha/alpha) times alpha*Lf(p[n]) recovered
+// from q[n] and q[n+1], the expression the Born kernel evaluates on the saved
+// history, so the two give identical bits.
 template <int RX, int RZ, int NT, bool TAN, bool STRIP, bool HIST>
 __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
     const int64_t Pn = (int64_t)NZ * NX;
     const float* pin = a.arena + w.off[R_PP] + a.p_in_slot * Pn;
     const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
-    float2* hist = HIST ? reinterpret_cast<float2*>(a.arena + w.off[R_HIST]) + a.hist_slot0 * Pn : nullptr;
+    float* hist = HIST ? a.arena + w.off[R_HIST] + a.hist_slot0 * Pn : nullptr;
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float amp = a.wparam[te.w].amp;
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       const float src = amp * sv;
       const float srcd = amp_d * sv;
       const int lo = 2 * (s + 1);
-      float2* hs = HIST ? hist + s * Pn : nullptr;
+      float* hs = HIST ? hist + s * Pn : nullptr;
       if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {  // the strip meets the valid band (wave-uniform)
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
@@ -368,20 +376,24 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
             float L;
             if constexpr (STRIP) L = lap_col<PW>(Prd, col, j, r, cc);
             else L = lap_lds<PW>(Prd, pv[i], r, cc);
-            if constexpr (HIST) {
-              if ((introw >> j) & 1u) {
-                if ((intmask >> i) & 1u) hs[gint0 + r * NX + cc] = make_float2(L, q[i]);
-              }
-            }
             float qn = fmaf(k1[i], q[i], al[i] * L);
             if ((srcrow >> j) & 1u) qn = ((srcmask >> i) & 1u) ? qn + src : qn;
+            if constexpr (HIST) {
+              if ((introw >> j) & 1u) {
+                if ((intmask >> i) & 1u) hs[gint0 + r * NX + cc] = qn;
+              }
+            }
             if constexpr (TAN) {
-              // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q
+              // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q, with
+              // dalpha Lf(p) = (dalpha/alpha) (alpha Lf(p)) and alpha Lf(p)
+              // recovered from q and q' as the Born kernel recovers it
               float Ld;
               if constexpr (STRIP) Ld = lap_col<PW>(Drd, dcol, j, r, cc);
               else Ld = lap_lds<PW>(Drd, dpv[i], r, cc);
+              float aL = fmaf(-k1[i], q[i], qn);
+              if ((srcrow >> j) & 1u) aL = ((srcmask >> i) & 1u) ? aL - src : aL;
               float dqn = fmaf(k1[i], dq[i], al[i] * Ld);
-              dqn = fmaf(dal[i], L, dqn);
+              dqn = fmaf(dal[i], aL, dqn);
               dqn = fmaf(-dk[i], q[i], dqn);
               if ((srcrow >> j) & 1u) dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
               const float dp1 = dpv[i] + dqn;
@@ -684,11 +696,12 @@ struct BornArgs {
 };
 
 // K steps of the Born recursion (update_p_diff :470-509) with the forward
-// wavefield taken from the saved history: (Lf(p[n]), q[n]) of every band cell
-// is read from HBM instead of advancing p, which leaves one stencil per step.
-// The next step's pairs are requested as soon as the current step's last use
-// of the registers they land in is issued, so the barrier covers part of
-// their latency.  Same window layout and double-buffering as k_fwd, and its
+// wavefield taken from the saved history: q[n+1] of every band cell is read
+// from HBM (q[n] is the previous step's) instead of advancing p, and
+// alpha*Lf(p[n]) = q[n+1] - (1-kappa) q[n] (minus the source term at the
+// source cell), which leaves one stencil per step.  The next step's q is
+// requested as soon as the current step's last use of the registers it lands
+// in is issued, so the barrier covers part of its latency.  Same window layout and double-buffering as k_fwd, and its
 // fused multiply-add sequence, so dseis and (dp, dq) are bit-identical to the
 // fused forward + tangent pass (whatever the row ownership of either).
 template <int RX, int RZ, int NT, bool STRIP>
@@ -713,16 +726,17 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
     const int64_t Pn = (int64_t)NZ * NX;
     const float* dpin = a.arena + w.off[R_DP] + a.dp_in_slot * Pn;
     const float* dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
-    const float2* hist = reinterpret_cast<const float2*>(a.arena + w.off[R_HIST]);
+    const float* hist = a.arena + w.off[R_HIST];   // q[n] at slot n
     const float* alpha = a.alpha + m.cell_off;
     const float* dalpha = a.dalpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float dkscale = a.params[w.model].dkscale;
+    const float amp = a.wparam[te.w].amp;
     const float amp_d = a.wparam[te.w].amp_d;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;
     float dpv[NC], dq[NC], al[NC], dal[NC], jj[NC];
-    float2 hc[NC];
+    float qo[NC], qn[NC];   // q[n], q[n+1] of the step
     int gw[NC];
     if (threadIdx.x < KMAX) {
       const int it = a.n0 + (int)threadIdx.x - 1;
@@ -733,7 +747,7 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
     int rcol[CPR];
 #pragma unroll
     for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
-    const float2* h0 = hist + a.n0 * Pn;
+    const float* h0 = hist + (int64_t)a.n0 * Pn;
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
       const int r = own_row<STRIP, NW, RPW>(wv, j);
@@ -747,6 +761,8 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
         const int gx = wrapi(ox + cc, NX);
         const int g = gz * NX + gx;
         gw[i] = g;
+        qo[i] = h0[g];
+        qn[i] = h0[Pn + g];
         dpv[i] = dpin[g];
         D0[r + 2][cc + 2] = dpv[i];
         dq[i] = dqin[g];
@@ -757,7 +773,6 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
         if (g == w.src_cell) srcmask |= 1u << i;
         if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
           intmask |= 1u << i;
-        hc[i] = h0[g];
       }
     }
     __syncthreads();
@@ -769,6 +784,7 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
       float (*const Dwr)[PW] = CUR ? D0 : D1;
       const int it = a.n0 + s - 1;
       const bool live = it >= 0 && it < a.nt;
+      const float src = amp * SW[s];
       const float srcd = amp_d * SW[s];
       const int lo = 2 * (s + 1);
       if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {
@@ -786,9 +802,11 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
             else Ld = lap_lds<PW>(Drd, dpv[i], r, cc);
             const float k1 = 1.0f - kscale * jj[i];
             const float dk = dkscale * jj[i];
+            float aL = fmaf(-k1, qo[i], qn[i]);
+            if ((srcrow >> j) & 1u) aL = ((srcmask >> i) & 1u) ? aL - src : aL;
             float dqn = fmaf(k1, dq[i], al[i] * Ld);
-            dqn = fmaf(dal[i], hc[i].x, dqn);
-            dqn = fmaf(-dk, hc[i].y, dqn);
+            dqn = fmaf(dal[i], aL, dqn);
+            dqn = fmaf(-dk, qo[i], dqn);
             if ((srcrow >> j) & 1u) dqn = ((srcmask >> i) & 1u) ? dqn + srcd : dqn;
             const float dp1 = dpv[i] + dqn;
             Dwr[r + 2][cc + 2] = dp1;
@@ -809,10 +827,13 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
           }
         }
       }
-      if (s + 1 < a.K) {  // pairs of the next step (every owned cell, back to back)
-        const float2* hx = hist + (int64_t)(a.n0 + s + 1) * Pn;
+      if (s + 1 < a.K) {  // q of the next step (every owned cell, back to back)
+        const float* hx = hist + (int64_t)(a.n0 + s + 2) * Pn;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) hc[i] = hx[gw[i]];
+        for (int i = 0; i < NC; ++i) {
+          qo[i] = qn[i];
+          qn[i] = hx[gw[i]];
+        }
       }
       __syncthreads();
     };
@@ -839,10 +860,11 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
 // KL steps of the Born recursion (update_p_diff :470-509) on the saved
 // history, streamed like k_fws: a column strip of NT padded columns (one per
 // thread) marched down the rows of a segment; pipeline step t takes in
-// window row t of dp[n0], dq[n0] and the row's coefficients (alpha, dalpha,
-// 1 - kappa, dkappa) as level 0, and level l = 1..KL computes window row
-// t - 2l of (dp, dq)[n0+l] from level l-1 and the pair (Lf(p[n]), q[n]) of
-// n = n0+l-1 of that row, requested SPH steps ahead.  The coefficients of a
+// window row t of dp[n0], dq[n0], q[n0] and the row's coefficients (alpha,
+// dalpha/alpha, 1 - kappa, dkappa) as level 0, and level l = 1..KL computes
+// window row t - 2l of (dp, dq)[n0+l] from level l-1, from q[n0+l] of that
+// row (requested SPH steps ahead) and from q[n0+l-1] of that row, which
+// level l-1 used two steps earlier and left in a three-slot register ring.  The coefficients of a
 // row sit in a private LDS ring of SUNB slots, written after the step's
 // levels have read the slot's previous row (the last level reads row t-2KL).
 // Same stencil order and fused multiply-add sequence as k_born, so dseis and
@@ -873,20 +895,21 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
     const float* dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
     float* dpout = a.arena + w.off[R_DP] + a.dp_out_slot * Pn;
     float* dqout = a.arena + w.off[R_DQ] + a.dq_out_slot * Pn;
-    const float* hist = a.arena + w.off[R_HIST];   // float2 pairs, slot n at 2 * n * Pn floats
+    const float* hist = a.arena + w.off[R_HIST];   // q[n] at slot n
     float* dseis = a.dseis + w.seis_off;
     const float* alpha = a.alpha + m.cell_off;
     const float* dalpha = a.dalpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float dkscale = a.params[w.model].dkscale;
+    const float amp = a.wparam[te.w].amp;
     const float amp_d = a.wparam[te.w].amp_d;
     const int n_rcv = m.n_rcv, rcv_row = m.rcv_row;
     const int IZ = te.pad;
     const int U = IZ + 2 * H;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const unsigned gx = wrapi(ox + (int)c, NX);
-    unsigned gx4 = gx * 4u, gx8 = gx * 8u;
-    asm volatile("" : "+v"(gx4), "+v"(gx8));
+    unsigned gx4 = gx * 4u;
+    asm volatile("" : "+v"(gx4));
     const int64_t rowb = (int64_t)NX * 4;
     const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
     const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
@@ -898,23 +921,29 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
     const int su0 = wrapi(srow - oz, NZ), su1 = su0 + NZ;
     const int ruz = rcv_row - te.z0;
     const int rui = ruz >= 0 && ruz < IZ ? ruz + H : -(1 << 20);
-    float srcv[KL], rec[KL];
+    float srcv[KL], srcp[KL], rec[KL];
 #pragma unroll
     for (int l = 1; l <= KL; ++l) {
       const int it = n0 + l - 2;
-      srcv[l - 1] = amp_d * ((it >= 0 && it < nt) ? a.wavelet[it] : 0.f);
+      const float wl = (it >= 0 && it < nt) ? a.wavelet[it] : 0.f;
+      srcv[l - 1] = amp_d * wl;
+      srcp[l - 1] = amp * wl;   // the forward's source term in q[n0+l]
       rec[l - 1] = 0.f;
     }
-    float pr[KL][6], qr[KL][3], pfp[SPB], pfq[SPB], pfa[SPB], pfd[SPB];
-    float2 hpf[KL][SPH];
+    float pr[KL][6], qr[KL][3], pfp[SPB], pfq[SPB], pfa[SPB], pfd[SPB], pfh[SPB];
+    float hq[KL][SPH];   // q[n0+l] of the row level l meets SPH steps ahead
+    float qps[KL][3];    // q[n0+l] of the row level l met at step t, at [t % 3]
 #pragma unroll
     for (int l = 0; l < KL; ++l) {
 #pragma unroll
       for (int s = 0; s < 6; ++s) pr[l][s] = 0.f;
 #pragma unroll
-      for (int s = 0; s < 3; ++s) qr[l][s] = 0.f;
+      for (int s = 0; s < 3; ++s) {
+        qr[l][s] = 0.f;
+        qps[l][s] = 0.f;
+      }
 #pragma unroll
-      for (int s = 0; s < SPH; ++s) hpf[l][s] = make_float2(0.f, 0.f);
+      for (int s = 0; s < SPH; ++s) hq[l][s] = 0.f;
     }
     // coefficients of window row u into the ring slot u % SUNB
     auto put_cf = [&](int slot, int u, float al, float dal) {
@@ -922,10 +951,9 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
       const float jj = isx ? jjx : (float)(jz * jz);
       CF[slot][c] = make_float4(al, dal, 1.0f - kscale * jj, dkscale * jj);
     };
-    auto hist_row = [&](int l, int u) {  // pair of level l, window row u (wrapped padded row)
-      const int64_t ro = (int64_t)(n0 + l - 1) * Pn * 8 + (int64_t)wrapi(oz + u, NZ) * rowb * 2;
-      const gfloat* hp = lane(rowp(hist, ro), gx8);
-      return make_float2(hp[0], hp[1]);
+    auto hist_row = [&](int n, int u) {  // q[n], window row u (wrapped padded row)
+      const int64_t ro = (int64_t)n * Pn * 4 + (int64_t)wrapi(oz + u, NZ) * rowb;
+      return *lane(rowp(hist, ro), gx4);
     };
 #pragma unroll
     for (int d = 0; d < SPB; ++d) {
@@ -934,12 +962,13 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
       pfq[d] = *lane(rowp(dqin, ro), gx4);
       pfa[d] = *lane(rowp(alpha, ro), gx4);
       pfd[d] = *lane(rowp(dalpha, ro), gx4);
+      pfh[d] = hist_row(n0, d);
     }
     // history of the first steps: level l meets window row t - 2l at step t
 #pragma unroll
     for (int d = 0; d < SPH; ++d) {
 #pragma unroll
-      for (int l = 1; l <= KL; ++l) hpf[l - 1][d] = hist_row(l, d - 2 * l);
+      for (int l = 1; l <= KL; ++l) hq[l - 1][d] = hist_row(n0 + l, d - 2 * l);
     }
     int gzn = wrapi(oz + SPB, NZ);      // padded row of the next level-0 request
     __syncthreads();
@@ -955,6 +984,7 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
         al0 = pfa[i % SPB];
         dal0 = pfd[i % SPB];
       }
+      const float qin0 = pfh[i % SPB];   // q[n0] of row t: level 1 reads it two steps on
       static_for<1, KL + 1>([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         const int u = t - 2 * l;
@@ -970,10 +1000,15 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
         t2 = t2 + (pr[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
         const float Ld = fmaf(kC2, t1, kC3 * t2);
         const float4 cf = CF[sr][c];
-        const float2 hc = hpf[l - 1][i % SPH];
+        const float qnh = hq[l - 1][i % SPH];                  // q[n0+l] of row u
+        const float qoh = qps[l - 1][(i - 2 + B) % 3];         // q[n0+l-1] of row u
+        float aL = fmaf(-cf.z, qoh, qnh);
+        if constexpr (SPECIAL) {
+          if (u == su0 || u == su1) aL = srcc ? aL - srcp[l - 1] : aL;
+        }
         float dqn = fmaf(cf.z, qr[l - 1][s3], cf.x * Ld);
-        dqn = fmaf(cf.y, hc.x, dqn);
-        dqn = fmaf(-cf.w, hc.y, dqn);
+        dqn = fmaf(cf.y, aL, dqn);
+        dqn = fmaf(-cf.w, qoh, dqn);
         if constexpr (SPECIAL) {
           if (u == su0 || u == su1) dqn = srcc ? dqn + srcv[l - 1] : dqn;
         }
@@ -994,9 +1029,11 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
             *lane(oq, gx4) = dqn;
           }
         }
-        // this level's pair for its row SPH steps ahead
-        hpf[l - 1][i % SPH] = hist_row(l, u + SPH);
+        if constexpr (l < KL) qps[l][i % 3] = qnh;      // for level l+1, two steps on
+        // this level's q for its row SPH steps ahead
+        hq[l - 1][i % SPH] = hist_row(n0 + l, u + SPH);
       });
+      qps[0][i % 3] = qin0;
       put_cf(i % SUNB, t, al0, dal0);
       {  // request of window row t + SPB
         const int64_t ro = (int64_t)gzn * rowb;
@@ -1004,6 +1041,7 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
         pfq[i % SPB] = *lane(rowp(dqin, ro), gx4);
         pfa[i % SPB] = *lane(rowp(alpha, ro), gx4);
         pfd[i % SPB] = *lane(rowp(dalpha, ro), gx4);
+        pfh[i % SPB] = *lane(rowp(hist, (int64_t)n0 * Pn * 4 + ro), gx4);
         gzn = gzn + 1 == NZ ? 0 : gzn + 1;
       }
       __syncthreads();
@@ -1042,7 +1080,7 @@ struct AdjArgs {
   int M0, K;            // produces lam[M0-1] .. lam[M0-K]
   int Kt;               // steps the tile list was built for (halo 2*Kt); K <= Kt
   int lam_in_slot, lam_out_slot, mu_in_slot, mu_out_slot;
-  int h_base;           // the forward's (Lf(p[n]), q[n]) sits at R_HIST slot n - h_base
+  int h_base;           // the forward's q[n] sits at R_HIST slot n - h_base
 };
 
 // K reverse steps of update_p_adjoint (:517-579) in gather form for every
@@ -1051,10 +1089,13 @@ struct AdjArgs {
 // alpha, (1-kappa) and the imaging accumulators of owned cells stay in
 // registers; alpha*lam is staged in a double-buffered LDS window for the
 // stencil (row ownership as k_fwd, one barrier per step).  The forward
-// enters only through the saved pairs (Lf(p[m-1]), q[m-1]) of interior cells;
-// the next step's pairs are requested right after the current step's last
-// use.  The accumulators see the same fused multiply-add sequence as a
-// per-step read-modify-write, so results do not depend on K.
+// enters only through the saved q of interior cells: step m uses q[m-1] and
+// q[m] (the previous step's), alpha*Lf(p[m-1]) = q[m] - (1-kappa) q[m-1]
+// (minus the source term at the source cell), and the alpha accumulator
+// collects lam * alpha*Lf(p), divided by alpha when the gradient is folded;
+// the next step's q is requested right after the current step's last use.
+// The accumulators see the same fused multiply-add sequence as a per-step
+// read-modify-write, so results do not depend on K.
 template <int RX, int RZ, int NT, bool STRIP>
 __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   constexpr int NW = NT / 64, CPR = RX / 64, RPW = RZ / NW, NC = CPR * RPW;
@@ -1080,14 +1121,15 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     const float* min_ = a.arena + w.off[R_MU] + a.mu_in_slot * Pn;
     float* GA = a.arena + w.off[R_GA];
     float* GK = a.arena + w.off[R_GK];
-    const float2* hist = reinterpret_cast<const float2*>(a.arena + w.off[R_HIST]);
+    const float* hist = a.arena + w.off[R_HIST];   // q[n] at slot n - h_base
     const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
+    const float amp = a.wparam[te.w].amp;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;
     float lam[NC], mu[NC], al[NC], k1[NC], ga[NC], gk[NC];
-    float2 hc[NC];
-    int hidx[NC];  // pair index of an interior cell; 0 (a dummy read) elsewhere
+    float qlo[NC], qhi[NC];   // q[m-1], q[m] of the step (interior cells)
+    int hidx[NC];  // history index of an interior cell; 0 (a dummy read) elsewhere
     unsigned srcmask = 0, rcvmask = 0, intmask = 0;
     int rcol[CPR];
 #pragma unroll
@@ -1107,7 +1149,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       }
       INJ[s][cc] = v;
     }
-    const float2* h0 = hist + (int64_t)(a.M0 - 2 - a.h_base) * Pn;  // pairs of n = M0-2
+    const float* h0 = hist + (int64_t)(a.M0 - 2 - a.h_base) * Pn;  // q[M0-2]; q[M0-1] one slot on
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
       const int r = own_row<STRIP, NW, RPW>(wv, j);
@@ -1131,14 +1173,16 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         const bool inner = r >= H && r < RZ - H && zz < NZ && cc >= H && cc < RX - H && xx < NX;
         ga[i] = 0.f;
         gk[i] = 0.f;
-        hc[i] = make_float2(0.f, 0.f);
+        qlo[i] = 0.f;
+        qhi[i] = 0.f;
         hidx[i] = 0;
         if (inner) {
           intmask |= 1u << i;
           const int gi = zz * NX + xx;
           ga[i] = GA[gi];
           gk[i] = GK[gi];
-          hc[i] = h0[gi];
+          qlo[i] = h0[gi];
+          qhi[i] = h0[Pn + gi];
           hidx[i] = gi;
         }
       }
@@ -1156,6 +1200,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       const int mm = a.M0 - 1 - s;
       const int lo = 2 * (s + 1);
       const float sw = SW[s];
+      const float srcf = amp * sw;   // the forward's source term in q[mm]
       if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
@@ -1188,24 +1233,29 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         }
       }
       // imaging on the interior (always inside the band): lam now holds lam[m]
-      // GA += lam[m] Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
+      // GA += lam[m] alpha Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
       // every cell accumulates; only interior cells' sums are stored
 #pragma unroll
       for (int i = 0; i < NC; ++i) {
-        ga[i] = fmaf(lam[i], hc[i].x, ga[i]);
-        gk[i] = fmaf(lam[i], -hc[i].y, gk[i]);
+        float aL = fmaf(-k1[i], qlo[i], qhi[i]);
+        aL = ((srcmask & intmask) >> i & 1u) ? aL - srcf : aL;
+        ga[i] = fmaf(lam[i], aL, ga[i]);
+        gk[i] = fmaf(lam[i], -qlo[i], gk[i]);
       }
       if (src_here) {  // only the thread owning the source cell
 #pragma unroll
         for (int i = 0; i < NC; ++i)
           if ((srcmask & intmask) >> i & 1u) ssum += (double)lam[i] * (double)sw;
       }
-      if (s + 1 < a.K) {  // pairs of n = mm - 2 for the next step
+      if (s + 1 < a.K) {  // q[mm - 2] for the next step
         // issued unconditionally (cells outside the interior all read one
         // dummy address) so that the loads go out back to back
-        const float2* hx = hist + (int64_t)(mm - 2 - a.h_base) * Pn;
+        const float* hx = hist + (int64_t)(mm - 2 - a.h_base) * Pn;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) hc[i] = hx[hidx[i]];
+        for (int i = 0; i < NC; ++i) {
+          qhi[i] = qlo[i];
+          qlo[i] = hx[hidx[i]];
+        }
       }
       __syncthreads();
     };
@@ -1295,7 +1345,9 @@ __global__ __launch_bounds__(NTHR) void k_prep(const ModelDev* models, const flo
     const int ix = min(max(x - m.nbc, 0), m.nx - 1);
     const double vv = v[iz * m.nx + ix];
     alpha[m.cell_off + i] = (float)(vv * vv * dtdx2);
-    if (dv) dalpha[m.cell_off + i] = (float)(2.0 * vv * (double)dv[iz * m.nx + ix] * dtdx2);
+    // the tangent kernels take dalpha / alpha = 2 dv / v (prep_run_diff,
+    // :352-355), the factor that turns the recovered alpha Lf(p) into dalpha Lf(p)
+    if (dv) dalpha[m.cell_off + i] = vv != 0.0 ? (float)(2.0 * (double)dv[iz * m.nx + ix] / vv) : 0.f;
   }
 }
 
@@ -1361,7 +1413,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // -> column nx-1), summing the alpha accumulator over shots in float64.  Also
 // the row's share of sum(j^2 * kappa accumulator) for d/d min_vel (:366-370).
 __global__ __launch_bounds__(NTHR) void k_fold_rows(const ModelDev* models, const WaveDev* waves,
-                                                     const float* arena, double* fold) {
+                                                     const float* arena, const float* alpha, double* fold) {
   __shared__ double red[NTHR];
   const ModelDev& m = models[blockIdx.y];
   if (!m.active || (int)blockIdx.x >= m.NZ) return;
@@ -1377,6 +1429,9 @@ __global__ __launch_bounds__(NTHR) void k_fold_rows(const ModelDev* models, cons
       a += (double)arena[w.off[R_GA] + g];
       k += (double)arena[w.off[R_GK] + g];
     }
+    // the adjoint accumulated lam * alpha Lf(p): the alpha adjoint is that over alpha
+    const double al = (double)alpha[m.cell_off + g];
+    a = al != 0.0 ? a / al : 0.0;
     const int j = damp_j(Z, X, m.nz, m.nx, m.nbc);
     sk += (double)(j * j) * k;
     const int ix = X - m.nbc;
@@ -1905,7 +1960,7 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
         w.off[r] = L.arena;
         L.arena += align64(P) * slots;
       };
-      region(R_HIST, 2 * (L.fwd_hist ? nt + 1 : (adj ? L.Kc + 1 : 0)));
+      region(R_HIST, L.fwd_hist ? nt + 2 : (adj ? L.Kc + 2 : 0));
       region(R_PP, 2);
       region(R_Q, 2);
       region(R_DP, tan ? 2 : 0);
@@ -2278,7 +2333,7 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
     f.K = K;
     f.p_in_slot = f.q_in_slot = f.dp_in_slot = f.dq_in_slot = li & 1;
     f.p_out_slot = f.q_out_slot = f.dp_out_slot = f.dq_out_slot = (li + 1) & 1;
-    f.hist_slot0 = hist ? n : -1;
+    f.hist_slot0 = hist ? n + 1 : -1;   // step n stores q[n+1] at slot n+1
     f.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
     if (tail) launch_fwd(cr, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
     else launch_fwd(c, tan, p->tl[kind].grid, s, f);
@@ -2379,7 +2434,7 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
       f.K = K;
       f.p_in_slot = f.q_in_slot = fi & 1;
       f.p_out_slot = f.q_out_slot = (fi + 1) & 1;
-      f.hist_slot0 = n - lo;
+      f.hist_slot0 = n + 1 - lo;        // q[n+1] at segment slot n+1-lo
       launch_fwd(cf, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
       n += K;
       ++fi;
@@ -2437,7 +2492,7 @@ int run_prep(seis_plan* p, hipStream_t s, const float* vec, const float* dvec) {
 int finish_grad(seis_plan* p, hipStream_t s, const float* vec, double* grad) {
   const Layout& L = p->L;
   hipLaunchKernelGGL(k_fold_rows, dim3(L.max_NZ, p->n_models), dim3(NTHR), 0, s, p->d_models,
-                     p->d_waves, p->d_arena, p->d_fold);
+                     p->d_waves, p->d_arena, p->d_alpha, p->d_fold);
   hipLaunchKernelGGL(k_fold_cols, dim3((L.max_cells_phys + NTHR - 1) / NTHR, p->n_models),
                      dim3(NTHR), 0, s, p->d_models, p->d_waves, p->d_wparam, p->d_fold, vec, grad);
   HIP_TRY(hipGetLastError());
@@ -2585,6 +2640,10 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
   if (e == hipSuccess) e = hipMemcpy(p->d_wavelet, wl.data(), sizeof(float) * p->nt, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->d_wparam, 0, sizeof(WaveParam) * p->n_waves);
   if (e == hipSuccess) e = hipMemset(p->d_params, 0, sizeof(ParamDev) * n_models);
+  // full history: slot 0 is q[0], the zero initial state, which no pass writes
+  if (L.fwd_hist)
+    for (const WaveDev& w : L.waves)
+      if (e == hipSuccess) e = hipMemset(p->d_arena + w.off[R_HIST], 0, sizeof(float) * (size_t)w.NZ * w.NX);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
   if (e != hipSuccess) {
     fail(SEIS_EHIP, std::string("plan setup: ") + hipGetErrorString(e));

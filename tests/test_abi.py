@@ -34,8 +34,8 @@ def test_plan_bytes_reference_geometry():
     full = plan_bytes([g], ckpt_every=0)
     ck = plan_bytes([g], ckpt_every=50)
     P = 310 * 310 * 4
-    # full history: 5 shots x (nt+1) steps x (Lf(p), q) pairs of padded fields
-    assert full >= 5 * 1000 * 2 * P
+    # full history: 5 shots x (nt+2) levels of q on the padded grid
+    assert 5 * 1001 * P <= full < 5 * 1001 * P + 10 * 2 ** 20 + 5 * 12 * P
     assert ck < full / 5
     fwd_only = plan_bytes([g], caps=_lib.CAP_FORWARD)
     assert fwd_only < 5 * 4 * P + 10 * 2 ** 20
