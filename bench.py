@@ -361,12 +361,15 @@ def serving_run(n_req, rate, device, seed=100, sweep=(50.0, 200.0, 400.0), sweep
         srv.log.clear()
         reqs = make_reqs(n)
         arrivals = serve.poisson_arrivals(n, r, sd)
+        srv.start_timing()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         recs = srv.run(reqs, arrivals)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        busy = srv.gpu_busy_s()
         return {"requests": n, "rate_req_per_s": r, "inversions_per_s": n / wall, "wall_s": wall,
+                "gpu_busy_s": busy, "host_time_share": max(0.0, 1.0 - busy / wall),
                 "ckpt_every": srv.ckpt_every,
                 "latency_s": serve.percentiles([x.latency for x in recs]),
                 "queue_wait_s": serve.percentiles([x.admitted - x.arrival for x in recs]),
@@ -556,14 +559,17 @@ def served_schedule_run(device, n_each=8, slots=12, rate=50.0):
     arrivals = serve.poisson_arrivals(len(reqs), rate, seed=7)
     srv = serve.InversionServer([g] * slots, hbm_budget=200 << 30, ckpt_every=None, history_size=100,
                                 machine=schedule.serving_machine(test_mode=True), device=device)
+    srv.start_timing()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     recs = srv.run(reqs, arrivals)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    busy = srv.gpu_busy_s()
     fams = [r.stats.get("family") for r in recs]
     lat = [r.latency for r in recs]
     return {"requests": len(reqs), "slots": slots, "rate_per_s": rate, "wall_s": wall,
+            "gpu_busy_s": busy, "host_time_share": max(0.0, 1.0 - busy / wall),
             "inversions_per_s": len(reqs) / wall, "latency_s": serve.percentiles(lat),
             "families": {f: fams.count(f) for f in sorted(set(fams))},
             "max_requests_per_launch": max(n for _, n in srv.log), "failed": sum(r.failed for r in recs),

@@ -105,6 +105,32 @@ class InversionServer:
     def footprint(self, req: Request) -> int:
         return self._fp[req.geom]
 
+    def _execute(self, pending: dict) -> dict:
+        """One loop iteration's batched launches; with ``timing`` set, HIP
+        events on the launch stream bracket them (see ``gpu_busy_s``)."""
+        if not getattr(self, "timing", False):
+            return self.ex.execute(pending)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = self.ex.execute(pending)
+        e1.record()
+        self._spans.append((e0, e1))
+        return out
+
+    def start_timing(self):
+        """Record the device time of every loop iteration from now on."""
+        self.timing = True
+        self._spans = []
+
+    def gpu_busy_s(self) -> float:
+        """Device time spanned by the recorded iterations' launches (from the
+        first to the last operation each iteration enqueued, so it includes
+        gaps the host leaves between launches inside one iteration); the rest
+        of the wall time is host work between iterations (machine steps,
+        scalar reads, admission)."""
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in getattr(self, "_spans", [])) / 1e3
+
     def _check_admissible(self, req: Request):
         """Refuse a request no state of the server could ever admit: no slot
         of its size, or a footprint above the whole budget."""
@@ -182,7 +208,7 @@ class InversionServer:
                     time.sleep(max(0.0, queue[0][1] - now()))
                 continue
             pending = {s: v[3] for s, v in active.items()}
-            results = self.ex.execute(pending)
+            results = self._execute(pending)
             self.log.append((now(), len(pending)))
             for s, res in results.items():
                 req, idx, m, _ = active[s]
@@ -231,7 +257,7 @@ class InversionServer:
                 if nxt is not None:   # admissible, yet refused by an idle server
                     raise RuntimeError("a request could not be admitted on an idle server")
                 break
-            results = self.ex.execute({s: v[3] for s, v in active.items()})
+            results = self._execute({s: v[3] for s, v in active.items()})
             self.log.append((now(), len(results)))
             for s, res in results.items():
                 req, rec, m, _ = active[s]
