@@ -5,7 +5,7 @@ Usage (under rocprofv3):
 
 Passes, with the tuned configs the inversion executor uses:
     linesearch_forward  forward of the line-search plan (batch x fanout models)
-    saving_forward      forward that stores the history pairs (batch plan)
+    saving_forward      forward that stores the q history (batch plan)
     born                tangent on the saved forward (batch plan)
     adjoint             reverse pass with imaging (batch plan)
     tangent             fused forward + tangent (batch plan; outside the GN loop)

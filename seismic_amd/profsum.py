@@ -43,7 +43,8 @@ def trace(path):
               f"{sum(v)/1e6:.2f} | {sum(v)/total:.1%} |")
 
 
-ALG = {"linesearch_forward": 20, "saving_forward": 28, "tangent": 40, "born": 36, "adjoint": 28}
+# algorithmic bytes per cell-update of each pass (the same table as bench.py)
+ALG = {"linesearch_forward": 20, "saving_forward": 24, "tangent": 40, "born": 32, "adjoint": 28}
 
 
 def _pattern(name, e):
