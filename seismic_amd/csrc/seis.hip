@@ -498,18 +498,28 @@ __device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float ksca
 // fused multiply-adds as k_fwd, so wavefields and seismograms are
 // bit-identical to it.  Launches always advance KL steps (a shorter last
 // launch runs k_fwd).
-// Row pointer base + ro (bytes), kept in scalar registers and opaque to the
-// compiler, and element of a row at a 32-bit lane byte offset: every access
-// of the streamed kernel is then one scalar-based address (saddr form)
-// instead of a hoisted 64-bit per-lane pointer plus a 64-bit add per step.
-typedef __attribute__((address_space(1))) float gfloat;   // global memory
-__device__ __forceinline__ gfloat* rowp(const float* base, int64_t ro) {
-  uint64_t v = reinterpret_cast<uint64_t>(base) + (uint64_t)ro;
-  asm volatile("" : "+s"(v));
-  return reinterpret_cast<gfloat*>(v);
+// Raw buffer resource over device memory starting at p.  A load or store
+// through it takes the thread's byte offset in a vector register and a
+// wave-uniform byte offset in a scalar register (buffer_load_dword ... offen),
+// so a row access costs no vector address arithmetic and row offsets advance
+// as 32-bit scalar adds.  Offsets must stay below 4 GiB: the plan refuses a
+// streamed config when one wavefield's state region is larger.  Word 3 is the
+// gfx9-family raw 32-bit buffer format.
+typedef __amdgpu_buffer_rsrc_t brsrc_t;
+__device__ __forceinline__ brsrc_t brsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
 }
-__device__ __forceinline__ gfloat* lane(gfloat* row, unsigned off) {
-  return reinterpret_cast<gfloat*>(reinterpret_cast<__attribute__((address_space(1))) char*>(row) + off);
+__device__ __forceinline__ float bld(brsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bst(brsrc_t r, unsigned voff, unsigned soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+// Byte offset of the next row (one padded row on, wrapping at the slot end
+// `end`, a slot being `slot` bytes): a compare and a subtract on the scalar unit.
+__device__ __forceinline__ unsigned next_row(unsigned o, unsigned rowb, unsigned end, unsigned slot) {
+  o += rowb;
+  return o >= end ? o - slot : o;
 }
 
 template <int NT, int KL>
@@ -533,15 +543,19 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const WaveDev& w = a.waves[te.w];
     const ModelDev& m = a.models[w.model];
     const int NZ = w.NZ, NX = w.NX, nz = m.nz, nx = m.nx, nbc = m.nbc;
-    const int64_t Pn = (int64_t)NZ * NX;
-    const float* pin = a.arena + w.off[R_PP] + a.p_in_slot * Pn;
-    const float* qin = a.arena + w.off[R_Q] + a.q_in_slot * Pn;
-    float* pout = a.arena + w.off[R_PP] + a.p_out_slot * Pn;
-    float* qout = a.arena + w.off[R_Q] + a.q_out_slot * Pn;
-    float* ckp = a.ck_slot >= 0 ? a.arena + w.off[R_CKP] + a.ck_slot * Pn : nullptr;
-    float* ckq = a.ck_slot >= 0 ? a.arena + w.off[R_CKQ] + a.ck_slot * Pn : nullptr;
+    const unsigned rowb = (unsigned)NX * 4u, P4 = (unsigned)NZ * rowb;   // bytes of a row, of a slot
+    // the wavefield's state through one buffer resource, alpha through its
+    // own; byte offsets of the slots this launch reads and writes
+    const brsrc_t VW = brsrc(a.arena + w.off[R_HIST]);
+    const brsrc_t VA = brsrc(a.alpha + m.cell_off);
+    auto reg_off = [&](int r, int slot) {
+      return (unsigned)(w.off[r] - w.off[R_HIST]) * 4u + (unsigned)slot * P4;
+    };
+    const unsigned o_pi = reg_off(R_PP, a.p_in_slot), o_qi = reg_off(R_Q, a.q_in_slot);
+    const unsigned o_po = reg_off(R_PP, a.p_out_slot), o_qo = reg_off(R_Q, a.q_out_slot);
+    const bool ck = a.ck_slot >= 0;
+    const unsigned o_cp = ck ? reg_off(R_CKP, a.ck_slot) : 0u, o_cq = ck ? reg_off(R_CKQ, a.ck_slot) : 0u;
     float* seis = a.write_seis ? a.seis + w.seis_off : nullptr;
-    const float* alpha = a.alpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float amp = a.wparam[te.w].amp;
     const int n_rcv = m.n_rcv, rcv_row = m.rcv_row;
@@ -550,11 +564,7 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const int U = IZ + 2 * H;           // window rows
     const int oz = te.z0 - H, ox = te.x0 - H;
     const unsigned gx = wrapi(ox + (int)c, NX);
-    // byte offset of the thread's column in a row, opaque to the compiler so
-    // that every access is (uniform row pointer) + (32-bit lane offset)
-    unsigned gx4 = gx * 4u;
-    asm volatile("" : "+v"(gx4));
-    const int64_t rowb = (int64_t)NX * 4;
+    const unsigned vx = gx * 4u;                  // the thread's column, bytes into a row
     const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
     const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
     const int jx = damp_j(0, gx, nz, nx, nbc);
@@ -584,13 +594,15 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     }
 #pragma unroll
     for (int d = 0; d < SPD; ++d) {
-      const int64_t ro = (int64_t)wrapi(oz + d, NZ) * rowb;
-      pfp[d] = *lane(rowp(pin, ro), gx4);
-      pfq[d] = *lane(rowp(qin, ro), gx4);
-      pfa[d] = *lane(rowp(alpha, ro), gx4);
+      const unsigned ro = (unsigned)wrapi(oz + d, NZ) * rowb;
+      pfp[d] = bld(VW, vx, o_pi + ro);
+      pfq[d] = bld(VW, vx, o_qi + ro);
+      pfa[d] = bld(VA, vx, ro);
       pfk[d] = kzt[oz + d];
     }
-    int gzn = wrapi(oz + SPD, NZ);      // padded row requested next
+    unsigned rq = (unsigned)wrapi(oz + SPD, NZ) * rowb;   // row requested next
+    // (oz + u) * rowb of level KL's row u = t - 2 KL (used on committed rows only)
+    unsigned so = (unsigned)(te.z0 - 2 * H) * rowb;
     __syncthreads();
     // one pipeline step; SPECIAL: some level meets the source or receiver row
     auto body = [&](auto ic, auto sc, const int t) {
@@ -631,30 +643,23 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
           pr[l][s6] = p1;
           qr[l][s3] = qn;
         } else if (u >= H && u < H + IZ) {  // committed row of p[n0+KL], q[n0+KL]
-          const int64_t ro = (int64_t)(oz + u) * rowb;
-          gfloat* const op = rowp(pout, ro);
-          gfloat* const oq = rowp(qout, ro);
-          gfloat* const cp = ckp ? rowp(ckp, ro) : nullptr;
-          gfloat* const cq = ckp ? rowp(ckq, ro) : nullptr;
           if (colint) {
-            *lane(op, gx4) = p1;
-            *lane(oq, gx4) = qn;
-            if (cp) {
-              *lane(cp, gx4) = p1;
-              *lane(cq, gx4) = qn;
+            bst(VW, vx, o_po + so, p1);
+            bst(VW, vx, o_qo + so, qn);
+            if (ck) {
+              bst(VW, vx, o_cp + so, p1);
+              bst(VW, vx, o_cq + so, qn);
             }
           }
         }
       });
+      so += rowb;
       {  // request of window row t + SPD
-        gfloat* const rp = rowp(pin, (int64_t)gzn * rowb);
-        gfloat* const rq = rowp(qin, (int64_t)gzn * rowb);
-        gfloat* const ra = rowp(alpha, (int64_t)gzn * rowb);
-        pfp[i % SPD] = *lane(rp, gx4);
-        pfq[i % SPD] = *lane(rq, gx4);
-        pfa[i % SPD] = *lane(ra, gx4);
+        pfp[i % SPD] = bld(VW, vx, o_pi + rq);
+        pfq[i % SPD] = bld(VW, vx, o_qi + rq);
+        pfa[i % SPD] = bld(VA, vx, rq);
         pfk[i % SPD] = kzt[oz + t + SPD];             // oz + t + SPD in [-H, NZ + H + SUN)
-        gzn = gzn + 1 == NZ ? 0 : gzn + 1;
+        rq = next_row(rq, rowb, P4, P4);
       }
       __syncthreads();
     };
@@ -890,15 +895,18 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
     const WaveDev& w = a.waves[te.w];
     const ModelDev& m = a.models[w.model];
     const int NZ = w.NZ, NX = w.NX, nz = m.nz, nx = m.nx, nbc = m.nbc;
-    const int64_t Pn = (int64_t)NZ * NX;
-    const float* dpin = a.arena + w.off[R_DP] + a.dp_in_slot * Pn;
-    const float* dqin = a.arena + w.off[R_DQ] + a.dq_in_slot * Pn;
-    float* dpout = a.arena + w.off[R_DP] + a.dp_out_slot * Pn;
-    float* dqout = a.arena + w.off[R_DQ] + a.dq_out_slot * Pn;
-    const float* hist = a.arena + w.off[R_HIST];   // q[n] at slot n
+    const unsigned rowb = (unsigned)NX * 4u, P4 = (unsigned)NZ * rowb;   // bytes of a row, of a slot
+    // the wavefield's state through one buffer resource (history region
+    // first: q[n] at slot n), each coefficient field through its own; byte
+    // offsets of the slots this launch reads and writes
+    const brsrc_t VW = brsrc(a.arena + w.off[R_HIST]);
+    const brsrc_t VA = brsrc(a.alpha + m.cell_off), VD = brsrc(a.dalpha + m.cell_off);
+    const unsigned rdp = (unsigned)(w.off[R_DP] - w.off[R_HIST]) * 4u;
+    const unsigned rdq = (unsigned)(w.off[R_DQ] - w.off[R_HIST]) * 4u;
+    const unsigned o_dpi = rdp + (unsigned)a.dp_in_slot * P4, o_dqi = rdq + (unsigned)a.dq_in_slot * P4;
+    const unsigned o_dpo = rdp + (unsigned)a.dp_out_slot * P4, o_dqo = rdq + (unsigned)a.dq_out_slot * P4;
+    const unsigned o_h0 = (unsigned)n0 * P4;      // q[n0]
     float* dseis = a.dseis + w.seis_off;
-    const float* alpha = a.alpha + m.cell_off;
-    const float* dalpha = a.dalpha + m.cell_off;
     const float kscale = a.params[w.model].kscale;
     const float dkscale = a.params[w.model].dkscale;
     const float amp = a.wparam[te.w].amp;
@@ -908,9 +916,7 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
     const int U = IZ + 2 * H;
     const int oz = te.z0 - H, ox = te.x0 - H;
     const unsigned gx = wrapi(ox + (int)c, NX);
-    unsigned gx4 = gx * 4u;
-    asm volatile("" : "+v"(gx4));
-    const int64_t rowb = (int64_t)NX * 4;
+    const unsigned vx = gx * 4u;                  // the thread's column, bytes into a row
     const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
     const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
     const int jx = damp_j(0, gx, nz, nx, nbc);
@@ -945,32 +951,38 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
 #pragma unroll
       for (int s = 0; s < SPH; ++s) hq[l][s] = 0.f;
     }
-    // coefficients of window row u into the ring slot u % SUNB
-    auto put_cf = [&](int slot, int u, float al, float dal) {
-      const int jz = damp_j(wrapi(oz + u, NZ), nbc, nz, nx, nbc);
+    // coefficients of padded row gz into the ring slot (window row u at u % SUNB)
+    auto put_cf = [&](int slot, int gz, float al, float dal) {
+      const int jz = damp_j(gz, nbc, nz, nx, nbc);
       const float jj = isx ? jjx : (float)(jz * jz);
       CF[slot][c] = make_float4(al, dal, 1.0f - kscale * jj, dkscale * jj);
     };
-    auto hist_row = [&](int n, int u) {  // q[n], window row u (wrapped padded row)
-      const int64_t ro = (int64_t)n * Pn * 4 + (int64_t)wrapi(oz + u, NZ) * rowb;
-      return *lane(rowp(hist, ro), gx4);
-    };
+    auto row_off = [&](int u) { return (unsigned)wrapi(oz + u, NZ) * rowb; };  // window row u
 #pragma unroll
     for (int d = 0; d < SPB; ++d) {
-      const int64_t ro = (int64_t)wrapi(oz + d, NZ) * rowb;
-      pfp[d] = *lane(rowp(dpin, ro), gx4);
-      pfq[d] = *lane(rowp(dqin, ro), gx4);
-      pfa[d] = *lane(rowp(alpha, ro), gx4);
-      pfd[d] = *lane(rowp(dalpha, ro), gx4);
-      pfh[d] = hist_row(n0, d);
+      const unsigned ro = row_off(d);
+      pfp[d] = bld(VW, vx, o_dpi + ro);
+      pfq[d] = bld(VW, vx, o_dqi + ro);
+      pfa[d] = bld(VA, vx, ro);
+      pfd[d] = bld(VD, vx, ro);
+      pfh[d] = bld(VW, vx, o_h0 + ro);
     }
-    // history of the first steps: level l meets window row t - 2l at step t
+    // history of the first steps: level l meets window row t - 2l at step t;
+    // hb[l-1] is the offset of the row level l requests next (q[n0+l] of
+    // window row t - 2l + SPH), he[l-1] the end of that slot
+    unsigned hb[KL], he[KL];
 #pragma unroll
-    for (int d = 0; d < SPH; ++d) {
+    for (int l = 1; l <= KL; ++l) {
+      const unsigned slot = o_h0 + (unsigned)l * P4;
 #pragma unroll
-      for (int l = 1; l <= KL; ++l) hq[l - 1][d] = hist_row(n0 + l, d - 2 * l);
+      for (int d = 0; d < SPH; ++d) hq[l - 1][d] = bld(VW, vx, slot + row_off(d - 2 * l));
+      hb[l - 1] = slot + row_off(SPH - 2 * l);
+      he[l - 1] = slot + P4;
     }
-    int gzn = wrapi(oz + SPB, NZ);      // padded row of the next level-0 request
+    unsigned rq = row_off(SPB);            // next level-0 request
+    int gzc = wrapi(oz, NZ);               // padded row of window row t
+    // (oz + u) * rowb of level KL's row u = t - 2 KL (used on committed rows only)
+    unsigned so = (unsigned)(te.z0 - 2 * H) * rowb;
     __syncthreads();
     auto body = [&](auto ic, auto sc, const int t) {
       constexpr int i = decltype(ic)::value;
@@ -1021,28 +1033,27 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
           pr[l][s6] = dp1;
           qr[l][s3] = dqn;
         } else if (u >= H && u < H + IZ) {  // committed row of (dp, dq)[n0+KL]
-          const int64_t ro = (int64_t)(oz + u) * rowb;
-          gfloat* const op = rowp(dpout, ro);
-          gfloat* const oq = rowp(dqout, ro);
           if (colint) {
-            *lane(op, gx4) = dp1;
-            *lane(oq, gx4) = dqn;
+            bst(VW, vx, o_dpo + so, dp1);
+            bst(VW, vx, o_dqo + so, dqn);
           }
         }
         if constexpr (l < KL) qps[l][i % 3] = qnh;      // for level l+1, two steps on
         // this level's q for its row SPH steps ahead
-        hq[l - 1][i % SPH] = hist_row(n0 + l, u + SPH);
+        hq[l - 1][i % SPH] = bld(VW, vx, hb[l - 1]);
+        hb[l - 1] = next_row(hb[l - 1], rowb, he[l - 1], P4);
       });
       qps[0][i % 3] = qin0;
-      put_cf(i % SUNB, t, al0, dal0);
+      put_cf(i % SUNB, gzc, al0, dal0);
+      gzc = gzc + 1 == NZ ? 0 : gzc + 1;
+      so += rowb;
       {  // request of window row t + SPB
-        const int64_t ro = (int64_t)gzn * rowb;
-        pfp[i % SPB] = *lane(rowp(dpin, ro), gx4);
-        pfq[i % SPB] = *lane(rowp(dqin, ro), gx4);
-        pfa[i % SPB] = *lane(rowp(alpha, ro), gx4);
-        pfd[i % SPB] = *lane(rowp(dalpha, ro), gx4);
-        pfh[i % SPB] = *lane(rowp(hist, (int64_t)n0 * Pn * 4 + ro), gx4);
-        gzn = gzn + 1 == NZ ? 0 : gzn + 1;
+        pfp[i % SPB] = bld(VW, vx, o_dpi + rq);
+        pfq[i % SPB] = bld(VW, vx, o_dqi + rq);
+        pfa[i % SPB] = bld(VA, vx, rq);
+        pfd[i % SPB] = bld(VD, vx, rq);
+        pfh[i % SPB] = bld(VW, vx, o_h0 + rq);
+        rq = next_row(rq, rowb, P4, P4);
       }
       __syncthreads();
     };
@@ -1898,6 +1909,7 @@ struct Layout {
   int Kc = 0, J = 0;          // checkpoint interval and count (checkpoint mode)
   bool full = true, fwd_hist = true;
   int max_P = 0, max_cells_phys = 0, max_NZ = 0;
+  int64_t max_wave_bytes = 0;   // largest state region of one wavefield (all its slots)
   int64_t bytes = 0;
 };
 
@@ -1971,6 +1983,7 @@ int validate_and_layout(const seis_model_desc* md, int n_models, const seis_time
       region(R_GK, adj ? 1 : 0);
       region(R_CKP, adj && !L.full ? L.J : 0);
       region(R_CKQ, adj && !L.full ? L.J : 0);
+      L.max_wave_bytes = std::max<int64_t>(L.max_wave_bytes, (L.arena - w.off[R_HIST]) * (int64_t)sizeof(float));
       L.waves.push_back(w);
     }
     L.models.push_back(m);
@@ -2739,9 +2752,13 @@ int seis_plan_set_config(seis_plan_t* p, int32_t kind, int32_t rx, int32_t rz, i
   if (!cfg_ok(kind, c))
     return fail(SEIS_EINVAL, "unsupported config: window " + std::to_string(rx) + "x" + std::to_string(rz) +
                                  ", k=" + std::to_string(k) + ", strips=" + std::to_string(strips));
-  if (c.strips == 2)
+  if (c.strips == 2) {
     for (const WaveDev& w : p->L.waves)
-      if (w.NZ < 4 * c.k) return fail(SEIS_EINVAL, "streamed forward needs padded grids of at least 4k rows");
+      if (w.NZ < 4 * c.k) return fail(SEIS_EINVAL, "streamed kernels need padded grids of at least 4k rows");
+    // the streamed kernels address a wavefield's state with 32-bit byte offsets
+    if (p->L.max_wave_bytes >= (int64_t)UINT32_MAX)
+      return fail(SEIS_EINVAL, "streamed kernels need each wavefield's state region below 4 GiB");
+  }
   HIP_TRY(hipDeviceSynchronize());
   drop_graphs(p);
   p->cfg[kind] = c;
