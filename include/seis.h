@@ -193,13 +193,15 @@ enum {
  * strips = 1 gives each wavefront consecutive window rows (vertical stencil
  * neighbours stay in registers), strips = 0 deals rows round-robin over the
  * wavefronts (even work per wavefront as the valid band shrinks).
- * strips = 2 (SEIS_KIND_FORWARD and SEIS_KIND_BORN) streams instead: a
- * workgroup of rx threads owns an rx-column strip (halo included) and
- * marches it down the rows of an rz-row segment (rz = 0: the whole padded
- * column), advancing k time levels in a software pipeline; rx in {64, 128,
- * 192, 256}, k in {3, 4}, padded grids of at least 4k rows.  Launches shorter
- * than k steps (the last one, or up to a checkpoint) run a tiled window: the
- * record kind's for the forward, the adjoint kind's for the Born pass.
+ * strips = 2 (SEIS_KIND_FORWARD, SEIS_KIND_BORN, SEIS_KIND_ADJOINT) streams
+ * instead: a workgroup of rx threads owns an rx-column strip (halo included)
+ * and marches it down the rows of an rz-row segment (rz = 0: the whole
+ * padded column), advancing k time levels in a software pipeline; forward
+ * and Born rx in {64, 128, 192, 256}, k in {3, 4}; adjoint rx in {64, 128},
+ * k in {4, 6}; padded grids of at least 4k rows, and each wavefield's state
+ * region below 4 GiB (32-bit offsets).  Launches shorter than k steps (the
+ * last one, or up to a checkpoint) run a tiled window: the record kind's for
+ * the forward, the tangent kind's for the Born and adjoint passes.
  * Windows (rx x rz): forward and record 64x32, 128x32, 64x64, 128x64, 256x32,
  * 128x96, 128x128; Born 64x32, 128x32, 64x64, 128x64, 256x32; adjoint 64x32,
  * 128x32, 64x64, 128x64; tangent 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4 rows and

@@ -1294,6 +1294,223 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   }
 }
 
+// KL reverse steps of update_p_adjoint (:517-579) with the imaging terms,
+// streamed like k_bns: a column strip of NT padded columns (one per thread)
+// marched down the rows of a segment.  Pipeline step t takes in window row t
+// of lam[M0], mu[M0], alpha, the imaging accumulators and q[M0-1] as level 0,
+// and level l = 1..KL computes window row t - 2l of (lam, mu)[M0-l] from level
+// l-1 and adds its imaging terms to that row's accumulators, which level l-1
+// passed on two steps earlier; level KL stores the committed rows.  Level l
+// reads q[M0-l-1] of its row (requested SPH steps ahead) and takes q[M0-l]
+// from level l-1, which read it two steps earlier.  alpha*lam of every level
+// sits in registers (own column) and in a three-row LDS ring (neighbours);
+// (alpha, 1 - kappa) of the rows in flight in a private LDS ring, 1 - kappa
+// outside the left/right damping strips from the model's row table.  Same
+// stencil order and fused multiply-add sequence as k_adj, and each cell's
+// accumulators see the same sequence of updates, so lam, mu, the imaging
+// accumulators and the source-wavelet correlation are bit-identical to it.
+// Launches always advance KL steps (shorter ones run k_adj).
+template <int NT, int KL>
+__global__ __launch_bounds__(NT) void k_ads(AdjArgs a) {
+  constexpr int SUNA = 2 * KL <= 12 ? 12 : 24;   // pipeline steps per unrolled block; coefficient ring length
+  static_assert(2 * KL <= SUNA && SUNA % 6 == 0 && SUNA % SPB == 0 && SUNA % SPH == 0, "pipeline rings");
+  constexpr int H = 2 * KL;
+  constexpr int RS = 3 * KL + 1;          // column-major level rings, odd stride (see k_fws)
+  __shared__ float RG[(NT + 4) * RS];     // alpha*lam of levels 0..KL-1
+  __shared__ float2 AK[SUNA][NT];         // (alpha, 1 - kappa) of window row u at [u % SUNA], own column
+  const unsigned c = threadIdx.x;
+  const int ntiles = *a.n_tiles;
+  const int M0 = a.M0, nt = a.nt;
+  for (int e = blockIdx.x; e < ntiles; e += gridDim.x) {
+    const TileEnt te = a.tiles[e];
+    if (te.w < 0) continue;
+    const WaveDev& w = a.waves[te.w];
+    const ModelDev& m = a.models[w.model];
+    const int NZ = w.NZ, NX = w.NX, nz = m.nz, nx = m.nx, nbc = m.nbc;
+    const unsigned rowb = (unsigned)NX * 4u, P4 = (unsigned)NZ * rowb;   // bytes of a row, of a slot
+    const brsrc_t VW = brsrc(a.arena + w.off[R_HIST]);
+    const brsrc_t VA = brsrc(a.alpha + m.cell_off);
+    auto reg_off = [&](int r, int slot) {
+      return (unsigned)(w.off[r] - w.off[R_HIST]) * 4u + (unsigned)slot * P4;
+    };
+    const unsigned o_li = reg_off(R_LAM, a.lam_in_slot), o_mi = reg_off(R_MU, a.mu_in_slot);
+    const unsigned o_lo = reg_off(R_LAM, a.lam_out_slot), o_mo = reg_off(R_MU, a.mu_out_slot);
+    const unsigned o_ga = reg_off(R_GA, 0), o_gk = reg_off(R_GK, 0);
+    auto o_q = [&](int n) { return (unsigned)(n - a.h_base) * P4; };   // q[n]
+    const unsigned o_q1 = o_q(M0 - 1);
+    const float kscale = a.params[w.model].kscale;
+    const float amp = a.wparam[te.w].amp;
+    const float* kzt = a.alpha + m.kz_off + kKzPad;   // kzt[z]: padded row z, -kKzPad <= z < NZ + kKzPad
+    const int IZ = te.pad;
+    const int U = IZ + 2 * H;
+    const int oz = te.z0 - H, ox = te.x0 - H;
+    const unsigned gx = wrapi(ox + (int)c, NX);
+    const unsigned vx = gx * 4u;
+    const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
+    const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
+    const int jx = damp_j(0, gx, nz, nx, nbc);
+    const float kx1 = 1.0f - kscale * (float)(jx * jx);
+    const int rc = rcv_index(gx, m);
+    const bool rcc = rc >= 0;
+    const int srow = w.src_cell / NX;
+    const bool srcc = (int)gx == w.src_cell - srow * NX;
+    // window rows holding the source row and the receiver row (twice when
+    // the window wraps); the committed instance of the source row, if any
+    const int su0 = wrapi(srow - oz, NZ), su1 = su0 + NZ;
+    const int ru0 = wrapi(m.rcv_row - oz, NZ), ru1 = ru0 + NZ;
+    const int suz = srow - te.z0;
+    const int sui = suz >= 0 && suz < IZ ? suz + H : -(1 << 20);
+    const bool src_here = srcc && colint && sui >= 0;
+    // per level: adjoint source sample of the thread's column, wavelet sample
+    float injv[KL], sw[KL];
+#pragma unroll
+    for (int l = 1; l <= KL; ++l) {
+      const int it = M0 - 2 - l;
+      sw[l - 1] = (it >= 0 && it < nt) ? a.wavelet[it] : 0.f;
+      float v = 0.f;
+      if (rcc) {
+        const int64_t si = w.seis_off + (int64_t)it * m.n_rcv + rc;
+        v = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
+      }
+      injv[l - 1] = v;
+    }
+    double ssum = src_here ? a.wparam[te.w].ssum : 0.0;
+    // rings of levels 0..KL-1 (own column): alpha*lam by row [u % 6]; lam, mu,
+    // the accumulators after the level, and q[M0-l] by row [u % 3]
+    float ar[KL][6], lr[KL][3], mr[KL][3], gar[KL][3], gkr[KL][3], qpr[KL][3];
+    float pfl[SPB], pfm[SPB], pfa[SPB], pfg[SPB], pfk[SPB], pfq[SPB], pfz[SPB];
+    float hq[KL][SPH];   // q[M0-l-1] of the row level l meets SPH steps ahead
+#pragma unroll
+    for (int l = 0; l < KL; ++l) {
+#pragma unroll
+      for (int s = 0; s < 6; ++s) ar[l][s] = 0.f;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        lr[l][s] = 0.f;
+        mr[l][s] = 0.f;
+        gar[l][s] = 0.f;
+        gkr[l][s] = 0.f;
+        qpr[l][s] = 0.f;
+      }
+    }
+    auto row_off = [&](int u) { return (unsigned)wrapi(oz + u, NZ) * rowb; };  // window row u
+#pragma unroll
+    for (int d = 0; d < SPB; ++d) {
+      const unsigned ro = row_off(d);
+      pfl[d] = bld(VW, vx, o_li + ro);
+      pfm[d] = bld(VW, vx, o_mi + ro);
+      pfa[d] = bld(VA, vx, ro);
+      pfg[d] = bld(VW, vx, o_ga + ro);
+      pfk[d] = bld(VW, vx, o_gk + ro);
+      pfq[d] = bld(VW, vx, o_q1 + ro);
+      pfz[d] = kzt[oz + d];
+    }
+    // hb[l-1]: offset of the row level l requests next (q[M0-l-1] of window
+    // row t - 2l + SPH), he[l-1] the end of that slot
+    unsigned hb[KL], he[KL];
+#pragma unroll
+    for (int l = 1; l <= KL; ++l) {
+      const unsigned slot = o_q(M0 - l - 1);
+#pragma unroll
+      for (int d = 0; d < SPH; ++d) hq[l - 1][d] = bld(VW, vx, slot + row_off(d - 2 * l));
+      hb[l - 1] = slot + row_off(SPH - 2 * l);
+      he[l - 1] = slot + P4;
+    }
+    unsigned rq = row_off(SPB);                     // next level-0 request
+    unsigned so = (unsigned)(te.z0 - 2 * H) * rowb;  // (oz + u) * rowb of level KL's row u = t - 2 KL
+    __syncthreads();
+    auto body = [&](auto ic, auto sc, const int t) {
+      constexpr int i = decltype(ic)::value;
+      constexpr bool SPECIAL = decltype(sc)::value;
+      float al0 = 0.f, k10 = 0.f;
+      {  // level 0: intake of window row t (its coefficients go to the ring after the levels)
+        al0 = pfa[i % SPB];
+        k10 = isx ? kx1 : pfz[i % SPB];
+        const float l0 = pfl[i % SPB];
+        const float a0 = al0 * l0;
+        RG[(c + 2) * RS + i % 3] = a0;
+        ar[0][i % 6] = a0;
+        lr[0][i % 3] = l0;
+        mr[0][i % 3] = pfm[i % SPB];
+        gar[0][i % 3] = pfg[i % SPB];
+        gkr[0][i % 3] = pfk[i % SPB];
+        qpr[0][i % 3] = pfq[i % SPB];
+      }
+      static_for<1, KL + 1>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        const int u = t - 2 * l;
+        constexpr int B = 4 * SUNA;
+        constexpr int s6 = (i - 2 * l + B) % 6, s3 = (i - 2 * l + B) % 3, sr = (i - 2 * l + B) % SUNA;
+        const float pc = ar[l - 1][s6];
+        const float* T = RG + c * RS + (l - 1) * 3 + s3;
+        float t1 = (T[3 * RS] - pc) + (T[RS] - pc);
+        t1 = t1 + (ar[l - 1][(i - 2 * l + 1 + B) % 6] - pc);
+        t1 = t1 + (ar[l - 1][(i - 2 * l - 1 + B) % 6] - pc);
+        float t2 = (T[4 * RS] - pc) + (T[0] - pc);
+        t2 = t2 + (ar[l - 1][(i - 2 * l + 2 + B) % 6] - pc);
+        t2 = t2 + (ar[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
+        const float La = fmaf(kC2, t1, kC3 * t2);
+        const float2 ak = AK[sr][c];
+        float mun = fmaf(ak.y, mr[l - 1][s3], La);
+        if constexpr (SPECIAL) {
+          if (u == ru0 || u == ru1) mun = rcc ? mun + injv[l - 1] : mun;
+        }
+        const float ln = lr[l - 1][s3] + mun;
+        // imaging: GA += lam[m] alpha Lf(p[m-1]), GK -= lam[m] q[m-1], m = M0 - l
+        const float qlo = hq[l - 1][i % SPH];      // q[m-1] of row u
+        const float qhi = qpr[l - 1][s3];          // q[m] of row u
+        float aL = fmaf(-ak.y, qlo, qhi);
+        if constexpr (SPECIAL) {
+          if (u == su0 || u == su1) aL = srcc ? aL - amp * sw[l - 1] : aL;
+          if (src_here && u == sui) ssum += (double)ln * (double)sw[l - 1];
+        }
+        const float gan = fmaf(ln, aL, gar[l - 1][s3]);
+        const float gkn = fmaf(ln, -qlo, gkr[l - 1][s3]);
+        if constexpr (l < KL) {
+          const float an = ak.x * ln;
+          RG[(c + 2) * RS + l * 3 + s3] = an;
+          ar[l][s6] = an;
+          lr[l][s3] = ln;
+          mr[l][s3] = mun;
+          gar[l][s3] = gan;
+          gkr[l][s3] = gkn;
+          qpr[l][s3] = qlo;
+        } else if (u >= H && u < H + IZ) {  // committed row of (lam, mu)[M0-KL] and the accumulators
+          if (colint) {
+            bst(VW, vx, o_lo + so, ln);
+            bst(VW, vx, o_mo + so, mun);
+            bst(VW, vx, o_ga + so, gan);
+            bst(VW, vx, o_gk + so, gkn);
+          }
+        }
+        hq[l - 1][i % SPH] = bld(VW, vx, hb[l - 1]);   // this level's q for its row SPH steps ahead
+        hb[l - 1] = next_row(hb[l - 1], rowb, he[l - 1], P4);
+      });
+      AK[i % SUNA][c] = make_float2(al0, k10);
+      so += rowb;
+      {  // request of window row t + SPB
+        pfl[i % SPB] = bld(VW, vx, o_li + rq);
+        pfm[i % SPB] = bld(VW, vx, o_mi + rq);
+        pfa[i % SPB] = bld(VA, vx, rq);
+        pfg[i % SPB] = bld(VW, vx, o_ga + rq);
+        pfk[i % SPB] = bld(VW, vx, o_gk + rq);
+        pfq[i % SPB] = bld(VW, vx, o_q1 + rq);
+        pfz[i % SPB] = kzt[oz + t + SPB];             // oz + t + SPB in [-H, NZ + H + SUNA)
+        rq = next_row(rq, rowb, P4, P4);
+      }
+      __syncthreads();
+    };
+    auto meets = [&](int tb, int r) { return r + 2 <= tb + SUNA - 1 && r + 2 * KL >= tb; };
+    for (int tb = 0; tb < U; tb += SUNA) {
+      if (meets(tb, su0) || meets(tb, su1) || meets(tb, ru0) || meets(tb, ru1))
+        static_for<0, SUNA>([&](auto ic) { body(ic, std::true_type(), tb + decltype(ic)::value); });
+      else
+        static_for<0, SUNA>([&](auto ic) { body(ic, std::false_type(), tb + decltype(ic)::value); });
+    }
+    if (src_here) a.wparam[te.w].ssum = ssum;
+  }
+}
+
 // Zero `nslots` slots of region `reg` of every active wavefield (graph-safe
 // fill: no hipMemset nodes are recorded in captured graphs).
 __global__ __launch_bounds__(NTHR) void k_fill(const WaveDev* waves, const ModelDev* models,
@@ -1876,6 +2093,9 @@ constexpr int kStreamKMin = 3, kStreamKMax = 4;
 // Streamed Born (strips = 2): the same strips, 3 or 4 levels per launch.
 constexpr int kStreamBornKMin = 3, kStreamBornKMax = 4;
 
+// Streamed adjoint (strips = 2): 64- or 128-column strips, 4 or 6 levels.
+bool stream_adj_ok(const KCfg& c) { return (c.rx == 64 || c.rx == 128) && (c.k == 4 || c.k == 6); }
+
 bool stream_cfg(int kind, const KCfg& c) {
   if (c.strips != 2) return false;
   bool nt_ok = false;
@@ -1883,6 +2103,7 @@ bool stream_cfg(int kind, const KCfg& c) {
   if (!nt_ok || !(c.rz == 0 || c.rz >= 8)) return false;
   if (kind == SEIS_KIND_FORWARD) return c.k >= kStreamKMin && c.k <= kStreamKMax;
   if (kind == SEIS_KIND_BORN) return c.k >= kStreamBornKMin && c.k <= kStreamBornKMax;
+  if (kind == SEIS_KIND_ADJOINT) return stream_adj_ok(c);
   return false;
 }
 
@@ -2116,8 +2337,15 @@ const void* bns_ptr(const KCfg& c) {
   return bns_ptr_nt<256>(c.k);
 }
 
+template <int NT>
+const void* ads_ptr_nt(int k) {
+  return k == 4 ? (const void*)k_ads<NT, 4> : (const void*)k_ads<NT, 6>;
+}
+
+const void* ads_ptr(const KCfg& c) { return c.rx == 64 ? ads_ptr_nt<64>(c.k) : ads_ptr_nt<128>(c.k); }
+
 const void* kernel_ptr(int kind, const KCfg& c) {
-  if (c.strips == 2) return kind == SEIS_KIND_BORN ? bns_ptr(c) : fws_ptr(c);
+  if (c.strips == 2) return kind == SEIS_KIND_BORN ? bns_ptr(c) : kind == SEIS_KIND_ADJOINT ? ads_ptr(c) : fws_ptr(c);
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
 #define PTR_R(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, true>(c);
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
@@ -2178,7 +2406,18 @@ void launch_adj_t(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
   else hipLaunchKernelGGL((k_adj<RX, RZ, NT, false>), dim3(grid), dim3(NT), 0, s, a);
 }
 
+template <int NT>
+void launch_ads_t(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
+  if (c.k == 4) hipLaunchKernelGGL((k_ads<NT, 4>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_ads<NT, 6>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 void launch_adj(const KCfg& c, int grid, hipStream_t s, const AdjArgs& a) {
+  if (c.strips == 2) {
+    if (c.rx == 64) launch_ads_t<64>(c, grid, s, a);
+    else launch_ads_t<128>(c, grid, s, a);
+    return;
+  }
 #define LA(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_adj_t<RX, RZ, NT>(c, grid, s, a); return; }
   SEIS_ADJ_SHAPES(LA)
 #undef LA
@@ -2368,8 +2607,9 @@ void enqueue_born(seis_plan* p, hipStream_t s) {
   a.arena = p->d_arena; a.dseis = p->d_dseis; a.nt = p->nt;
   a.Kt = c.k;
   // a streamed Born pass always advances c.k steps; the shorter last launch
-  // runs k_born with the adjoint kind's window (a Born shape too)
-  const KCfg& ct = p->cfg[SEIS_KIND_ADJOINT];
+  // runs k_born with the fused tangent kind's window and work list (that kind
+  // is always tiled, and its windows are Born shapes too)
+  const KCfg& ct = p->cfg[SEIS_KIND_TANGENT];
   int n = 0, li = 0;
   while (n < p->nt + 1) {
     int K = std::min(c.k, p->nt + 1 - n);
@@ -2381,10 +2621,10 @@ void enqueue_born(seis_plan* p, hipStream_t s) {
     a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
     if (tail) {
       BornArgs at = a;
-      at.tiles = p->tl[SEIS_KIND_ADJOINT].d;
-      at.n_tiles = p->tl[SEIS_KIND_ADJOINT].d_count;
+      at.tiles = p->tl[SEIS_KIND_TANGENT].d;
+      at.n_tiles = p->tl[SEIS_KIND_TANGENT].d_count;
       at.Kt = ct.k;
-      launch_born(ct, p->tl[SEIS_KIND_ADJOINT].grid, s, at);
+      launch_born(ct, p->tl[SEIS_KIND_TANGENT].grid, s, at);
     } else {
       launch_born(c, p->tl[SEIS_KIND_BORN].grid, s, a);
     }
@@ -2413,17 +2653,29 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
   else if (kind == 1) { a.inj_a = p->d_dd; a.inj_b = nullptr; }
   else { a.inj_a = p->d_dseis; a.inj_b = nullptr; }
   const int grid = p->tl[SEIS_KIND_ADJOINT].grid;
+  // a streamed adjoint always advances ca.k steps; shorter launches run k_adj
+  // with the fused tangent kind's window and work list (always tiled; its
+  // windows are adjoint shapes too)
+  const KCfg& ct = p->cfg[SEIS_KIND_TANGENT];
+  AdjArgs at = a;
+  at.tiles = p->tl[SEIS_KIND_TANGENT].d;
+  at.n_tiles = p->tl[SEIS_KIND_TANGENT].d_count;
+  at.Kt = ct.k;
   int li = 0;
   auto run_range = [&](int m_hi, int m_lo, int base) {
     int M0 = m_hi + 1;
     while (M0 - 1 >= m_lo) {
-      const int K = std::min(ca.k, M0 - m_lo);
-      a.M0 = M0;
-      a.K = K;
-      a.lam_in_slot = a.mu_in_slot = li & 1;
-      a.lam_out_slot = a.mu_out_slot = (li + 1) & 1;
-      a.h_base = base;
-      launch_adj(ca, grid, s, a);
+      int K = std::min(ca.k, M0 - m_lo);
+      const bool tail = ca.strips == 2 && K < ca.k;
+      if (tail) K = std::min(K, ct.k);
+      AdjArgs& f = tail ? at : a;
+      f.M0 = M0;
+      f.K = K;
+      f.lam_in_slot = f.mu_in_slot = li & 1;
+      f.lam_out_slot = f.mu_out_slot = (li + 1) & 1;
+      f.h_base = base;
+      if (tail) launch_adj(ct, p->tl[SEIS_KIND_TANGENT].grid, s, f);
+      else launch_adj(ca, grid, s, f);
       M0 -= K;
       ++li;
     }

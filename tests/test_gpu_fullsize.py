@@ -53,8 +53,8 @@ def _configs(P):
 
 def test_tuned_batch32_plan_golden():
     """The 32-model Gauss-Newton plan of the benchmark (full history): the
-    golden 70x70 model at slot 13 of 32.  Forward (streamed 128-column
-    strips), gradient (128x64 record and adjoint windows, k = 6), Born on the
+    golden 70x70 model at slot 13 of 32.  Forward (streamed strips),
+    gradient (128x96 record and 128x64 adjoint windows, k = 6), Born on the
     saved forward (streamed, 160-row segments) in a dot-product test, the
     fused tangent bit-identical to the Born pass."""
     g = Geometry()
@@ -65,7 +65,7 @@ def test_tuned_batch32_plan_golden():
     P = Propagator([g] * B)
     applied = autotune.apply(P)
     print("batch32 configs", _configs(P), flush=True)
-    assert applied["born"][3] == 2 and applied["forward"][3] == 2 and applied["adjoint"][:3] == (128, 64, 6)
+    assert applied["born"][3] == 2 and applied["forward"][3] == 2 and applied["adjoint"] == (128, 64, 6, 1)
     v = torch.tensor(np.concatenate(vecs))
     sl = slice(int(P.seis_off[at]), int(P.seis_off[at + 1]))
     vl = slice(int(P.vec_off[at]), int(P.vec_off[at + 1]))
