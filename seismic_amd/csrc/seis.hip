@@ -1419,9 +1419,9 @@ __global__ __launch_bounds__(NT) void k_ads(AdjArgs a) {
     unsigned rq = row_off(SPB);                     // next level-0 request
     unsigned so = (unsigned)(te.z0 - 2 * H) * rowb;  // (oz + u) * rowb of level KL's row u = t - 2 KL
     __syncthreads();
-    auto body = [&](auto ic, auto sc, const int t) {
+    unsigned sm[KL];   // per level: bit i set if step tb + i meets a source or receiver row
+    auto body = [&](auto ic, const int t) {
       constexpr int i = decltype(ic)::value;
-      constexpr bool SPECIAL = decltype(sc)::value;
       float al0 = 0.f, k10 = 0.f;
       {  // level 0: intake of window row t (its coefficients go to the ring after the levels)
         al0 = pfa[i % SPB];
@@ -1452,7 +1452,8 @@ __global__ __launch_bounds__(NT) void k_ads(AdjArgs a) {
         const float La = fmaf(kC2, t1, kC3 * t2);
         const float2 ak = AK[sr][c];
         float mun = fmaf(ak.y, mr[l - 1][s3], La);
-        if constexpr (SPECIAL) {
+        const bool sp = (sm[l - 1] >> i) & 1u;
+        if (sp) {
           if (u == ru0 || u == ru1) mun = rcc ? mun + injv[l - 1] : mun;
         }
         const float ln = lr[l - 1][s3] + mun;
@@ -1460,7 +1461,7 @@ __global__ __launch_bounds__(NT) void k_ads(AdjArgs a) {
         const float qlo = hq[l - 1][i % SPH];      // q[m-1] of row u
         const float qhi = qpr[l - 1][s3];          // q[m] of row u
         float aL = fmaf(-ak.y, qlo, qhi);
-        if constexpr (SPECIAL) {
+        if (sp) {
           if (u == su0 || u == su1) aL = srcc ? aL - amp * sw[l - 1] : aL;
           if (src_here && u == sui) ssum += (double)ln * (double)sw[l - 1];
         }
@@ -1500,12 +1501,23 @@ __global__ __launch_bounds__(NT) void k_ads(AdjArgs a) {
       }
       __syncthreads();
     };
-    auto meets = [&](int tb, int r) { return r + 2 <= tb + SUNA - 1 && r + 2 * KL >= tb; };
+    // One copy of the unrolled block: a wave-uniform bit per level and step
+    // guards the source and receiver terms.  (Two copies of the block, picked
+    // per block, make the register allocator keep both copies' values apart
+    // across the loop: about 90 more VGPRs at 6 levels.)
     for (int tb = 0; tb < U; tb += SUNA) {
-      if (meets(tb, su0) || meets(tb, su1) || meets(tb, ru0) || meets(tb, ru1))
-        static_for<0, SUNA>([&](auto ic) { body(ic, std::true_type(), tb + decltype(ic)::value); });
-      else
-        static_for<0, SUNA>([&](auto ic) { body(ic, std::false_type(), tb + decltype(ic)::value); });
+#pragma unroll
+      for (int l = 1; l <= KL; ++l) {
+        unsigned msk = 0;
+        const int rows[4] = {su0, su1, ru0, ru1};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int d = rows[q] + 2 * l - tb;
+          msk |= (d >= 0 && d < SUNA) ? 1u << d : 0u;
+        }
+        sm[l - 1] = msk;
+      }
+      static_for<0, SUNA>([&](auto ic) { body(ic, tb + decltype(ic)::value); });
     }
     if (src_here) a.wparam[te.w].ssum = ssum;
   }
