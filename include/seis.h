@@ -193,15 +193,15 @@ enum {
  * strips = 1 gives each wavefront consecutive window rows (vertical stencil
  * neighbours stay in registers), strips = 0 deals rows round-robin over the
  * wavefronts (even work per wavefront as the valid band shrinks).
- * strips = 2 (SEIS_KIND_FORWARD, SEIS_KIND_BORN, SEIS_KIND_ADJOINT) streams
- * instead: a workgroup of rx threads owns an rx-column strip (halo included)
- * and marches it down the rows of an rz-row segment (rz = 0: the whole
- * padded column), advancing k time levels in a software pipeline; forward
- * and Born rx in {64, 128, 192, 256}, k in {3, 4}; adjoint rx in {64, 128},
- * k in {4, 6}; padded grids of at least 4k rows, and each wavefield's state
- * region below 4 GiB (32-bit offsets).  Launches shorter than k steps (the
- * last one, or up to a checkpoint) run a tiled window: the record kind's for
- * the forward, the tangent kind's for the Born and adjoint passes.
+ * strips = 2 streams instead: a workgroup of rx threads owns an rx-column
+ * strip (halo included) and marches it down the rows of an rz-row segment
+ * (rz = 0: the whole padded column, rz >= 8 otherwise), advancing k time
+ * levels in a software pipeline; forward and Born rx in {64, 128, 192, 256},
+ * k in {3, 4}; record and tangent rx in {128, 192}, k in {3, 4}; adjoint rx
+ * in {64, 128}, k in {4, 6}; padded grids of at least 4k rows, and each
+ * wavefield's state region below 4 GiB (32-bit offsets).  Launches shorter
+ * than k steps (the last one, or up to a checkpoint) run a tiled 64x64
+ * window.
  * Windows (rx x rz): forward and record 64x32, 128x32, 64x64, 128x64, 256x32,
  * 128x96, 128x128; Born 64x32, 128x32, 64x64, 128x64, 256x32; adjoint 64x32,
  * 128x32, 64x64, 128x64; tangent 64x32, 128x32, 64x64; 1 <= k <= 8 with an interior of at least 4 rows and
@@ -213,6 +213,16 @@ int seis_plan_set_config(seis_plan_t* plan, int32_t kind, int32_t rx, int32_t rz
                          int32_t k, int32_t strips);
 int seis_plan_get_config(const seis_plan_t* plan, int32_t kind, int32_t* rx,
                          int32_t* rz, int32_t* k, int32_t* strips /* host */);
+
+/* A streamed config (strips = 2) runs a pass only while at least
+ * min_active_tiles of its strip segments belong to active models; with fewer
+ * (a few models of a large plan active) the pass runs the tiled 64x64 window
+ * instead, which splits the same cells into more, shorter pieces.  Results
+ * are the same bit for bit either way.  Negative restores the default, the
+ * device's compute-unit count; 0 always streams.  No reference counterpart
+ * (the reference runs one model per call).  Host-only; takes effect at the
+ * next pass. */
+int seis_plan_set_stream_threshold(seis_plan_t* plan, int32_t min_active_tiles);
 
 /* Device bytes held by the plan. */
 int64_t seis_plan_device_bytes(const seis_plan_t* plan);

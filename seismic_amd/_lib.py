@@ -59,6 +59,7 @@ EXPORTS = {
     "seis_plan_set_config": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32]),
     "seis_plan_get_config": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                        POINTER(c_int32)]),
+    "seis_plan_set_stream_threshold": (c_int32, [c_void_p, c_int32]),
     "seis_label_flat": (c_int32, [POINTER(c_int32), POINTER(c_int32), c_int32, c_void_p, c_double, c_void_p,
                                   c_void_p, c_void_p]),
     "seis_lbfgs_direction": (c_int32, [POINTER(LbfgsReq), c_int32, c_void_p]),

@@ -3,9 +3,9 @@
 Each kernel kind (forward, forward recording the history, fused tangent,
 adjoint, Born on the saved forward) runs an rx x rz cell window per workgroup, advances k time steps
 per launch and deals window rows to wavefronts either as consecutive strips
-or round-robin (libseis ``seis_plan_set_config``); the plain forward, the
-Born pass and the adjoint can also march rx-column strips down rz-row
-segments (strips = 2; the tuner does not try the streamed adjoint).  The best choice depends on how many cells one
+or round-robin (libseis ``seis_plan_set_config``); every kind can also
+march rx-column strips down rz-row segments (strips = 2; the tuner does not
+try the streamed adjoint or saving forward).  The best choice depends on how many cells one
 time step covers (batch size x model size: a single 70x70 model leaves most
 of the GPU idle with large windows) and on whether the forward writes the full
 history.  ``tune`` times every candidate on a live plan; ``apply`` looks a
@@ -46,11 +46,15 @@ CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WI
 # whole columns), k levels
 CANDIDATES["forward"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 40, 80, 160) for k in (3, 4)]
 CANDIDATES["born"] += [(rx, rz, k, 2) for rx in (64, 128) for rz in (0, 80, 160) for k in (3, 4)]
-# the row-streamed adjoint (strips = 2) is not a default candidate: it wins
-# only with hundreds of strips active at once and falls far behind the tiled
-# kernel when a plan runs few of its models (serving, chained schedules),
-# which a table keyed by the plan's size cannot tell apart
+# row-streamed fused forward + tangent: 128- or 192-column strips
+CANDIDATES["tangent"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 80, 160) for k in (3, 4)]
+# the row-streamed adjoint and saving forward (strips = 2) are not default
+# candidates: timed alone on a full batch they win (adjoint by ~10 %, saving
+# forward by ~20 %), but inside served and chained inversions, where the
+# active models change from launch to launch, the tiled kernels keep those
+# workloads faster, which a table keyed by the plan's size cannot see
 STREAM_ADJOINT = [(rx, rz, k, 2) for rx in (64, 128) for rz in (0, 80, 160) for k in (4, 6)]
+STREAM_RECORD = [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 80, 160) for k in (3, 4)]
 # tuning order: the adjoint and Born passes run after the forward that
 # records their history
 KINDS = ("forward", "record", "tangent", "adjoint", "born")

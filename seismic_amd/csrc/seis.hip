@@ -54,6 +54,9 @@ constexpr float kC3 = -1.0f / 12.0f;  // core/seis_forward2.py:81
 constexpr int NTHR = 256;        // 4 wavefronts per workgroup
 constexpr int NXCD = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr int kKzPad = 64;       // rows of periodic apron on each side of a model's 1 - kappa row table
+// Work lists: one per kernel kind, plus an internal one with a fixed tiled
+// window that runs the launches of streamed passes shorter than their k.
+constexpr int kTail = SEIS_KIND_COUNT, kLists = SEIS_KIND_COUNT + 1;
 
 // Per-wavefield state regions in the float arena.
 enum Region {
@@ -475,29 +478,6 @@ __device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float ksca
   return 1.0f - kscale * (float)(j * j);
 }
 
-// KL forward time steps (update_p :428-462, lapg :381-421, source injection
-// fused) over a column strip of NT padded columns, one per thread, marched
-// down the rows of a segment in a software pipeline.  Pipeline step t takes
-// in window row t of p[n0], q[n0], alpha and the row's 1 - kappa (requested
-// SPD steps earlier) as level 0, and level l = 1..KL computes window row
-// t - 2l of p[n0+l], q[n0+l] from level l-1; levels run in increasing order,
-// so the lower neighbour two rows down is the value level l-1 produced in
-// the same step.  Each thread keeps the last rows of its own column of every
-// level in registers (centre, vertical neighbours, q) and reads the
-// horizontal neighbours from a three-row LDS ring per level, with one
-// barrier per pipeline step; the (alpha, 1 - kappa) pairs of its column's
-// rows in flight sit in a private LDS ring.  1 - kappa outside the
-// left/right damping strips, one value per padded row, comes from the
-// model's row table.  The window is the segment's rows (TileEnt.pad of them)
-// plus 2*KL rows above and below and 2*KL columns either side, wrapped
-// periodically like k_fwd's tiles; cells outside the valid band compute
-// values no committed cell depends on.  Blocks of SUN pipeline steps in
-// which some level meets the source row or the receiver row run a separate
-// copy of the block, so the others carry no per-cell source or receiver
-// work; receiver samples are kept in registers and stored after the march.  Same stencil order and
-// fused multiply-adds as k_fwd, so wavefields and seismograms are
-// bit-identical to it.  Launches always advance KL steps (a shorter last
-// launch runs k_fwd).
 // Raw buffer resource over device memory starting at p.  A load or store
 // through it takes the thread's byte offset in a vector register and a
 // wave-uniform byte offset in a scalar register (buffer_load_dword ... offen),
@@ -522,7 +502,35 @@ __device__ __forceinline__ unsigned next_row(unsigned o, unsigned rowb, unsigned
   return o >= end ? o - slot : o;
 }
 
-template <int NT, int KL>
+// KL forward time steps (update_p :428-462, lapg :381-421, source injection
+// fused) over a column strip of NT padded columns, one per thread, marched
+// down the rows of a segment in a software pipeline.  Pipeline step t takes
+// in window row t of p[n0], q[n0], alpha and the row's 1 - kappa (requested
+// SPD steps earlier) as level 0, and level l = 1..KL computes window row
+// t - 2l of p[n0+l], q[n0+l] from level l-1; levels run in increasing order,
+// so the lower neighbour two rows down is the value level l-1 produced in
+// the same step.  Each thread keeps the last rows of its own column of every
+// level in registers (centre, vertical neighbours, q) and reads the
+// horizontal neighbours from a three-row LDS ring per level, with one
+// barrier per pipeline step; the (alpha, 1 - kappa) pairs of its column's
+// rows in flight sit in a private LDS ring.  1 - kappa outside the
+// left/right damping strips, one value per padded row, comes from the
+// model's row table.  The window is the segment's rows (TileEnt.pad of them)
+// plus 2*KL rows above and below and 2*KL columns either side, wrapped
+// periodically like k_fwd's tiles; cells outside the valid band compute
+// values no committed cell depends on.  Blocks of SUN pipeline steps in
+// which some level meets the source row or the receiver row run a separate
+// copy of the block, so the others carry no per-cell source or receiver
+// work; receiver samples are kept in registers and stored after the march.
+// With TAN every level also advances the Born recursion (update_p_diff
+// :470-509) on (dp, dq), from its own LDS ring and registers, with the
+// source term dalpha*Lf(p) taken as (dalpha/alpha) times alpha*Lf(p)
+// recovered from the level's q and q' as k_fwd does; with HIST each level
+// stores its q of the committed rows into the history (saving forward).
+// Same stencil order and fused multiply-adds as k_fwd, so wavefields,
+// seismograms and histories are bit-identical to it.  Launches always
+// advance KL steps (shorter ones run k_fwd).
+template <int NT, int KL, bool TAN, bool HIST>
 __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
   static_assert(2 * KL + SPD < SUN && SUN % 6 == 0 && SUN % SPD == 0, "pipeline rings too short");
   constexpr int H = 2 * KL;
@@ -534,6 +542,8 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
   constexpr int RS = 3 * KL + 1;
   __shared__ float RG[(NT + 4) * RS];
   __shared__ float2 AK[SUN][NT];  // (alpha, 1 - kappa) of window row u at [u % SUN], own column
+  __shared__ float DG[TAN ? (NT + 4) * RS : 1];      // dp rings
+  __shared__ float2 DK[TAN ? SUN : 1][TAN ? NT : 1];  // (dalpha/alpha, dkappa), like AK
   const unsigned c = threadIdx.x;
   const int ntiles = *a.n_tiles;
   const int n0 = a.n0, nt = a.nt;
@@ -544,10 +554,11 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const ModelDev& m = a.models[w.model];
     const int NZ = w.NZ, NX = w.NX, nz = m.nz, nx = m.nx, nbc = m.nbc;
     const unsigned rowb = (unsigned)NX * 4u, P4 = (unsigned)NZ * rowb;   // bytes of a row, of a slot
-    // the wavefield's state through one buffer resource, alpha through its
-    // own; byte offsets of the slots this launch reads and writes
+    // the wavefield's state through one buffer resource, alpha (and dalpha)
+    // through their own; byte offsets of the slots this launch reads and writes
     const brsrc_t VW = brsrc(a.arena + w.off[R_HIST]);
     const brsrc_t VA = brsrc(a.alpha + m.cell_off);
+    const brsrc_t VD = brsrc(TAN ? a.dalpha + m.cell_off : a.alpha);
     auto reg_off = [&](int r, int slot) {
       return (unsigned)(w.off[r] - w.off[R_HIST]) * 4u + (unsigned)slot * P4;
     };
@@ -555,9 +566,15 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const unsigned o_po = reg_off(R_PP, a.p_out_slot), o_qo = reg_off(R_Q, a.q_out_slot);
     const bool ck = a.ck_slot >= 0;
     const unsigned o_cp = ck ? reg_off(R_CKP, a.ck_slot) : 0u, o_cq = ck ? reg_off(R_CKQ, a.ck_slot) : 0u;
+    const unsigned o_dpi = TAN ? reg_off(R_DP, a.dp_in_slot) : 0u, o_dqi = TAN ? reg_off(R_DQ, a.dq_in_slot) : 0u;
+    const unsigned o_dpo = TAN ? reg_off(R_DP, a.dp_out_slot) : 0u, o_dqo = TAN ? reg_off(R_DQ, a.dq_out_slot) : 0u;
+    const unsigned o_h = HIST ? reg_off(R_HIST, a.hist_slot0) : 0u;   // q[n0+1]; q[n0+l] one slot per level on
     float* seis = a.write_seis ? a.seis + w.seis_off : nullptr;
+    float* dseis = TAN && a.write_seis ? a.dseis + w.seis_off : nullptr;
     const float kscale = a.params[w.model].kscale;
+    const float dkscale = TAN ? a.params[w.model].dkscale : 0.f;
     const float amp = a.wparam[te.w].amp;
+    const float amp_d = TAN ? a.wparam[te.w].amp_d : 0.f;
     const int n_rcv = m.n_rcv, rcv_row = m.rcv_row;
     const float* kzt = a.alpha + m.kz_off + kKzPad;   // kzt[z]: padded row z, -kKzPad <= z < NZ + kKzPad
     const int IZ = te.pad;              // rows of the segment (all inside the grid)
@@ -568,7 +585,8 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const bool colint = (int)c >= H && (int)c < NT - H && ox + (int)c < NX;
     const bool isx = (int)gx < nbc || (int)gx >= nx + nbc;
     const int jx = damp_j(0, gx, nz, nx, nbc);
-    const float kx1 = 1.0f - kscale * (float)(jx * jx);
+    const float jjx = (float)(jx * jx);
+    const float kx1 = 1.0f - kscale * jjx;
     const int rc = rcv_index(gx, m);
     const int srow = w.src_cell / NX;
     const bool srcc = (int)gx == w.src_cell - srow * NX;
@@ -577,20 +595,32 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
     const int su0 = wrapi(srow - oz, NZ), su1 = su0 + NZ;
     const int ruz = rcv_row - te.z0;
     const int rui = ruz >= 0 && ruz < IZ ? ruz + H : -(1 << 20);
-    float srcv[KL], rec[KL];
+    float srcv[KL], srcd[TAN ? KL : 1], rec[KL], drec[TAN ? KL : 1];
 #pragma unroll
     for (int l = 1; l <= KL; ++l) {
       const int it = n0 + l - 2;
-      srcv[l - 1] = amp * ((it >= 0 && it < nt) ? a.wavelet[it] : 0.f);
+      const float wl = (it >= 0 && it < nt) ? a.wavelet[it] : 0.f;
+      srcv[l - 1] = amp * wl;
       rec[l - 1] = 0.f;
+      if constexpr (TAN) {
+        srcd[l - 1] = amp_d * wl;
+        drec[l - 1] = 0.f;
+      }
     }
     float pr[KL][6], qr[KL][3], pfp[SPD], pfq[SPD], pfa[SPD], pfk[SPD];
+    float dpr[TAN ? KL : 1][6], dqr[TAN ? KL : 1][3], pfdp[TAN ? SPD : 1], pfdq[TAN ? SPD : 1], pfd[TAN ? SPD : 1];
 #pragma unroll
     for (int l = 0; l < KL; ++l) {
 #pragma unroll
       for (int s = 0; s < 6; ++s) pr[l][s] = 0.f;
 #pragma unroll
       for (int s = 0; s < 3; ++s) qr[l][s] = 0.f;
+      if constexpr (TAN) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) dpr[l][s] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) dqr[l][s] = 0.f;
+      }
     }
 #pragma unroll
     for (int d = 0; d < SPD; ++d) {
@@ -599,10 +629,16 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
       pfq[d] = bld(VW, vx, o_qi + ro);
       pfa[d] = bld(VA, vx, ro);
       pfk[d] = kzt[oz + d];
+      if constexpr (TAN) {
+        pfdp[d] = bld(VW, vx, o_dpi + ro);
+        pfdq[d] = bld(VW, vx, o_dqi + ro);
+        pfd[d] = bld(VD, vx, ro);
+      }
     }
     unsigned rq = (unsigned)wrapi(oz + SPD, NZ) * rowb;   // row requested next
     // (oz + u) * rowb of level KL's row u = t - 2 KL (used on committed rows only)
     unsigned so = (unsigned)(te.z0 - 2 * H) * rowb;
+    int gzc = wrapi(oz, NZ);            // padded row of window row t (dkappa of the row)
     __syncthreads();
     // one pipeline step; SPECIAL: some level meets the source or receiver row
     auto body = [&](auto ic, auto sc, const int t) {
@@ -614,6 +650,14 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
         pr[0][i % 6] = p0;
         qr[0][i % 3] = pfq[i % SPD];
         AK[i][c] = make_float2(pfa[i % SPD], isx ? kx1 : pfk[i % SPD]);
+        if constexpr (TAN) {
+          const float d0 = pfdp[i % SPD];
+          DG[(c + 2) * RS + i % 3] = d0;
+          dpr[0][i % 6] = d0;
+          dqr[0][i % 3] = pfdq[i % SPD];
+          const int jz = damp_j(gzc, nbc, nz, nx, nbc);
+          DK[i][c] = make_float2(pfd[i % SPD], dkscale * (isx ? jjx : (float)(jz * jz)));
+        }
       }
       static_for<1, KL + 1>([&](auto lc) {
         constexpr int l = decltype(lc)::value;
@@ -630,7 +674,8 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
         t2 = t2 + (pr[l - 1][(i - 2 * l - 2 + B) % 6] - pc);
         const float L = fmaf(kC2, t1, kC3 * t2);
         const float2 ak = AK[sr][c];
-        float qn = fmaf(ak.y, qr[l - 1][s3], ak.x * L);
+        const float qo = qr[l - 1][s3];
+        float qn = fmaf(ak.y, qo, ak.x * L);
         if constexpr (SPECIAL) {
           if (u == su0 || u == su1) qn = srcc ? qn + srcv[l - 1] : qn;
         }
@@ -638,11 +683,50 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
         if constexpr (SPECIAL) {
           if (u == rui) rec[l - 1] = p1;
         }
+        float dqn = 0.f, dp1 = 0.f;
+        if constexpr (TAN) {
+          // dq' = (1-kappa) dq + alpha Lf(dp) + dalpha Lf(p) - dkappa q, with
+          // dalpha Lf(p) = (dalpha/alpha) (alpha Lf(p)) and alpha Lf(p)
+          // recovered from q and q' (k_fwd, k_bns)
+          const float dc = dpr[l - 1][s6];
+          const float* D = DG + c * RS + (l - 1) * 3 + s3;
+          float d1 = (D[3 * RS] - dc) + (D[RS] - dc);
+          d1 = d1 + (dpr[l - 1][(i - 2 * l + 1 + B) % 6] - dc);
+          d1 = d1 + (dpr[l - 1][(i - 2 * l - 1 + B) % 6] - dc);
+          float d2 = (D[4 * RS] - dc) + (D[0] - dc);
+          d2 = d2 + (dpr[l - 1][(i - 2 * l + 2 + B) % 6] - dc);
+          d2 = d2 + (dpr[l - 1][(i - 2 * l - 2 + B) % 6] - dc);
+          const float Ld = fmaf(kC2, d1, kC3 * d2);
+          const float2 dk = DK[sr][c];
+          float aL = fmaf(-ak.y, qo, qn);
+          if constexpr (SPECIAL) {
+            if (u == su0 || u == su1) aL = srcc ? aL - srcv[l - 1] : aL;
+          }
+          dqn = fmaf(ak.y, dqr[l - 1][s3], ak.x * Ld);
+          dqn = fmaf(dk.x, aL, dqn);
+          dqn = fmaf(-dk.y, qo, dqn);
+          if constexpr (SPECIAL) {
+            if (u == su0 || u == su1) dqn = srcc ? dqn + srcd[l - 1] : dqn;
+          }
+          dp1 = dc + dqn;
+          if constexpr (SPECIAL) {
+            if (u == rui) drec[l - 1] = dp1;
+          }
+        }
+        if constexpr (HIST) {   // q[n0+l] of committed rows into history slot hist_slot0 + l - 1
+          if (u >= H && u < H + IZ && colint)
+            bst(VW, vx, o_h + (unsigned)(l - 1) * P4 + so + (unsigned)(2 * (KL - l)) * rowb, qn);
+        }
         if constexpr (l < KL) {
           RG[(c + 2) * RS + l * 3 + s3] = p1;
           pr[l][s6] = p1;
           qr[l][s3] = qn;
-        } else if (u >= H && u < H + IZ) {  // committed row of p[n0+KL], q[n0+KL]
+          if constexpr (TAN) {
+            DG[(c + 2) * RS + l * 3 + s3] = dp1;
+            dpr[l][s6] = dp1;
+            dqr[l][s3] = dqn;
+          }
+        } else if (u >= H && u < H + IZ) {  // committed row of p[n0+KL], q[n0+KL] (and dp, dq)
           if (colint) {
             bst(VW, vx, o_po + so, p1);
             bst(VW, vx, o_qo + so, qn);
@@ -650,15 +734,25 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
               bst(VW, vx, o_cp + so, p1);
               bst(VW, vx, o_cq + so, qn);
             }
+            if constexpr (TAN) {
+              bst(VW, vx, o_dpo + so, dp1);
+              bst(VW, vx, o_dqo + so, dqn);
+            }
           }
         }
       });
       so += rowb;
+      if constexpr (TAN) gzc = gzc + 1 == NZ ? 0 : gzc + 1;
       {  // request of window row t + SPD
         pfp[i % SPD] = bld(VW, vx, o_pi + rq);
         pfq[i % SPD] = bld(VW, vx, o_qi + rq);
         pfa[i % SPD] = bld(VA, vx, rq);
         pfk[i % SPD] = kzt[oz + t + SPD];             // oz + t + SPD in [-H, NZ + H + SUN)
+        if constexpr (TAN) {
+          pfdp[i % SPD] = bld(VW, vx, o_dpi + rq);
+          pfdq[i % SPD] = bld(VW, vx, o_dqi + rq);
+          pfd[i % SPD] = bld(VD, vx, rq);
+        }
         rq = next_row(rq, rowb, P4, P4);
       }
       __syncthreads();
@@ -675,7 +769,10 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
 #pragma unroll
       for (int l = 1; l <= KL; ++l) {
         const int it = n0 + l - 2;
-        if (it >= 0 && it < nt) seis[(int64_t)it * n_rcv + rc] = rec[l - 1];
+        if (it >= 0 && it < nt) {
+          seis[(int64_t)it * n_rcv + rc] = rec[l - 1];
+          if constexpr (TAN) dseis[(int64_t)it * n_rcv + rc] = drec[l - 1];
+        }
       }
     }
   }
@@ -2016,11 +2113,11 @@ __global__ __launch_bounds__(NTHR) void k_copy_active(const ModelDev* models, co
 // Activity mask for k_compact, passed by value: one bit per model.
 constexpr int kMaskWords = 768;   // 24576 models
 struct CompactArgs {
-  const TileEnt* all[SEIS_KIND_COUNT];
-  TileEnt* tmp[SEIS_KIND_COUNT];
-  TileEnt* out[SEIS_KIND_COUNT];
-  int* count[SEIS_KIND_COUNT];
-  int n_all[SEIS_KIND_COUNT];
+  const TileEnt* all[kLists];
+  TileEnt* tmp[kLists];
+  TileEnt* out[kLists];
+  int* count[kLists];
+  int n_all[kLists];
   uint32_t bits[kMaskWords];
 };
 
@@ -2114,6 +2211,9 @@ bool stream_cfg(int kind, const KCfg& c) {
   for (int nt : kStreamNT) nt_ok = nt_ok || c.rx == nt;
   if (!nt_ok || !(c.rz == 0 || c.rz >= 8)) return false;
   if (kind == SEIS_KIND_FORWARD) return c.k >= kStreamKMin && c.k <= kStreamKMax;
+  // saving forward and fused forward + tangent: 128- or 192-column strips
+  if (kind == SEIS_KIND_RECORD || kind == SEIS_KIND_TANGENT)
+    return (c.rx == 128 || c.rx == 192) && c.k >= kStreamKMin && c.k <= kStreamKMax;
   if (kind == SEIS_KIND_BORN) return c.k >= kStreamBornKMin && c.k <= kStreamBornKMax;
   if (kind == SEIS_KIND_ADJOINT) return stream_adj_ok(c);
   return false;
@@ -2133,6 +2233,12 @@ bool cfg_ok(int kind, const KCfg& c) {
 
 const KCfg kDefaultCfg[SEIS_KIND_COUNT] = {
     {64, 32, 3, 1}, {64, 32, 3, 0}, {64, 32, 3, 0}, {64, 32, 2, 1}, {64, 32, 3, 1}};
+// Window of the tail list: 64x64 is a shape of every kind (forward and record
+// on 512 threads, the others on 1024), round-robin rows, 3 steps per launch:
+// the config the tuner picks for tiled fused-tangent passes, and close to the
+// single-model picks of the others, since the list also runs whole passes of
+// streamed kinds when few models are active.
+const KCfg kTailCfg = {64, 64, 3, 0};
 
 struct Layout {
   std::vector<ModelDev> models;
@@ -2240,6 +2346,8 @@ struct TileList {
   TileEnt* d_all = nullptr;  // every model's tiles in sequence order (compaction input)
   TileEnt* d_tmp = nullptr;  // compacted sequence (before the XCD interleave)
   int n_all = 0;
+  std::vector<int> per_model;  // tiles of each model
+  int n_active = 0;            // tiles of the active models
 };
 
 }  // namespace
@@ -2261,24 +2369,31 @@ struct seis_plan {
   double* d_fold = nullptr;
   double* d_part = nullptr;
   int* d_mask = nullptr;      // activity mask staged for the device-side compaction
-  KCfg cfg[SEIS_KIND_COUNT];
-  TileList tl[SEIS_KIND_COUNT];
+  KCfg cfg[kLists];
+  TileList tl[kLists];
   hipStream_t cap = nullptr;
-  hipGraphExec_t g_fwd[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-  hipGraphExec_t g_adj[3] = {nullptr, nullptr, nullptr};
-  hipGraphExec_t g_born = nullptr;
+  // captured passes, by pass and by whether their streamed kinds ran on the
+  // tail list (few active models): forward [tan][save][tail], adjoint
+  // [injection][adjoint on tail + 2 * record on tail], Born [tail]
+  hipGraphExec_t g_fwd[2][2][2] = {};
+  hipGraphExec_t g_adj[3][4] = {};
+  hipGraphExec_t g_born[2] = {};
   std::vector<char> hist_ok;  // per model: a saving forward has filled R_HIST
+  int stream_min = -1;        // active tiles a streamed config needs (< 0: the CU count)
 };
 
 namespace {
 
 void drop_graphs(seis_plan* p) {
   for (auto& a : p->g_fwd)
+    for (auto& b : a)
+      for (auto& g : b)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  for (auto& a : p->g_adj)
     for (auto& g : a)
       if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-  for (auto& g : p->g_adj)
+  for (auto& g : p->g_born)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-  if (p->g_born) { (void)hipGraphExecDestroy(p->g_born); p->g_born = nullptr; }
 }
 
 void plan_free(seis_plan* p) {
@@ -2327,10 +2442,24 @@ const void* fwd_ptr(const KCfg& c) {
 
 template <int NT>
 const void* fws_ptr_nt(int k) {
-  return k == 3 ? (const void*)k_fws<NT, 3> : (const void*)k_fws<NT, 4>;
+  return k == 3 ? (const void*)k_fws<NT, 3, false, false> : (const void*)k_fws<NT, 4, false, false>;
 }
 
-const void* fws_ptr(const KCfg& c) {
+// the streamed forward variants: plain (64-256 columns), saving forward
+// (history) and fused forward + tangent (with or without history; 128 or 192)
+template <int NT, bool TAN, bool HIST>
+const void* fws_ptr_th(int k) {
+  return k == 3 ? (const void*)k_fws<NT, 3, TAN, HIST> : (const void*)k_fws<NT, 4, TAN, HIST>;
+}
+
+template <bool TAN, bool HIST>
+const void* fws_ptr_w(const KCfg& c) {
+  return c.rx == 128 ? fws_ptr_th<128, TAN, HIST>(c.k) : fws_ptr_th<192, TAN, HIST>(c.k);
+}
+
+const void* fws_ptr(const KCfg& c, bool tan, bool hist) {
+  if (tan) return hist ? fws_ptr_w<true, true>(c) : fws_ptr_w<true, false>(c);
+  if (hist) return fws_ptr_w<false, true>(c);
   if (c.rx == 64) return fws_ptr_nt<64>(c.k);
   if (c.rx == 128) return fws_ptr_nt<128>(c.k);
   if (c.rx == 192) return fws_ptr_nt<192>(c.k);
@@ -2357,7 +2486,10 @@ const void* ads_ptr_nt(int k) {
 const void* ads_ptr(const KCfg& c) { return c.rx == 64 ? ads_ptr_nt<64>(c.k) : ads_ptr_nt<128>(c.k); }
 
 const void* kernel_ptr(int kind, const KCfg& c) {
-  if (c.strips == 2) return kind == SEIS_KIND_BORN ? bns_ptr(c) : kind == SEIS_KIND_ADJOINT ? ads_ptr(c) : fws_ptr(c);
+  if (c.strips == 2)
+    return kind == SEIS_KIND_BORN ? bns_ptr(c)
+           : kind == SEIS_KIND_ADJOINT ? ads_ptr(c)
+                                       : fws_ptr(c, kind == SEIS_KIND_TANGENT, kind == SEIS_KIND_RECORD);
 #define PTR_F(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, false>(c);
 #define PTR_R(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, false, true>(c);
 #define PTR_T(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) return fwd_ptr<RX, RZ, NT, true, false>(c);
@@ -2392,16 +2524,43 @@ void launch_fwd_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
 
 template <int NT>
 void launch_fws_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
-  if (c.k == 3) hipLaunchKernelGGL((k_fws<NT, 3>), dim3(grid), dim3(NT), 0, s, a);
-  else hipLaunchKernelGGL((k_fws<NT, 4>), dim3(grid), dim3(NT), 0, s, a);
+  if (c.k == 3) hipLaunchKernelGGL((k_fws<NT, 3, false, false>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_fws<NT, 4, false, false>), dim3(grid), dim3(NT), 0, s, a);
 }
 
+template <int NT, bool TAN, bool HIST>
+void launch_fws_th(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
+  if (c.k == 3) hipLaunchKernelGGL((k_fws<NT, 3, TAN, HIST>), dim3(grid), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_fws<NT, 4, TAN, HIST>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+template <bool TAN, bool HIST>
+void launch_fws_w(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
+  if (c.rx == 128) launch_fws_th<128, TAN, HIST>(c, grid, s, a);
+  else launch_fws_th<192, TAN, HIST>(c, grid, s, a);
+}
+
+// One launch of the forward (with tan: the fused forward + tangent) on
+// config c; a launch that stores the history (hist_slot0 >= 0) runs the
+// history variant.  Streamed configs of the kinds that take them: plain
+// forward 64-256 columns, history or tangent 128 or 192 (stream_cfg).
 void launch_fwd(const KCfg& c, bool tan, int grid, hipStream_t s, const FwdArgs& a) {
   if (c.strips == 2) {
-    if (c.rx == 64) launch_fws_t<64>(c, grid, s, a);
-    else if (c.rx == 128) launch_fws_t<128>(c, grid, s, a);
-    else if (c.rx == 192) launch_fws_t<192>(c, grid, s, a);
-    else launch_fws_t<256>(c, grid, s, a);
+    const bool hist = a.hist_slot0 >= 0;
+    if (tan) {
+      if (hist) launch_fws_w<true, true>(c, grid, s, a);
+      else launch_fws_w<true, false>(c, grid, s, a);
+    } else if (hist) {
+      launch_fws_w<false, true>(c, grid, s, a);
+    } else if (c.rx == 64) {
+      launch_fws_t<64>(c, grid, s, a);
+    } else if (c.rx == 128) {
+      launch_fws_t<128>(c, grid, s, a);
+    } else if (c.rx == 192) {
+      launch_fws_t<192>(c, grid, s, a);
+    } else {
+      launch_fws_t<256>(c, grid, s, a);
+    }
     return;
   }
 #define LF(RX, RZ, NT) if (c.rx == RX && c.rz == RZ) { launch_fwd_t<RX, RZ, NT, false>(c, grid, s, a); return; }
@@ -2492,6 +2651,23 @@ std::vector<TileEnt> make_tiles(const Layout& L, const KCfg& c, bool all_active)
   return out;
 }
 
+// Active tiles of every work list, from the host's activity flags.
+void count_active(seis_plan* p) {
+  for (TileList& t : p->tl) {
+    int n = 0;
+    for (int m = 0; m < p->n_models && m < (int)t.per_model.size(); ++m)
+      if (p->L.models[m].active) n += t.per_model[m];
+    t.n_active = n;
+  }
+}
+
+// A streamed config needs at least one strip segment per CU to fill the
+// chip: with fewer active (few models of a large plan active, as in serving
+// or chained schedules) the pass runs on the tail list's tiled window.
+bool use_tail(const seis_plan* p, int kind) {
+  return p->cfg[kind].strips == 2 && p->tl[kind].n_active < (p->stream_min >= 0 ? p->stream_min : p->num_cu);
+}
+
 int upload_tiles(seis_plan* p, int kind) {
   TileList& t = p->tl[kind];
   const std::vector<TileEnt> v = make_tiles(p->L, p->cfg[kind], false);
@@ -2531,9 +2707,15 @@ int setup_kind(seis_plan* p, int kind) {
   }
   t.n_all = (int)all.size();
   if (t.n_all) HIP_TRY(hipMemcpy(t.d_all, all.data(), sizeof(TileEnt) * t.n_all, hipMemcpyHostToDevice));
+  t.per_model.assign(p->n_models, 0);
+  for (const TileEnt& te : all) ++t.per_model[p->L.waves[te.w].model];
+  count_active(p);
   int per_cu = 0;
-  const int nthr = p->cfg[kind].strips == 2 ? p->cfg[kind].rx : shape_threads(kind, p->cfg[kind].rx, p->cfg[kind].rz);
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kind, p->cfg[kind]), nthr, 0));
+  // the tail list's grid: the occupancy of the adjoint kernel on its window
+  // (the fewest blocks per CU of the kernels that run it)
+  const int kk = kind == kTail ? (int)SEIS_KIND_ADJOINT : kind;
+  const int nthr = p->cfg[kind].strips == 2 ? p->cfg[kind].rx : shape_threads(kk, p->cfg[kind].rx, p->cfg[kind].rz);
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(kk, p->cfg[kind]), nthr, 0));
   int gmax = std::max(1, per_cu) * p->num_cu;
   gmax = std::max(NXCD, gmax / NXCD * NXCD);
   t.grid = std::max(NXCD, std::min(cap, gmax));
@@ -2566,7 +2748,8 @@ FwdArgs fwd_args(seis_plan* p, int kind) {
 void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   const Layout& L = p->L;
   const bool hist = save && L.fwd_hist;
-  const int kind = tan ? SEIS_KIND_TANGENT : (hist ? SEIS_KIND_RECORD : SEIS_KIND_FORWARD);
+  const int kind0 = tan ? SEIS_KIND_TANGENT : (hist ? SEIS_KIND_RECORD : SEIS_KIND_FORWARD);
+  const int kind = use_tail(p, kind0) ? kTail : kind0;
   const KCfg& c = p->cfg[kind];
   const bool ck = save && !L.full && (p->caps & SEIS_CAP_ADJOINT);
   fill(p, s, R_Q, 0, 1);
@@ -2582,9 +2765,9 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
   FwdArgs a = fwd_args(p, kind);
   a.write_seis = 1;
   // a streamed forward always advances c.k steps; shorter launches (the last
-  // one, or up to a checkpoint) run k_fwd with the record kind's window
-  const KCfg& cr = p->cfg[SEIS_KIND_RECORD];
-  FwdArgs ar = fwd_args(p, SEIS_KIND_RECORD);
+  // one, or up to a checkpoint) run k_fwd on the tail list
+  const KCfg& cr = p->cfg[kTail];
+  FwdArgs ar = fwd_args(p, kTail);
   int n = 0, li = 0;
   while (n < p->nt + 1) {
     int K = std::min(c.k, p->nt + 1 - n);
@@ -2599,7 +2782,7 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
     f.p_out_slot = f.q_out_slot = f.dp_out_slot = f.dq_out_slot = (li + 1) & 1;
     f.hist_slot0 = hist ? n + 1 : -1;   // step n stores q[n+1] at slot n+1
     f.ck_slot = (ck && (n + K) % L.Kc == 0 && (n + K) / L.Kc < L.J) ? (n + K) / L.Kc : -1;
-    if (tail) launch_fwd(cr, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
+    if (tail) launch_fwd(cr, tan, p->tl[kTail].grid, s, f);
     else launch_fwd(c, tan, p->tl[kind].grid, s, f);
     n += K;
     ++li;
@@ -2611,17 +2794,17 @@ void enqueue_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
 void enqueue_born(seis_plan* p, hipStream_t s) {
   fill(p, s, R_DP, 0, 1);
   fill(p, s, R_DQ, 0, 1);
-  const KCfg& c = p->cfg[SEIS_KIND_BORN];
+  const int kb = use_tail(p, SEIS_KIND_BORN) ? kTail : (int)SEIS_KIND_BORN;
+  const KCfg& c = p->cfg[kb];
   BornArgs a{};
-  a.tiles = p->tl[SEIS_KIND_BORN].d; a.n_tiles = p->tl[SEIS_KIND_BORN].d_count;
+  a.tiles = p->tl[kb].d; a.n_tiles = p->tl[kb].d_count;
   a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params; a.wparam = p->d_wparam;
   a.alpha = p->d_alpha; a.dalpha = p->d_dalpha; a.wavelet = p->d_wavelet;
   a.arena = p->d_arena; a.dseis = p->d_dseis; a.nt = p->nt;
   a.Kt = c.k;
   // a streamed Born pass always advances c.k steps; the shorter last launch
-  // runs k_born with the fused tangent kind's window and work list (that kind
-  // is always tiled, and its windows are Born shapes too)
-  const KCfg& ct = p->cfg[SEIS_KIND_TANGENT];
+  // runs k_born on the tail list
+  const KCfg& ct = p->cfg[kTail];
   int n = 0, li = 0;
   while (n < p->nt + 1) {
     int K = std::min(c.k, p->nt + 1 - n);
@@ -2633,12 +2816,12 @@ void enqueue_born(seis_plan* p, hipStream_t s) {
     a.dp_out_slot = a.dq_out_slot = (li + 1) & 1;
     if (tail) {
       BornArgs at = a;
-      at.tiles = p->tl[SEIS_KIND_TANGENT].d;
-      at.n_tiles = p->tl[SEIS_KIND_TANGENT].d_count;
+      at.tiles = p->tl[kTail].d;
+      at.n_tiles = p->tl[kTail].d_count;
       at.Kt = ct.k;
-      launch_born(ct, p->tl[SEIS_KIND_TANGENT].grid, s, at);
+      launch_born(ct, p->tl[kTail].grid, s, at);
     } else {
-      launch_born(c, p->tl[SEIS_KIND_BORN].grid, s, a);
+      launch_born(c, p->tl[kb].grid, s, a);
     }
     n += K;
     ++li;
@@ -2655,23 +2838,23 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
   fill(p, s, R_MU, 0, 1);
   fill(p, s, R_GA, 0, 1);
   fill(p, s, R_GK, 0, 1);
-  const KCfg& ca = p->cfg[SEIS_KIND_ADJOINT];
+  const int ka = use_tail(p, SEIS_KIND_ADJOINT) ? kTail : (int)SEIS_KIND_ADJOINT;
+  const KCfg& ca = p->cfg[ka];
   AdjArgs a{};
-  a.tiles = p->tl[SEIS_KIND_ADJOINT].d; a.n_tiles = p->tl[SEIS_KIND_ADJOINT].d_count;
+  a.tiles = p->tl[ka].d; a.n_tiles = p->tl[ka].d_count;
   a.waves = p->d_waves; a.models = p->d_models; a.params = p->d_params; a.wparam = p->d_wparam;
   a.alpha = p->d_alpha; a.wavelet = p->d_wavelet; a.arena = p->d_arena; a.nt = p->nt;
   a.Kt = ca.k;
   if (kind == 0) { a.inj_a = p->d_seis; a.inj_b = p->d_dd; }
   else if (kind == 1) { a.inj_a = p->d_dd; a.inj_b = nullptr; }
   else { a.inj_a = p->d_dseis; a.inj_b = nullptr; }
-  const int grid = p->tl[SEIS_KIND_ADJOINT].grid;
+  const int grid = p->tl[ka].grid;
   // a streamed adjoint always advances ca.k steps; shorter launches run k_adj
-  // with the fused tangent kind's window and work list (always tiled; its
-  // windows are adjoint shapes too)
-  const KCfg& ct = p->cfg[SEIS_KIND_TANGENT];
+  // on the tail list
+  const KCfg& ct = p->cfg[kTail];
   AdjArgs at = a;
-  at.tiles = p->tl[SEIS_KIND_TANGENT].d;
-  at.n_tiles = p->tl[SEIS_KIND_TANGENT].d_count;
+  at.tiles = p->tl[kTail].d;
+  at.n_tiles = p->tl[kTail].d_count;
   at.Kt = ct.k;
   int li = 0;
   auto run_range = [&](int m_hi, int m_lo, int base) {
@@ -2686,7 +2869,7 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
       f.lam_in_slot = f.mu_in_slot = li & 1;
       f.lam_out_slot = f.mu_out_slot = (li + 1) & 1;
       f.h_base = base;
-      if (tail) launch_adj(ct, p->tl[SEIS_KIND_TANGENT].grid, s, f);
+      if (tail) launch_adj(ct, p->tl[kTail].grid, s, f);
       else launch_adj(ca, grid, s, f);
       M0 -= K;
       ++li;
@@ -2696,9 +2879,12 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
     run_range(p->nt + 1, 2, 0);
     return;
   }
-  const KCfg& cf = p->cfg[SEIS_KIND_RECORD];
-  FwdArgs f = fwd_args(p, SEIS_KIND_RECORD);
+  const int kr = use_tail(p, SEIS_KIND_RECORD) ? kTail : (int)SEIS_KIND_RECORD;
+  const KCfg& cf = p->cfg[kr];
+  FwdArgs f = fwd_args(p, kr);
   f.write_seis = 0;
+  FwdArgs ft = fwd_args(p, kTail);   // launches shorter than a streamed record config's k
+  ft.write_seis = 0;
   for (int j = L.J - 1; j >= 0; --j) {
     const int lo = j * L.Kc;
     const int hi = std::min(lo + L.Kc, p->nt);  // largest n whose pair the segment needs
@@ -2706,13 +2892,17 @@ void enqueue_adjoint(seis_plan* p, hipStream_t s, int kind) {
                        p->d_models, p->n_waves, p->d_arena, j);
     int n = lo, fi = 0;
     while (n <= hi) {
-      const int K = std::min(cf.k, hi + 1 - n);
-      f.n0 = n;
-      f.K = K;
-      f.p_in_slot = f.q_in_slot = fi & 1;
-      f.p_out_slot = f.q_out_slot = (fi + 1) & 1;
-      f.hist_slot0 = n + 1 - lo;        // q[n+1] at segment slot n+1-lo
-      launch_fwd(cf, false, p->tl[SEIS_KIND_RECORD].grid, s, f);
+      int K = std::min(cf.k, hi + 1 - n);
+      const bool tail = cf.strips == 2 && K < cf.k;
+      if (tail) K = std::min(K, p->cfg[kTail].k);
+      FwdArgs& g = tail ? ft : f;
+      g.n0 = n;
+      g.K = K;
+      g.p_in_slot = g.q_in_slot = fi & 1;
+      g.p_out_slot = g.q_out_slot = (fi + 1) & 1;
+      g.hist_slot0 = n + 1 - lo;        // q[n+1] at segment slot n+1-lo
+      if (tail) launch_fwd(p->cfg[kTail], false, p->tl[kTail].grid, s, g);
+      else launch_fwd(cf, false, p->tl[kr].grid, s, g);
       n += K;
       ++fi;
     }
@@ -2795,6 +2985,31 @@ int copy_out(seis_plan* p, hipStream_t s, const float* src, float* dst) {
     int rc_ = (expr);               \
     if (rc_ != SEIS_OK) return rc_; \
   } while (0)
+
+// Launch a pass's captured graph (captured on first use), keyed by whether
+// its streamed kinds run on the tail list for the current active set.
+int launch_pass_forward(seis_plan* p, hipStream_t s, bool tan, bool save) {
+  const int kind = tan ? SEIS_KIND_TANGENT : (save && p->L.fwd_hist ? SEIS_KIND_RECORD : SEIS_KIND_FORWARD);
+  hipGraphExec_t* g = &p->g_fwd[tan][save][use_tail(p, kind)];
+  TRY(get_graph(p, g, [&](hipStream_t c) { enqueue_forward(p, c, tan, save); }));
+  HIP_TRY(hipGraphLaunch(*g, s));
+  return SEIS_OK;
+}
+
+int launch_pass_adjoint(seis_plan* p, hipStream_t s, int kind) {
+  hipGraphExec_t* g = &p->g_adj[kind][use_tail(p, SEIS_KIND_ADJOINT) + 2 * use_tail(p, SEIS_KIND_RECORD)];
+  TRY(get_graph(p, g, [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
+  HIP_TRY(hipGraphLaunch(*g, s));
+  return SEIS_OK;
+}
+
+int launch_pass_born(seis_plan* p, hipStream_t s) {
+  hipGraphExec_t* g = &p->g_born[use_tail(p, SEIS_KIND_BORN)];
+  TRY(get_graph(p, g, [&](hipStream_t c) { enqueue_born(p, c); }));
+  HIP_TRY(hipGraphLaunch(*g, s));
+  return SEIS_OK;
+}
+
 
 // Grid table of a labelling call: validated sizes, per-grid offsets into the
 // concatenated cells (int64) and (nz, nx) pairs, staged on the device as
@@ -2892,6 +3107,7 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
   p->caps = caps;
   p->hist_ok.assign(n_models, 0);
   for (int k = 0; k < SEIS_KIND_COUNT; ++k) p->cfg[k] = kDefaultCfg[k];
+  p->cfg[kTail] = kTailCfg;
   const bool tan = caps & SEIS_CAP_TANGENT, adj = caps & SEIS_CAP_ADJOINT;
   auto bail = [&](int code) { plan_free(p); return code; };
   int dev = 0;
@@ -2926,7 +3142,7 @@ int seis_plan_create(const seis_model_desc* models, int32_t n_models, const seis
     fail(SEIS_EHIP, std::string("plan setup: ") + hipGetErrorString(e));
     return bail(SEIS_EHIP);
   }
-  for (int k = 0; k < SEIS_KIND_COUNT; ++k)
+  for (int k = 0; k < kLists; ++k)
     if ((rc = setup_kind(p, k))) return bail(rc);
   *plan_out = p;
   return SEIS_OK;
@@ -3040,13 +3256,20 @@ int seis_plan_get_config(const seis_plan_t* p, int32_t kind, int32_t* rx, int32_
   return SEIS_OK;
 }
 
+int seis_plan_set_stream_threshold(seis_plan_t* p, int32_t min_active_tiles) {
+  if (!p) return fail(SEIS_EINVAL, "plan is NULL");
+  p->stream_min = min_active_tiles < 0 ? -1 : min_active_tiles;
+  return SEIS_OK;
+}
+
 int seis_set_active(seis_plan_t* p, const int32_t* active) {
   if (!p || !active) return fail(SEIS_EINVAL, "NULL argument");
   HIP_TRY(hipDeviceSynchronize());
   for (int i = 0; i < p->n_models; ++i) p->L.models[i].active = active[i] ? 1 : 0;
   HIP_TRY(hipMemcpy(p->d_models, p->L.models.data(), sizeof(ModelDev) * p->n_models,
                     hipMemcpyHostToDevice));
-  for (int k = 0; k < SEIS_KIND_COUNT; ++k) TRY(upload_tiles(p, k));
+  for (int k = 0; k < kLists; ++k) TRY(upload_tiles(p, k));
+  count_active(p);
   return SEIS_OK;
 }
 
@@ -3061,7 +3284,8 @@ int seis_set_active_async(seis_plan_t* p, const int32_t* active, seis_stream_t s
     p->L.models[i].active = active[i] ? 1 : 0;
     if (active[i]) a.bits[i >> 5] |= 1u << (i & 31);
   }
-  for (int k = 0; k < SEIS_KIND_COUNT; ++k) {
+  count_active(p);
+  for (int k = 0; k < kLists; ++k) {
     const TileList& t = p->tl[k];
     a.all[k] = t.d_all;
     a.tmp[k] = t.d_tmp;
@@ -3069,7 +3293,7 @@ int seis_set_active_async(seis_plan_t* p, const int32_t* active, seis_stream_t s
     a.count[k] = t.d_count;
     a.n_all[k] = t.n_all;
   }
-  hipLaunchKernelGGL(k_compact, dim3(SEIS_KIND_COUNT), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), a,
+  hipLaunchKernelGGL(k_compact, dim3(kLists), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), a,
                      p->d_waves, p->d_models, p->n_models);
   HIP_TRY(hipGetLastError());
   return SEIS_OK;
@@ -3079,8 +3303,7 @@ int seis_forward(seis_plan_t* p, const float* vec, float* seis, seis_stream_t st
   if (!p || !vec) return fail(SEIS_EINVAL, "NULL argument");
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, nullptr));
-  TRY(get_graph(p, &p->g_fwd[0][0], [&](hipStream_t c) { enqueue_forward(p, c, false, false); }));
-  HIP_TRY(hipGraphLaunch(p->g_fwd[0][0], s));
+  TRY(launch_pass_forward(p, s, false, false));
   return copy_out(p, s, p->d_seis, seis);
 }
 
@@ -3090,8 +3313,7 @@ int seis_forward_misfit(seis_plan_t* p, const float* vec, const float* dd, float
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, nullptr));
   HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
-  TRY(get_graph(p, &p->g_fwd[0][0], [&](hipStream_t c) { enqueue_forward(p, c, false, false); }));
-  HIP_TRY(hipGraphLaunch(p->g_fwd[0][0], s));
+  TRY(launch_pass_forward(p, s, false, false));
   launch_misfit(p, s, misfit);
   HIP_TRY(hipGetLastError());
   return copy_out(p, s, p->d_seis, seis);
@@ -3102,8 +3324,7 @@ int seis_forward_save(seis_plan_t* p, const float* vec, float* seis, seis_stream
   if (!(p->caps & SEIS_CAP_ADJOINT)) return fail(SEIS_ESTATE, "plan lacks SEIS_CAP_ADJOINT");
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, nullptr));
-  TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
-  HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
+  TRY(launch_pass_forward(p, s, false, true));
   mark_saved(p);
   return copy_out(p, s, p->d_seis, seis);
 }
@@ -3117,8 +3338,7 @@ int seis_adjoint(seis_plan_t* p, const float* vec, const float* dd, int32_t on_r
   HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
   if (on_residual && misfit) launch_misfit(p, s, misfit);
   const int kind = on_residual ? 0 : 1;
-  TRY(get_graph(p, &p->g_adj[kind], [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
-  HIP_TRY(hipGraphLaunch(p->g_adj[kind], s));
+  TRY(launch_pass_adjoint(p, s, kind));
   return finish_grad(p, s, vec, grad);
 }
 
@@ -3128,8 +3348,7 @@ int seis_tangent(seis_plan_t* p, const float* vec, const float* dvec, float* sei
   if (!(p->caps & SEIS_CAP_TANGENT)) return fail(SEIS_ESTATE, "plan lacks SEIS_CAP_TANGENT");
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, dvec));
-  TRY(get_graph(p, &p->g_fwd[1][0], [&](hipStream_t c) { enqueue_forward(p, c, true, false); }));
-  HIP_TRY(hipGraphLaunch(p->g_fwd[1][0], s));
+  TRY(launch_pass_forward(p, s, true, false));
   TRY(copy_out(p, s, p->d_seis, seis));
   return copy_out(p, s, p->d_dseis, dseis);
 }
@@ -3141,13 +3360,11 @@ int seis_gradient(seis_plan_t* p, const float* vec, const float* dd, int32_t on_
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, nullptr));
   HIP_TRY(hipMemcpyAsync(p->d_dd, dd, sizeof(float) * p->L.seis, hipMemcpyDeviceToDevice, s));
-  TRY(get_graph(p, &p->g_fwd[0][1], [&](hipStream_t c) { enqueue_forward(p, c, false, true); }));
-  HIP_TRY(hipGraphLaunch(p->g_fwd[0][1], s));
+  TRY(launch_pass_forward(p, s, false, true));
   mark_saved(p);
   if (on_residual && misfit) launch_misfit(p, s, misfit);
   const int kind = on_residual ? 0 : 1;
-  TRY(get_graph(p, &p->g_adj[kind], [&](hipStream_t c) { enqueue_adjoint(p, c, kind); }));
-  HIP_TRY(hipGraphLaunch(p->g_adj[kind], s));
+  TRY(launch_pass_adjoint(p, s, kind));
   TRY(finish_grad(p, s, vec, grad));
   return copy_out(p, s, p->d_seis, seis);
 }
@@ -3159,11 +3376,9 @@ int seis_gn_product(seis_plan_t* p, const float* vec, const float* dvec, float* 
     return fail(SEIS_ESTATE, "plan lacks SEIS_CAP_TANGENT | SEIS_CAP_ADJOINT");
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, dvec));
-  TRY(get_graph(p, &p->g_fwd[1][1], [&](hipStream_t c) { enqueue_forward(p, c, true, true); }));
-  HIP_TRY(hipGraphLaunch(p->g_fwd[1][1], s));
+  TRY(launch_pass_forward(p, s, true, true));
   mark_saved(p);
-  TRY(get_graph(p, &p->g_adj[2], [&](hipStream_t c) { enqueue_adjoint(p, c, 2); }));
-  HIP_TRY(hipGraphLaunch(p->g_adj[2], s));
+  TRY(launch_pass_adjoint(p, s, 2));
   TRY(finish_grad(p, s, vec, grad));
   return copy_out(p, s, p->d_dseis, dseis);
 }
@@ -3174,8 +3389,7 @@ int seis_tangent_saved(seis_plan_t* p, const float* vec, const float* dvec, floa
   TRY(check_saved(p));
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, dvec));
-  TRY(get_graph(p, &p->g_born, [&](hipStream_t c) { enqueue_born(p, c); }));
-  HIP_TRY(hipGraphLaunch(p->g_born, s));
+  TRY(launch_pass_born(p, s));
   return copy_out(p, s, p->d_dseis, dseis);
 }
 
@@ -3185,10 +3399,8 @@ int seis_gn_product_saved(seis_plan_t* p, const float* vec, const float* dvec, f
   TRY(check_saved(p));
   hipStream_t s = (hipStream_t)stream;
   TRY(run_prep(p, s, vec, dvec));
-  TRY(get_graph(p, &p->g_born, [&](hipStream_t c) { enqueue_born(p, c); }));
-  HIP_TRY(hipGraphLaunch(p->g_born, s));
-  TRY(get_graph(p, &p->g_adj[2], [&](hipStream_t c) { enqueue_adjoint(p, c, 2); }));
-  HIP_TRY(hipGraphLaunch(p->g_adj[2], s));
+  TRY(launch_pass_born(p, s));
+  TRY(launch_pass_adjoint(p, s, 2));
   TRY(finish_grad(p, s, vec, grad));
   return copy_out(p, s, p->d_dseis, dseis);
 }

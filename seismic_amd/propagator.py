@@ -139,13 +139,20 @@ class Propagator:
     def set_config(self, kind: str, rx: int, rz: int, k: int, strips: int | None = None):
         """Launch config of one kernel kind: rx x rz LDS window, k steps per
         launch, row ownership (1 = consecutive rows per wavefront, 0 =
-        round-robin; None keeps the current one).  strips = 2 (forward and
-        Born kinds) streams rx-column strips down rz-row segments (rz = 0:
-        whole columns) with k pipelined time levels."""
+        round-robin; None keeps the current one).  strips = 2 streams
+        rx-column strips down rz-row segments (rz = 0: whole columns) with k
+        pipelined time levels."""
         if strips is None:
             strips = self.get_config(kind)[3]
         _lib.check(_lib.lib().seis_plan_set_config(self._h, self.KINDS[kind], rx, rz, k, int(strips)),
                    "seis_plan_set_config")
+
+    def set_stream_threshold(self, min_active_tiles: int):
+        """Streamed configs run a pass only with at least this many active
+        strip segments, else the tiled tail window runs it (same results);
+        -1: the device's CU count (default), 0: always stream."""
+        _lib.check(_lib.lib().seis_plan_set_stream_threshold(self._h, int(min_active_tiles)),
+                   "seis_plan_set_stream_threshold")
 
     def get_config(self, kind: str):
         """(rx, rz, k, strips)."""

@@ -154,6 +154,7 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
     target = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 1.01), og)[0])
     for ckpt in (0, 17):
         P = Propagator([g], ckpt_every=ckpt)
+        P.set_stream_threshold(0)   # stream however few strips are active
         ref = _np(P.forward(torch.tensor(vec)))
         _, _, gref = P.gradient(torch.tensor(vec), target)
         P.set_config("forward", *cfg)
@@ -179,6 +180,7 @@ def test_streamed_forward_bit_identical(small_geoms, cfg):
     target = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 1.01), og)[0])
     for ckpt in (0, 17):
         P = Propagator([g], ckpt_every=ckpt)
+        P.set_stream_threshold(0)   # stream however few strips are active
         ref = _np(P.forward(torch.tensor(vec)))
         mref = float(P.forward_misfit(torch.tensor(vec), target)[0])
         _, _, gref = P.gradient(torch.tensor(vec), target)
@@ -207,22 +209,134 @@ def test_streamed_forward_ragged_batch():
     vec = np.concatenate([synth.to_vec(np.linspace(1800, 3600, gg.nz)[:, None] * np.ones((1, gg.nx))
                                        + rng.uniform(-100, 100, (gg.nz, gg.nx))) for gg in geoms])
     P = Propagator(geoms, ckpt_every=0)
+    P.set_stream_threshold(0)   # stream however few strips are active
     ref = _np(P.forward(torch.tensor(vec)))
     for cfg in STREAM_CONFIGS:
         P.set_config("forward", *cfg)
         assert np.array_equal(_np(P.forward(torch.tensor(vec))), ref), cfg
     Ps = Propagator([Geometry(nz=2, nx=5, nbc=6, nt=150)])
+    Ps.set_stream_threshold(0)   # stream however few strips are active
     Ps.set_config("forward", 128, 0, 3, 2)      # 14 padded rows >= 12
     with pytest.raises(ValueError):
         Ps.set_config("forward", 128, 0, 4, 2)  # 14 < 16
     with pytest.raises(ValueError):
-        P.set_config("record", 128, 0, 4, 2)    # streamed: plain forward, Born and adjoint kinds only
+        P.set_config("record", 128, 0, 5, 2)    # streamed record: 3 or 4 levels
     with pytest.raises(ValueError):
         P.set_config("born", 128, 0, 5, 2)      # streamed Born: 3 or 4 levels
     with pytest.raises(ValueError):
         P.set_config("adjoint", 128, 0, 5, 2)   # streamed adjoint: 4 or 6 levels
     with pytest.raises(ValueError):
         P.set_config("adjoint", 192, 0, 4, 2)   # streamed adjoint: 64- or 128-column strips
+
+
+TAN_STREAM_CONFIGS = [(128, 0, 3, 2), (192, 0, 4, 2), (128, 9, 4, 2), (192, 13, 3, 2), (128, 160, 4, 2)]
+
+
+@pytest.mark.parametrize("cfg", TAN_STREAM_CONFIGS)
+def test_streamed_tangent_and_record_bit_identical(small_geoms, cfg):
+    """The row-streamed fused forward + tangent (tangent kind) and saving
+    forward (record kind) give the tiled kernels' seismograms, tangents,
+    gradients and GN products: the drop-in tangent pass, the GN product that
+    records the history on a full-history plan, the Born pass on a history
+    the streamed saving forward wrote, and on a plan with checkpoints every 17
+    steps the GN product and the gradient whose segment recomputes stream
+    (launches shorter than k run the tiled tail window)."""
+    g, og = small_geoms
+    v, vec = _case(g, "faulted", 25)
+    target = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 1.01), og)[0])
+    rng = np.random.default_rng(12)
+    dm = torch.tensor(np.concatenate([rng.normal(0, 20, g.nz * g.nx), [3.0]]))
+    v = torch.tensor(vec)
+    for ckpt in (0, 17):
+        P = Propagator([g], ckpt_every=ckpt)
+        P.set_stream_threshold(0)   # stream however few strips are active
+        s_ref, ds_ref = [_np(x) for x in P.tangent(v, dm)]
+        gp_ref = [_np(x) for x in P.gn_product(v, dm)]
+        _, m_ref, g_ref = P.gradient(v, target)
+        g_ref, m_ref = _np(g_ref), float(m_ref[0])
+        b_ref = _np(P.tangent_saved(v, dm)) if ckpt == 0 else None
+        P.set_config("tangent", *cfg)
+        P.set_config("record", *cfg)
+        assert P.get_config("tangent") == cfg and P.get_config("record") == cfg
+        s_out, ds_out = [_np(x) for x in P.tangent(v, dm)]
+        print(cfg, ckpt, "tangent rel to tiled", rel(ds_out, ds_ref), flush=True)
+        assert np.array_equal(s_out, s_ref) and np.array_equal(ds_out, ds_ref), (cfg, ckpt)
+        gp = [_np(x) for x in P.gn_product(v, dm)]
+        assert np.array_equal(gp[0], gp_ref[0]) and np.array_equal(gp[1], gp_ref[1]), (cfg, ckpt)
+        _, mis, gr = P.gradient(v, target)
+        assert np.array_equal(_np(gr), g_ref) and float(mis[0]) == m_ref, (cfg, ckpt)
+        if ckpt == 0:   # Born on the history the streamed saving forward wrote
+            assert np.array_equal(_np(P.tangent_saved(v, dm)), b_ref), cfg
+    _, ds_o, _ = fd2d.vel_to_seis(vec, og, vec_diff=dm.numpy())
+    assert rel(ds_out, ds_o) < SEIS_TOL
+
+
+def test_streamed_tangent_ragged_batch():
+    """Streamed fused tangent and saving forward over mixed grids, shot counts
+    and source / receiver rows (incl. the surface row) equal the tiled
+    kernels, also with models inactive."""
+    kw = dict(nbc=8, nt=170)
+    geoms = [Geometry(nz=12, nx=16, src_x=(7,), src_z=0, rcv_z=3, rcv_x=(2, 5, 8, 11, 14), **kw),
+             Geometry(nz=20, nx=10, src_x=(0, 4, 9), src_z=2, rcv_z=0, rcv_x=(0, 3, 6, 9), **kw),
+             Geometry(nz=8, nx=30, n_shots=7, src_z=5, rcv_z=7, **kw),
+             Geometry(nz=40, nx=170, n_shots=3, **kw)]
+    rng = np.random.default_rng(13)
+    vs = [np.linspace(1800, 3600, gg.nz)[:, None] * np.ones((1, gg.nx)) + rng.uniform(-100, 100, (gg.nz, gg.nx))
+          for gg in geoms]
+    vec = torch.tensor(np.concatenate([synth.to_vec(v) for v in vs]))
+    dv = torch.tensor(np.concatenate([np.concatenate([rng.normal(0, 15, gg.nz * gg.nx), [2.0]]) for gg in geoms]))
+    P = Propagator(geoms, ckpt_every=0)
+    P.set_stream_threshold(0)   # stream however few strips are active
+    dd = P.forward(vec * 1.01).clone()
+    ref_t = _np(P.tangent(vec, dv)[1])
+    ref_g = _np(P.gradient(vec, dd)[2])
+    for cfg in [(128, 0, 3, 2), (192, 8, 4, 2), (128, 11, 4, 2)]:
+        P.set_config("tangent", *cfg)
+        P.set_config("record", *cfg)
+        assert np.array_equal(_np(P.tangent(vec, dv)[1]), ref_t), cfg
+        assert np.array_equal(_np(P.gradient(vec, dd)[2]), ref_g), cfg
+    P.set_active([True, False, False, True])
+    out = P.tangent(vec, dv)[1]
+    for i in (0, 3):
+        a, b = P.seis_off[i], P.seis_off[i + 1]
+        assert np.array_equal(_np(out[a:b]), ref_t[a:b])
+    with pytest.raises(ValueError):
+        P.set_config("tangent", 64, 0, 3, 2)    # streamed tangent and record: 128- or 192-column strips
+    with pytest.raises(ValueError):
+        P.set_config("record", 256, 0, 4, 2)
+
+
+def test_streamed_passes_fall_back_with_few_active(small_geoms):
+    """With streamed configs on every kind and the default threshold (one
+    active strip segment per compute unit), a plan of 8 small models streams
+    while all are active (280 segments) and runs the tiled tail window when
+    one is (35); gradients, tangents and Born passes equal a tiled plan's in
+    both regimes as the mask switches back and forth."""
+    g, _ = small_geoms
+    B = 8
+    v = torch.tensor(np.concatenate([synth.to_vec(synth.make_model("curved", g.nz, g.nx, 30 + i))
+                                     for i in range(B)]))
+    rng = np.random.default_rng(14)
+    dm = torch.tensor(np.concatenate([np.concatenate([rng.normal(0, 20, g.nz * g.nx), [3.0]]) for _ in range(B)]))
+    Pt = Propagator([g] * B)
+    dd = Pt.forward(v * 1.01).clone()
+    ref_g = _np(Pt.gradient(v, dd)[2])
+    ref_b = _np(Pt.tangent_saved(v, dm))
+    ref_t = _np(Pt.tangent(v, dm)[1])
+    P = Propagator([g] * B)
+    for kind in ("forward", "record", "tangent", "born", "adjoint"):
+        P.set_config(kind, 128, 8, 4, 2)
+    for mask in ([True] * B, [True] + [False] * (B - 1), [True] * B, [False] * (B - 1) + [True]):
+        P.set_active(mask)
+        gr = _np(P.gradient(v, dd)[2])
+        b = _np(P.tangent_saved(v, dm))
+        t = _np(P.tangent(v, dm)[1])
+        for i in range(B):
+            if not mask[i]:
+                continue
+            vs, ss = slice(P.vec_off[i], P.vec_off[i + 1]), slice(P.seis_off[i], P.seis_off[i + 1])
+            assert np.array_equal(gr[vs], ref_g[vs]), (mask, i)
+            assert np.array_equal(b[ss], ref_b[ss]) and np.array_equal(t[ss], ref_t[ss]), (mask, i)
 
 
 ADJ_STREAM_CONFIGS = [(128, 0, 4, 2), (64, 0, 4, 2), (128, 9, 4, 2), (64, 13, 6, 2), (128, 0, 6, 2)]
@@ -242,6 +356,7 @@ def test_streamed_adjoint_bit_identical(small_geoms, cfg):
     dm = torch.tensor(np.concatenate([rng.normal(0, 20, g.nz * g.nx), [3.0]]))
     for ckpt in (0, 17):
         P = Propagator([g], ckpt_every=ckpt)
+        P.set_stream_threshold(0)   # stream however few strips are active
         _, mref, gref = P.gradient(torch.tensor(vec), torch.tensor(target))
         gref, mref = _np(gref), float(mref[0])
         if ckpt == 0:
@@ -273,6 +388,7 @@ def test_streamed_adjoint_ragged_batch():
     vec = torch.tensor(np.concatenate([synth.to_vec(v) for v in vs]))
     for ckpt in (0, 13):
         P = Propagator(geoms, ckpt_every=ckpt)
+        P.set_stream_threshold(0)   # stream however few strips are active
         dd = P.forward(vec * 1.01).clone()
         _, mref, gref = P.gradient(vec, dd)
         gref, mref = _np(gref), _np(mref)
@@ -301,6 +417,7 @@ def test_streamed_born_ragged_batch():
                                        + rng.uniform(-100, 100, (gg.nz, gg.nx))) for gg in geoms])
     dv = np.concatenate([np.concatenate([rng.normal(0, 15, gg.nz * gg.nx), [2.0]]) for gg in geoms])
     P = Propagator(geoms, ckpt_every=0)
+    P.set_stream_threshold(0)   # stream however few strips are active
     _, ref = P.tangent(torch.tensor(vec), torch.tensor(dv))
     ref = _np(ref)
     P.forward_save(torch.tensor(vec))
@@ -326,6 +443,7 @@ def test_born_saved_matches_fused(small_geoms, cfg):
     rng = np.random.default_rng(5)
     dm = np.concatenate([rng.normal(0, 20, g.nz * g.nx), [3.0]])
     P = Propagator([g])
+    P.set_stream_threshold(0)   # stream however few strips are active
     P.set_config("born", *cfg)
     _, ds_fused = P.tangent(torch.tensor(vec), torch.tensor(dm))
     dd = torch.tensor(fd2d.vel_to_seis(synth.to_vec(v * 0.99), og)[0])
