@@ -1239,23 +1239,38 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     float qlo[NC], qhi[NC];   // q[m-1], q[m] of the step (interior cells)
     int hidx[NC];  // history index of an interior cell; 0 (a dummy read) elsewhere
     unsigned srcmask = 0, rcvmask = 0, intmask = 0;
-    int rcol[CPR];
+    // per own column, once per tile: wrapped x, receiver index, 1 - kappa in
+    // the left/right damping strips, interior flag
+    int gxc[CPR], rcol[CPR];
+    bool isxc[CPR], cin[CPR];
+    float kxc[CPR];
 #pragma unroll
-    for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
+    for (int c = 0; c < CPR; ++c) {
+      const int cc = lane + 64 * c;
+      const int gx = wrapi(ox + cc, NX);
+      gxc[c] = gx;
+      rcol[c] = rcv_index(gx, m);
+      isxc[c] = gx < m.nbc || gx >= m.nx + m.nbc;
+      const int jx = damp_j(m.nbc, gx, m.nz, m.nx, m.nbc);   // the x profile (used in the strips)
+      kxc[c] = 1.0f - kscale * (float)(jx * jx);
+      cin[c] = cc >= H && cc < RX - H && ox + cc < NX;
+    }
     if (threadIdx.x < KMAX) {
       const int t = a.M0 - 3 - (int)threadIdx.x;
       SW[threadIdx.x] = ((int)threadIdx.x < a.K && t >= 0 && t < a.nt) ? a.wavelet[t] : 0.f;
     }
-    // adjoint sources of this launch's steps along the window's columns
-    for (int idx = threadIdx.x; idx < a.K * RX; idx += NT) {
-      const int s = idx / RX, cc = idx - (idx / RX) * RX;
-      const int rr = rcv_index(wrapi(ox + cc, NX), m);
-      float v = 0.f;
-      if (rr >= 0) {
-        const int64_t si = w.seis_off + (int64_t)(a.M0 - 3 - s) * m.n_rcv + rr;
-        v = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
+    // adjoint sources of this launch's steps along the window's columns (each
+    // wavefront fills steps wv, wv + NW, ... of its lanes' columns)
+    for (int s = wv; s < a.K; s += NW) {
+#pragma unroll
+      for (int c = 0; c < CPR; ++c) {
+        float v = 0.f;
+        if (rcol[c] >= 0) {
+          const int64_t si = w.seis_off + (int64_t)(a.M0 - 3 - s) * m.n_rcv + rcol[c];
+          v = a.inj_b ? a.inj_a[si] - a.inj_b[si] : a.inj_a[si];
+        }
+        INJ[s][lane + 64 * c] = v;
       }
-      INJ[s][cc] = v;
     }
     const float* h0 = hist + (int64_t)(a.M0 - 2 - a.h_base) * Pn;  // q[M0-2]; q[M0-1] one slot on
 #pragma unroll
@@ -1263,36 +1278,30 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << j;
-      const int zz = oz + r;
+      const bool rin = r >= H && r < RZ - H && oz + r < NZ;        // wave-uniform
+      const int jz = damp_j(gz, m.nbc, m.nz, m.nx, m.nbc);        // the z profile (outside the strips)
+      const float kz = 1.0f - kscale * (float)(jz * jz);
+      const int grow = gz * NX, gir = gint0 + r * NX;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = j * CPR + c;
         const int cc = lane + 64 * c;
-        const int gx = wrapi(ox + cc, NX);
-        const int g = gz * NX + gx;
+        const int g = grow + gxc[c];
         lam[i] = lin[g];
         mu[i] = min_[g];
         al[i] = alpha[g];
         A0[r + 2][cc + 2] = al[i] * lam[i];
-        const int jd = damp_j(gz, gx, m.nz, m.nx, m.nbc);
-        k1[i] = 1.0f - kscale * (float)(jd * jd);
+        k1[i] = isxc[c] ? kxc[c] : kz;
         if (g == w.src_cell) srcmask |= 1u << i;
-        const int xx = ox + cc;
-        const bool inner = r >= H && r < RZ - H && zz < NZ && cc >= H && cc < RX - H && xx < NX;
-        ga[i] = 0.f;
-        gk[i] = 0.f;
-        qlo[i] = 0.f;
-        qhi[i] = 0.f;
-        hidx[i] = 0;
-        if (inner) {
-          intmask |= 1u << i;
-          const int gi = zz * NX + xx;
-          ga[i] = GA[gi];
-          gk[i] = GK[gi];
-          qlo[i] = h0[gi];
-          qhi[i] = h0[Pn + gi];
-          hidx[i] = gi;
-        }
+        // interior cells load their accumulators and history; the others read
+        // one dummy address (their sums are never stored)
+        const bool inner = rin && cin[c];
+        if (inner) intmask |= 1u << i;
+        hidx[i] = inner ? gir + cc : 0;
+        ga[i] = GA[hidx[i]];
+        gk[i] = GK[hidx[i]];
+        qlo[i] = h0[hidx[i]];
+        qhi[i] = h0[Pn + hidx[i]];
       }
     }
     double ssum = 0.0;
