@@ -48,6 +48,8 @@ L2_PEAK = 8 * 16 * 128 * 2.4e9
 ALG_BYTES = {"linesearch_forward": 20, "saving_forward": 24, "tangent": 40, "born": 32, "adjoint": 28}
 KERNEL = {"linesearch_forward": "k_fwd", "saving_forward": "k_fwd (history)", "tangent": "k_fwd (tangent)",
           "born": "k_born", "adjoint": "k_adj"}
+STREAMED = {"linesearch_forward": "k_fws", "saving_forward": "k_fws (history)", "tangent": "k_fws (tangent)",
+            "born": "k_bns", "adjoint": "k_ads"}
 N_OUTER, N_CG, N_SCALES = 10, 20, 50
 
 
@@ -142,7 +144,7 @@ def kernel_roofline(ex, B, g, reps=3):
         rx, rz, k, strips = plan.get_config(kind)
         launches = math.ceil(steps / k)
         cells = plan.n_models * g.n_shots * g.padded_cells
-        kname = ("k_bns" if name == "born" else "k_fws") if strips == 2 else KERNEL[name]
+        kname = STREAMED[name] if strips == 2 else KERNEL[name]
         out[name] = {"kernel": kname, "pass_s": t_pass, "launches": launches, "launch_s": t_pass / launches,
                      "cell_updates_per_launch": cells * k, "k": k, "window": [rx, rz], "strips": strips,
                      "models": plan.n_models}

@@ -50,8 +50,13 @@ ALG = {"linesearch_forward": 20, "saving_forward": 24, "tangent": 40, "born": 32
 def _pattern(name, e):
     """Kernel-name pattern of a kernel_probe pass with config entry e."""
     rx, rz = e["window"]
-    if e["strips"] == 2:   # row-streamed kernels: k_fws / k_bns<threads, levels>
-        return rf"k_bns<{rx}, {e['k']}>" if name == "born" else rf"k_fws<{rx}, {e['k']}>"
+    if e["strips"] == 2:   # row-streamed kernels: k_bns / k_ads<threads, levels>, k_fws<threads, levels, tan, hist>
+        k = e["k"]
+        return {"linesearch_forward": rf"k_fws<{rx}, {k}, false, false>",
+                "saving_forward": rf"k_fws<{rx}, {k}, false, true>",
+                "tangent": rf"k_fws<{rx}, {k}, true, false>",
+                "born": rf"k_bns<{rx}, {k}>",
+                "adjoint": rf"k_ads<{rx}, {k}>"}[name]
     st = "true" if e["strips"] else "false"
     return {"linesearch_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, false>",
             "saving_forward": rf"k_fwd<{rx}, {rz}, \d+, false, {st}, true>",
