@@ -310,32 +310,46 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
     // row bits (srcrow, introw, rcvmask) are wave-uniform; cell bits per lane
     unsigned srcmask = 0, rcvmask = 0, intmask = 0, srcrow = 0, introw = 0;
     const int srow = w.src_cell / NX;
-    int rcol[CPR];
+    // per own column, once per tile: wrapped x, receiver index, the x damping
+    // profile (used in the left/right strips), interior flag
+    int gxc[CPR], rcol[CPR];
+    bool isxc[CPR], cin[CPR];
+    float jjxc[CPR];
 #pragma unroll
-    for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
+    for (int c = 0; c < CPR; ++c) {
+      const int cc = lane + 64 * c;
+      const int gx = wrapi(ox + cc, NX);
+      gxc[c] = gx;
+      rcol[c] = rcv_index(gx, m);
+      isxc[c] = gx < m.nbc || gx >= m.nx + m.nbc;
+      const int jx = damp_j(m.nbc, gx, m.nz, m.nx, m.nbc);
+      jjxc[c] = (float)(jx * jx);
+      cin[c] = cc >= H && cc < RX - H && ox + cc < NX;
+    }
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
       const int r = own_row<STRIP, NW, RPW>(wv, j);
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << j;
       if (gz == srow) srcrow |= 1u << j;
-      if (r >= H && r < RZ - H && oz + r < NZ) introw |= 1u << j;
+      const bool rin = r >= H && r < RZ - H && oz + r < NZ;
+      if (rin) introw |= 1u << j;
+      const int jz = damp_j(gz, m.nbc, m.nz, m.nx, m.nbc);   // the z profile (outside the strips)
+      const float jjz = (float)(jz * jz);
+      const int grow = gz * NX;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = j * CPR + c;
         const int cc = lane + 64 * c;
-        const int gx = wrapi(ox + cc, NX);
-        const int g = gz * NX + gx;
+        const int g = grow + gxc[c];
         pv[i] = pin[g];
         P0[r + 2][cc + 2] = pv[i];
         q[i] = qin[g];
         al[i] = alpha[g];
-        const int jd = damp_j(gz, gx, m.nz, m.nx, m.nbc);
-        const float jj = (float)(jd * jd);
+        const float jj = isxc[c] ? jjxc[c] : jjz;
         k1[i] = 1.0f - kscale * jj;
         if (g == w.src_cell) srcmask |= 1u << i;
-        if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
-          intmask |= 1u << i;
+        if (rin && cin[c]) intmask |= 1u << i;
         if constexpr (TAN) {
           dpv[i] = dpin[g];
           D0[r + 2][cc + 2] = dpv[i];
@@ -846,9 +860,22 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
     }
     unsigned srcmask = 0, rcvmask = 0, intmask = 0, srcrow = 0;
     const int srow = w.src_cell / NX;
-    int rcol[CPR];
+    // per own column, once per tile: wrapped x, receiver index, the x damping
+    // profile (used in the left/right strips), interior flag
+    int gxc[CPR], rcol[CPR];
+    bool isxc[CPR], cin[CPR];
+    float jjxc[CPR];
 #pragma unroll
-    for (int c = 0; c < CPR; ++c) rcol[c] = rcv_index(wrapi(ox + lane + 64 * c, NX), m);
+    for (int c = 0; c < CPR; ++c) {
+      const int cc = lane + 64 * c;
+      const int gx = wrapi(ox + cc, NX);
+      gxc[c] = gx;
+      rcol[c] = rcv_index(gx, m);
+      isxc[c] = gx < m.nbc || gx >= m.nx + m.nbc;
+      const int jx = damp_j(m.nbc, gx, m.nz, m.nx, m.nbc);
+      jjxc[c] = (float)(jx * jx);
+      cin[c] = cc >= H && cc < RX - H && ox + cc < NX;
+    }
     const float* h0 = hist + (int64_t)a.n0 * Pn;
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
@@ -856,12 +883,15 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
       const int gz = wrapi(oz + r, NZ);
       if (gz == m.rcv_row) rcvmask |= 1u << j;
       if (gz == srow) srcrow |= 1u << j;
+      const bool rin = r >= H && r < RZ - H && oz + r < NZ;
+      const int jz = damp_j(gz, m.nbc, m.nz, m.nx, m.nbc);   // the z profile (outside the strips)
+      const float jjz = (float)(jz * jz);
+      const int grow = gz * NX;
 #pragma unroll
       for (int c = 0; c < CPR; ++c) {
         const int i = j * CPR + c;
         const int cc = lane + 64 * c;
-        const int gx = wrapi(ox + cc, NX);
-        const int g = gz * NX + gx;
+        const int g = grow + gxc[c];
         gw[i] = g;
         qo[i] = h0[g];
         qn[i] = h0[Pn + g];
@@ -870,11 +900,9 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
         dq[i] = dqin[g];
         al[i] = alpha[g];
         dal[i] = dalpha[g];
-        const int jd = damp_j(gz, gx, m.nz, m.nx, m.nbc);
-        jj[i] = (float)(jd * jd);
+        jj[i] = isxc[c] ? jjxc[c] : jjz;
         if (g == w.src_cell) srcmask |= 1u << i;
-        if (r >= H && r < RZ - H && oz + r < NZ && cc >= H && cc < RX - H && ox + cc < NX)
-          intmask |= 1u << i;
+        if (rin && cin[c]) intmask |= 1u << i;
       }
     }
     __syncthreads();
