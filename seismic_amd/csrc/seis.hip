@@ -131,6 +131,30 @@ __device__ __forceinline__ int wrapi(int i, int n) {
   return i;
 }
 
+// Raw buffer resource over device memory starting at p.  A load or store
+// through it takes the thread's byte offset in a vector register and a
+// wave-uniform byte offset in a scalar register (buffer_load_dword ... offen),
+// so a row access costs no vector address arithmetic and row offsets advance
+// as 32-bit scalar adds.  Offsets must stay below 4 GiB: the plan refuses a
+// streamed config when one wavefield's state region is larger.  Word 3 is the
+// gfx9-family raw 32-bit buffer format.
+typedef __amdgpu_buffer_rsrc_t brsrc_t;
+__device__ __forceinline__ brsrc_t brsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ float bld(brsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bst(brsrc_t r, unsigned voff, unsigned soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+// Byte offset of the next row (one padded row on, wrapping at the slot end
+// `end`, a slot being `slot` bytes): a compare and a subtract on the scalar unit.
+__device__ __forceinline__ unsigned next_row(unsigned o, unsigned rowb, unsigned end, unsigned slot) {
+  o += rowb;
+  return o >= end ? o - slot : o;
+}
+
 // Damping index j of padded cell (z, x): AbcCoef2D (:46-59).  Left/right
 // strips span all rows, top/bottom strips only interior columns, so corners
 // take the x profile; damp = kappa0 * (j/(nbc-1))^2, zero inside.
@@ -374,7 +398,8 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
       const float src = amp * sv;
       const float srcd = amp_d * sv;
       const int lo = 2 * (s + 1);
-      float* hs = HIST ? hist + s * Pn : nullptr;
+      // history slot of this step, through a buffer resource (see k_adj)
+      const brsrc_t VH = brsrc(HIST ? hist + s * Pn : a.arena);
       if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {  // the strip meets the valid band (wave-uniform)
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
@@ -397,7 +422,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
             if ((srcrow >> j) & 1u) qn = ((srcmask >> i) & 1u) ? qn + src : qn;
             if constexpr (HIST) {
               if ((introw >> j) & 1u) {
-                if ((intmask >> i) & 1u) hs[gint0 + r * NX + cc] = qn;
+                if ((intmask >> i) & 1u) bst(VH, (unsigned)(gint0 + r * NX + cc) * 4u, 0u, qn);
               }
             }
             if constexpr (TAN) {
@@ -490,30 +515,6 @@ constexpr int SPD = 3;   // pipeline steps between a row's load and its intake
 __device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float kscale) {
   const int j = damp_j(gz, m.nbc, m.nz, m.nx, m.nbc);
   return 1.0f - kscale * (float)(j * j);
-}
-
-// Raw buffer resource over device memory starting at p.  A load or store
-// through it takes the thread's byte offset in a vector register and a
-// wave-uniform byte offset in a scalar register (buffer_load_dword ... offen),
-// so a row access costs no vector address arithmetic and row offsets advance
-// as 32-bit scalar adds.  Offsets must stay below 4 GiB: the plan refuses a
-// streamed config when one wavefield's state region is larger.  Word 3 is the
-// gfx9-family raw 32-bit buffer format.
-typedef __amdgpu_buffer_rsrc_t brsrc_t;
-__device__ __forceinline__ brsrc_t brsrc(const void* p) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
-}
-__device__ __forceinline__ float bld(brsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
-__device__ __forceinline__ void bst(brsrc_t r, unsigned voff, unsigned soff, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
-}
-// Byte offset of the next row (one padded row on, wrapping at the slot end
-// `end`, a slot being `slot` bytes): a compare and a subtract on the scalar unit.
-__device__ __forceinline__ unsigned next_row(unsigned o, unsigned rowb, unsigned end, unsigned slot) {
-  o += rowb;
-  return o >= end ? o - slot : o;
 }
 
 // KL forward time steps (update_p :428-462, lapg :381-421, source injection
@@ -958,11 +959,11 @@ __global__ __launch_bounds__(NT) void k_born(BornArgs a) {
         }
       }
       if (s + 1 < a.K) {  // q of the next step (every owned cell, back to back)
-        const float* hx = hist + (int64_t)(a.n0 + s + 2) * Pn;
+        const brsrc_t VH = brsrc(hist + (int64_t)(a.n0 + s + 2) * Pn);   // the slot (see k_adj)
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
           qo[i] = qn[i];
-          qn[i] = hx[gw[i]];
+          qn[i] = bld(VH, (unsigned)gw[i] * 4u, 0u);
         }
       }
       __syncthreads();
@@ -1395,11 +1396,13 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       if (s + 1 < a.K) {  // q[mm - 2] for the next step
         // issued unconditionally (cells outside the interior all read one
         // dummy address) so that the loads go out back to back
-        const float* hx = hist + (int64_t)(mm - 2 - a.h_base) * Pn;
+        // through a buffer resource over the slot (built on the scalar unit),
+        // so no load carries vector address arithmetic
+        const brsrc_t VH = brsrc(hist + (int64_t)(mm - 2 - a.h_base) * Pn);
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
           qhi[i] = qlo[i];
-          qlo[i] = hx[hidx[i]];
+          qlo[i] = bld(VH, (unsigned)hidx[i] * 4u, 0u);
         }
       }
       __syncthreads();
