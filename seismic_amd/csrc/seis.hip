@@ -198,6 +198,31 @@ __device__ __forceinline__ float lap_lds(const float (*T)[W], float pc, int r, i
   return fmaf(kC2, t1, kC3 * t2);
 }
 
+// LDS floats addressed by their 32-bit LDS address.  A row base passed
+// through an empty asm is opaque to the compiler, so it stays in one
+// register and the four horizontal neighbours of a row load as two ds_read2
+// with immediate offsets, with no address arithmetic per step.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ unsigned lds_row(const float* p) {
+  unsigned a = (unsigned)(size_t)(const lds_float*)p;
+  asm("" : "+v"(a));
+  return a;
+}
+__device__ __forceinline__ lds_float* lds_at(unsigned a) { return (lds_float*)(size_t)a; }
+
+// lap_col with the horizontal neighbours read through rp = &T[r + 2][cc]
+// (same operations in the same order).
+__device__ __forceinline__ float lap_row(const lds_float* rp, const float* col, int j) {
+  const float pc = col[j + 2];
+  float t1 = (rp[3] - pc) + (rp[1] - pc);
+  t1 = t1 + (col[j + 3] - pc);
+  t1 = t1 + (col[j + 1] - pc);
+  float t2 = (rp[4] - pc) + (rp[0] - pc);
+  t2 = t2 + (col[j + 4] - pc);
+  t2 = t2 + (col[j] - pc);
+  return fmaf(kC2, t1, kC3 * t2);
+}
+
 // Window row of a thread's j-th owned row.  STRIP: each wavefront owns RPW
 // consecutive rows (vertical neighbours of inner rows stay in registers);
 // otherwise rows are dealt round-robin over the NW wavefronts, which keeps
@@ -1265,7 +1290,10 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;
     float lam[NC], mu[NC], al[NC], k1[NC], ga[NC], gk[NC];
-    float qlo[NC], qhi[NC];   // q[m-1], q[m] of the step (interior cells)
+    // q of interior cells: at even steps qx holds q[m] and qy q[m-1], at odd
+    // steps the roles swap, so the history needs no register copies; the
+    // array holding q[m] takes alpha*Lf(p[m-1]) and then the next load
+    float qx[NC], qy[NC];
     int hidx[NC];  // history index of an interior cell; 0 (a dummy read) elsewhere
     unsigned srcmask = 0, rcvmask = 0, intmask = 0;
     // per own column, once per tile: wrapped x, receiver index, 1 - kappa in
@@ -1329,13 +1357,22 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         hidx[i] = inner ? gir + cc : 0;
         ga[i] = GA[hidx[i]];
         gk[i] = GK[hidx[i]];
-        qlo[i] = h0[hidx[i]];
-        qhi[i] = h0[Pn + hidx[i]];
+        qy[i] = h0[hidx[i]];
+        qx[i] = h0[Pn + hidx[i]];
       }
     }
     double ssum = 0.0;
     const bool src_here = (srcmask & intmask) != 0;
     if (src_here) ssum = a.wparam[te.w].ssum;
+    // LDS addresses of the own rows in both window buffers, &A0/A1[r + 2][lane]
+    unsigned rb0[STRIP ? RPW : 1], rb1[STRIP ? RPW : 1];
+    if constexpr (STRIP) {
+#pragma unroll
+      for (int j = 0; j < RPW; ++j) {
+        rb0[j] = lds_row(&A0[r0 + j + 2][lane]);
+        rb1[j] = lds_row(&A1[r0 + j + 2][lane]);
+      }
+    }
     __syncthreads();
     // Step s reads the window buffer s&1 and writes the other; the buffer
     // is a compile-time choice so that reads and writes never alias.
@@ -1343,10 +1380,13 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       constexpr int CUR = decltype(cb)::value;
       float (*const Ard)[PW] = CUR ? A1 : A0;
       float (*const Awr)[PW] = CUR ? A0 : A1;
+      unsigned (&rbr)[STRIP ? RPW : 1] = CUR ? rb1 : rb0;   // own rows of Ard
+      unsigned (&rbw)[STRIP ? RPW : 1] = CUR ? rb0 : rb1;   // own rows of Awr
+      float (&qm)[NC] = CUR ? qy : qx;   // q[m]; becomes alpha*Lf(p[m-1]), then q[m-2]
+      float (&ql)[NC] = CUR ? qx : qy;   // q[m-1]
       const int mm = a.M0 - 1 - s;
       const int lo = 2 * (s + 1);
       const float sw = SW[s];
-      const float srcf = amp * sw;   // the forward's source term in q[mm]
       if (!STRIP || (r0 + RPW > lo && r0 < RZ - lo)) {
 #pragma unroll
         for (int c = 0; c < CPR; ++c) {
@@ -1367,12 +1407,13 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
             const int r = own_row<STRIP, NW, RPW>(wv, j);
             const int i = j * CPR + c;
             float La;
-            if constexpr (STRIP) La = lap_col<PW>(Ard, acol, j, r, cc);
+            if constexpr (STRIP) La = lap_row(lds_at(rbr[j]) + 64 * c, acol, j);
             else La = lap_lds<PW>(Ard, al[i] * lam[i], r, cc);
             float mun = fmaf(k1[i], mu[i], La);
             if ((rcvmask >> j) & 1u) mun = rc ? mun + inj : mun;
             const float ln = lam[i] + mun;
-            Awr[r + 2][cc + 2] = al[i] * ln;
+            if constexpr (STRIP) lds_at(rbw[j])[64 * c + 2] = al[i] * ln;
+            else Awr[r + 2][cc + 2] = al[i] * ln;
             lam[i] = ln;
             mu[i] = mun;
           }
@@ -1380,30 +1421,35 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
       }
       // imaging on the interior (always inside the band): lam now holds lam[m]
       // GA += lam[m] alpha Lf(p[m-1]);  GK += lam[m] (p[m-2] - p[m-1]) = -lam[m] q[m-1]
-      // every cell accumulates; only interior cells' sums are stored
+      // every cell accumulates; only interior cells' sums are stored.
+      // alpha Lf(p[m-1]) = q[m] - (1-kappa) q[m-1], less the forward's source
+      // term at the source cell (a divergent branch taken by one lane only)
 #pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        float aL = fmaf(-k1[i], qlo[i], qhi[i]);
-        aL = ((srcmask & intmask) >> i & 1u) ? aL - srcf : aL;
-        ga[i] = fmaf(lam[i], aL, ga[i]);
-        gk[i] = fmaf(lam[i], -qlo[i], gk[i]);
-      }
+      for (int i = 0; i < NC; ++i) qm[i] = fmaf(-k1[i], ql[i], qm[i]);
       if (src_here) {  // only the thread owning the source cell
+        const float srcf = amp * sw;   // the forward's source term in q[mm]
 #pragma unroll
         for (int i = 0; i < NC; ++i)
-          if ((srcmask & intmask) >> i & 1u) ssum += (double)lam[i] * (double)sw;
+          if ((srcmask & intmask) >> i & 1u) {
+            qm[i] = qm[i] - srcf;
+            ssum += (double)lam[i] * (double)sw;
+          }
       }
-      if (s + 1 < a.K) {  // q[mm - 2] for the next step
-        // issued unconditionally (cells outside the interior all read one
-        // dummy address) so that the loads go out back to back
-        // through a buffer resource over the slot (built on the scalar unit),
-        // so no load carries vector address arithmetic
-        const brsrc_t VH = brsrc(hist + (int64_t)(mm - 2 - a.h_base) * Pn);
 #pragma unroll
-        for (int i = 0; i < NC; ++i) {
-          qhi[i] = qlo[i];
-          qlo[i] = bld(VH, (unsigned)hidx[i] * 4u, 0u);
-        }
+      for (int i = 0; i < NC; ++i) {
+        ga[i] = fmaf(lam[i], qm[i], ga[i]);
+        gk[i] = fmaf(lam[i], -ql[i], gk[i]);
+      }
+      // q[mm - 2] for the next step into the array that held q[m], issued
+      // unconditionally (cells outside the interior all read one dummy
+      // address; the last step re-reads q[mm - 1], just read) so that the
+      // loads go out back to back, through a buffer resource over the slot
+      // (built on the scalar unit) so no load carries vector address arithmetic
+      {
+        const int slot = (s + 1 < a.K ? mm - 2 : mm - 1) - a.h_base;
+        const brsrc_t VH = brsrc(hist + (int64_t)slot * Pn);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) qm[i] = bld(VH, (unsigned)hidx[i] * 4u, 0u);
       }
       __syncthreads();
     };
