@@ -318,7 +318,11 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
   // stencil inside the array.  The apron and cells outside the valid band
   // hold stale values that no committed cell depends on.
   constexpr int PW = RX + 4, PH = RZ + 4;
-  __shared__ float P0[PH][PW], P1[PH][PW];
+  // the two p windows interleaved by rows (row r of window b at PP[r][b]):
+  // a register holding the address of PP[r][0][lane] reaches the
+  // neighbours of both windows with ds_read2 offsets
+  __shared__ float PP[PH][2][PW];
+  constexpr int PS = 2 * PW;   // row stride of either window
   __shared__ float D0[TAN ? PH : 1][TAN ? PW : 1], D1[TAN ? PH : 1][TAN ? PW : 1];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63;
@@ -392,7 +396,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
         const int cc = lane + 64 * c;
         const int g = grow + gxc[c];
         pv[i] = pin[g];
-        P0[r + 2][cc + 2] = pv[i];
+        PP[r + 2][0][cc + 2] = pv[i];
         q[i] = qin[g];
         al[i] = alpha[g];
         const float jj = isxc[c] ? jjxc[c] : jjz;
@@ -408,13 +412,21 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
         }
       }
     }
+    // LDS addresses of the own rows, &PP[r + 2][0][lane] (strips without
+    // the tangent, whose register budget is tighter)
+    constexpr bool RB = STRIP && !TAN;
+    unsigned rb[RB ? RPW : 1];
+    if constexpr (RB) {
+#pragma unroll
+      for (int j = 0; j < RPW; ++j) rb[j] = lds_row(&PP[r0 + j + 2][0][lane]);
+    }
     __syncthreads();
     // Step s reads the window buffer s&1 and writes the other; the buffer
     // is a compile-time choice so that reads and writes never alias.
     auto step = [&](auto cb, const int s) {
       constexpr int CUR = decltype(cb)::value;
-      float (*const Prd)[PW] = CUR ? P1 : P0;
-      float (*const Pwr)[PW] = CUR ? P0 : P1;
+      float (*const Prd)[PS] = reinterpret_cast<float (*)[PS]>(&PP[0][CUR][0]);
+      float (*const Pwr)[PS] = reinterpret_cast<float (*)[PS]>(&PP[0][1 - CUR][0]);
       float (*const Drd)[TAN ? PW : 1] = CUR ? D1 : D0;
       float (*const Dwr)[TAN ? PW : 1] = CUR ? D0 : D1;
       const int it = a.n0 + s - 1;
@@ -430,7 +442,7 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
         for (int c = 0; c < CPR; ++c) {
           const int cc = lane + 64 * c;
           float col[STRIP ? RPW + 4 : 1];
-          if constexpr (STRIP) load_col<PW, RPW, CPR>(Prd, pv, c, r0, cc, col);
+          if constexpr (STRIP) load_col<PS, RPW, CPR>(Prd, pv, c, r0, cc, col);
           float dcol[TAN && STRIP ? RPW + 4 : 1];
           if constexpr (TAN && STRIP) load_col<PW, RPW, CPR>(Drd, dpv, c, r0, cc, dcol);
 #pragma unroll
@@ -441,8 +453,9 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
             }
             const int i = j * CPR + c;
             float L;
-            if constexpr (STRIP) L = lap_col<PW>(Prd, col, j, r, cc);
-            else L = lap_lds<PW>(Prd, pv[i], r, cc);
+            if constexpr (RB) L = lap_row(lds_at(rb[j]) + CUR * PW + 64 * c, col, j);
+            else if constexpr (STRIP) L = lap_col<PS>(Prd, col, j, r, cc);
+            else L = lap_lds<PS>(Prd, pv[i], r, cc);
             float qn = fmaf(k1[i], q[i], al[i] * L);
             if ((srcrow >> j) & 1u) qn = ((srcmask >> i) & 1u) ? qn + src : qn;
             if constexpr (HIST) {
@@ -469,7 +482,8 @@ __global__ __launch_bounds__(NT) void k_fwd(FwdArgs a) {
               dq[i] = dqn;
             }
             const float p1 = pv[i] + qn;
-            Pwr[r + 2][cc + 2] = p1;
+            if constexpr (RB) lds_at(rb[j])[(1 - CUR) * PW + 64 * c + 2] = p1;
+            else Pwr[r + 2][cc + 2] = p1;
             pv[i] = p1;
             q[i] = qn;
           }
