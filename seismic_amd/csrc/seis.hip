@@ -137,10 +137,17 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 // so a row access costs no vector address arithmetic and row offsets advance
 // as 32-bit scalar adds.  Offsets must stay below 4 GiB: the plan refuses a
 // streamed config when one wavefield's state region is larger.  Word 3 is the
-// gfx9-family raw 32-bit buffer format.
+// gfx9-family raw 32-bit buffer format.  Every resource here is built from
+// a wave-uniform base; the base is read from the first lane so that the
+// compiler never treats it as divergent (a descriptor in vector registers
+// makes it loop over the distinct values, several instructions per access).
 typedef __amdgpu_buffer_rsrc_t brsrc_t;
 __device__ __forceinline__ brsrc_t brsrc(const void* p) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  void* const u = (void*)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(u, 0, -1, 0x00020000);
 }
 __device__ __forceinline__ float bld(brsrc_t r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
