@@ -1285,7 +1285,11 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
   static_assert(RZ % NW == 0, "rows must split evenly over the wavefronts");
   static_assert(NC <= 32 && RPW <= 32, "tile too large for the bit masks");
   constexpr int PW = RX + 4, PH = RZ + 4;     // alpha*lam windows with a 2-cell apron
-  __shared__ float A0[PH][PW], A1[PH][PW];
+  // the two alpha*lam windows interleaved by rows (row r of window b at
+  // AA[r][b]): one register holding the address of AA[r][0][lane] reaches
+  // the neighbours of both windows with ds_read2 offsets
+  __shared__ float AA[PH][2][PW];
+  constexpr int PS = 2 * PW;   // row stride of either window
   __shared__ float INJ[KMAX][RX];
   __shared__ float SW[KMAX];
   const int lane = threadIdx.x & 63;
@@ -1311,6 +1315,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     const int oz = te.z0 - H, ox = te.x0 - H;
     const int gint0 = oz * NX + ox;
     float lam[NC], mu[NC], al[NC], k1[NC], ga[NC], gk[NC];
+    float alm[NC];   // alpha * lam, the value the stencil sees
     // q of interior cells: at even steps qx holds q[m] and qy q[m-1], at odd
     // steps the roles swap, so the history needs no register copies; the
     // array holding q[m] takes alpha*Lf(p[m-1]) and then the next load
@@ -1368,7 +1373,8 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
         lam[i] = lin[g];
         mu[i] = min_[g];
         al[i] = alpha[g];
-        A0[r + 2][cc + 2] = al[i] * lam[i];
+        alm[i] = al[i] * lam[i];
+        AA[r + 2][0][cc + 2] = alm[i];
         k1[i] = isxc[c] ? kxc[c] : kz;
         if (g == w.src_cell) srcmask |= 1u << i;
         // interior cells load their accumulators and history; the others read
@@ -1385,24 +1391,19 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
     double ssum = 0.0;
     const bool src_here = (srcmask & intmask) != 0;
     if (src_here) ssum = a.wparam[te.w].ssum;
-    // LDS addresses of the own rows in both window buffers, &A0/A1[r + 2][lane]
-    unsigned rb0[STRIP ? RPW : 1], rb1[STRIP ? RPW : 1];
+    // LDS addresses of the own rows, &AA[r + 2][0][lane]
+    unsigned rb[STRIP ? RPW : 1];
     if constexpr (STRIP) {
 #pragma unroll
-      for (int j = 0; j < RPW; ++j) {
-        rb0[j] = lds_row(&A0[r0 + j + 2][lane]);
-        rb1[j] = lds_row(&A1[r0 + j + 2][lane]);
-      }
+      for (int j = 0; j < RPW; ++j) rb[j] = lds_row(&AA[r0 + j + 2][0][lane]);
     }
     __syncthreads();
     // Step s reads the window buffer s&1 and writes the other; the buffer
     // is a compile-time choice so that reads and writes never alias.
     auto step = [&](auto cb, const int s) {
       constexpr int CUR = decltype(cb)::value;
-      float (*const Ard)[PW] = CUR ? A1 : A0;
-      float (*const Awr)[PW] = CUR ? A0 : A1;
-      unsigned (&rbr)[STRIP ? RPW : 1] = CUR ? rb1 : rb0;   // own rows of Ard
-      unsigned (&rbw)[STRIP ? RPW : 1] = CUR ? rb0 : rb1;   // own rows of Awr
+      float (*const Ard)[PS] = reinterpret_cast<float (*)[PS]>(&AA[0][CUR][0]);
+      float (*const Awr)[PS] = reinterpret_cast<float (*)[PS]>(&AA[0][1 - CUR][0]);
       float (&qm)[NC] = CUR ? qy : qx;   // q[m]; becomes alpha*Lf(p[m-1]), then q[m-2]
       float (&ql)[NC] = CUR ? qx : qy;   // q[m-1]
       const int mm = a.M0 - 1 - s;
@@ -1417,7 +1418,7 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
             acol[0] = Ard[r0][cc + 2];
             acol[1] = Ard[r0 + 1][cc + 2];
 #pragma unroll
-            for (int j = 0; j < RPW; ++j) acol[j + 2] = al[j * CPR + c] * lam[j * CPR + c];
+            for (int j = 0; j < RPW; ++j) acol[j + 2] = alm[j * CPR + c];
             acol[RPW + 2] = Ard[r0 + RPW + 2][cc + 2];
             acol[RPW + 3] = Ard[r0 + RPW + 3][cc + 2];
           }
@@ -1428,13 +1429,14 @@ __global__ __launch_bounds__(NT) void k_adj(AdjArgs a) {
             const int r = own_row<STRIP, NW, RPW>(wv, j);
             const int i = j * CPR + c;
             float La;
-            if constexpr (STRIP) La = lap_row(lds_at(rbr[j]) + 64 * c, acol, j);
-            else La = lap_lds<PW>(Ard, al[i] * lam[i], r, cc);
+            if constexpr (STRIP) La = lap_row(lds_at(rb[j]) + CUR * PW + 64 * c, acol, j);
+            else La = lap_lds<PS>(Ard, alm[i], r, cc);
             float mun = fmaf(k1[i], mu[i], La);
             if ((rcvmask >> j) & 1u) mun = rc ? mun + inj : mun;
             const float ln = lam[i] + mun;
-            if constexpr (STRIP) lds_at(rbw[j])[64 * c + 2] = al[i] * ln;
-            else Awr[r + 2][cc + 2] = al[i] * ln;
+            alm[i] = al[i] * ln;
+            if constexpr (STRIP) lds_at(rb[j])[(1 - CUR) * PW + 64 * c + 2] = alm[i];
+            else Awr[r + 2][cc + 2] = alm[i];
             lam[i] = ln;
             mu[i] = mun;
           }
