@@ -197,7 +197,8 @@ enum {
  * strip (halo included) and marches it down the rows of an rz-row segment
  * (rz = 0: the whole padded column, rz >= 8 otherwise), advancing k time
  * levels in a software pipeline; forward and Born rx in {64, 128, 192, 256},
- * k in {3, 4}; record and tangent rx in {128, 192}, k in {3, 4}; adjoint rx
+ * forward k in {3, 4, 5}, Born k in {3, 4}; record and tangent rx in
+ * {128, 192}, k in {3, 4}; adjoint rx
  * in {64, 128}, k in {4, 6}; padded grids of at least 4k rows, and each
  * wavefield's state region below 4 GiB (32-bit offsets).  Launches shorter
  * than k steps (the last one, or up to a checkpoint) run a tiled 64x64

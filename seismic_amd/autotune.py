@@ -44,7 +44,7 @@ _WINDOWS["record"] = [(64, 32, 3), (128, 32, 4), (64, 64, 4), (128, 64, 4), (128
 CANDIDATES = {kind: [w + (st,) for w in ws for st in (1, 0)] for kind, ws in _WINDOWS.items()}
 # row-streamed forward (strips = 2): rx-column strips, rz-row segments (0:
 # whole columns), k levels
-CANDIDATES["forward"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 40, 80, 160) for k in (3, 4)]
+CANDIDATES["forward"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 40, 80, 160) for k in (3, 4, 5)]
 CANDIDATES["born"] += [(rx, rz, k, 2) for rx in (64, 128) for rz in (0, 80, 160) for k in (3, 4)]
 # row-streamed fused forward + tangent: 128- or 192-column strips
 CANDIDATES["tangent"] += [(rx, rz, k, 2) for rx in (128, 192) for rz in (0, 80, 160) for k in (3, 4)]

@@ -593,18 +593,22 @@ __device__ __forceinline__ float kappa_row(int gz, const ModelDev& m, float ksca
 // advance KL steps (shorter ones run k_fwd).
 template <int NT, int KL, bool TAN, bool HIST>
 __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
-  static_assert(2 * KL + SPD < SUN && SUN % 6 == 0 && SUN % SPD == 0, "pipeline rings too short");
+  // pipeline steps per unrolled block: 12, or 18 when the rows in flight
+  // (2*KL + SPD) need a longer (alpha, 1 - kappa) ring (5 levels)
+  constexpr int SUNF = 2 * KL + SPD < SUN ? SUN : 18;
+  static_assert(2 * KL + SPD < SUNF && SUNF % 6 == 0 && SUNF % SPD == 0, "pipeline rings too short");
   constexpr int H = 2 * KL;
   // rows of the level rings, column-major: window column x (two apron
   // columns either side) holds its KL*3 ring slots at [(x + 2) * RS], so a
   // thread reaches every slot of its own and its neighbours' columns from one
-  // base address with immediate offsets; RS is odd, so the 64 lanes of a
-  // wavefront (stride RS) fall in distinct banks
-  constexpr int RS = 3 * KL + 1;
+  // base address with immediate offsets; RS (at least the 3*KL slots of
+  // levels 0..KL-1) is odd, so the 64 lanes of a wavefront (stride RS) fall
+  // in distinct banks (an even stride would put them in a few banks)
+  constexpr int RS = (3 * KL) | 1;
   __shared__ float RG[(NT + 4) * RS];
-  __shared__ float2 AK[SUN][NT];  // (alpha, 1 - kappa) of window row u at [u % SUN], own column
+  __shared__ float2 AK[SUNF][NT];  // (alpha, 1 - kappa) of window row u at [u % SUNF], own column
   __shared__ float DG[TAN ? (NT + 4) * RS : 1];      // dp rings
-  __shared__ float2 DK[TAN ? SUN : 1][TAN ? NT : 1];  // (dalpha/alpha, dkappa), like AK
+  __shared__ float2 DK[TAN ? SUNF : 1][TAN ? NT : 1];  // (dalpha/alpha, dkappa), like AK
   const unsigned c = threadIdx.x;
   const int ntiles = *a.n_tiles;
   const int n0 = a.n0, nt = a.nt;
@@ -723,8 +727,8 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
       static_for<1, KL + 1>([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         const int u = t - 2 * l;                      // window row of level l
-        constexpr int B = 4 * SUN;                    // keeps the ring indices non-negative
-        constexpr int s6 = (i - 2 * l + B) % 6, s3 = (i - 2 * l + B) % 3, sr = (i - 2 * l + B) % SUN;
+        constexpr int B = 4 * SUNF;                    // keeps the ring indices non-negative
+        constexpr int s6 = (i - 2 * l + B) % 6, s3 = (i - 2 * l + B) % 3, sr = (i - 2 * l + B) % SUNF;
         const float pc = pr[l - 1][s6];
         const float* T = RG + c * RS + (l - 1) * 3 + s3;   // T[k * RS]: window column c + k - 2
         float t1 = (T[3 * RS] - pc) + (T[RS] - pc);
@@ -808,7 +812,7 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
         pfp[i % SPD] = bld(VW, vx, o_pi + rq);
         pfq[i % SPD] = bld(VW, vx, o_qi + rq);
         pfa[i % SPD] = bld(VA, vx, rq);
-        pfk[i % SPD] = kzt[oz + t + SPD];             // oz + t + SPD in [-H, NZ + H + SUN)
+        pfk[i % SPD] = kzt[oz + t + SPD];             // oz + t + SPD in [-H, NZ + H + SUNF)
         if constexpr (TAN) {
           pfdp[i % SPD] = bld(VW, vx, o_dpi + rq);
           pfdq[i % SPD] = bld(VW, vx, o_dqi + rq);
@@ -818,13 +822,13 @@ __global__ __launch_bounds__(NT) void k_fws(FwdArgs a) {
       }
       __syncthreads();
     };
-    // does some level 1..KL compute window row r in steps tb .. tb+SUN-1
-    auto meets = [&](int tb, int r) { return r + 2 <= tb + SUN - 1 && r + 2 * KL >= tb; };
-    for (int tb = 0; tb < U; tb += SUN) {
+    // does some level 1..KL compute window row r in steps tb .. tb+SUNF-1
+    auto meets = [&](int tb, int r) { return r + 2 <= tb + SUNF - 1 && r + 2 * KL >= tb; };
+    for (int tb = 0; tb < U; tb += SUNF) {
       if (meets(tb, su0) || meets(tb, su1) || meets(tb, rui))
-        static_for<0, SUN>([&](auto ic) { body(ic, std::true_type(), tb + decltype(ic)::value); });
+        static_for<0, SUNF>([&](auto ic) { body(ic, std::true_type(), tb + decltype(ic)::value); });
       else
-        static_for<0, SUN>([&](auto ic) { body(ic, std::false_type(), tb + decltype(ic)::value); });
+        static_for<0, SUNF>([&](auto ic) { body(ic, std::false_type(), tb + decltype(ic)::value); });
     }
     if (seis && rui >= H && colint && rc >= 0) {
 #pragma unroll
@@ -1055,7 +1059,7 @@ __global__ __launch_bounds__(NT) void k_bns(BornArgs a) {
   constexpr int SUNB = 2 * KL <= 6 ? 6 : 12;
   static_assert(2 * KL <= SUNB && SUNB % 6 == 0 && SUNB % SPB == 0 && SUNB % SPH == 0, "pipeline rings");
   constexpr int H = 2 * KL;
-  constexpr int RS = 3 * KL + 1;          // column-major level rings, odd stride (see k_fws)
+  constexpr int RS = (3 * KL) | 1;        // column-major level rings, odd stride (see k_fws)
   __shared__ float RG[(NT + 4) * RS];
   __shared__ float4 CF[SUNB][NT];         // (alpha, dalpha, 1 - kappa, dkappa) of window row u at [u % SUNB]
   const unsigned c = threadIdx.x;
@@ -1521,7 +1525,7 @@ __global__ __launch_bounds__(NT) void k_ads(AdjArgs a) {
   constexpr int SUNA = 2 * KL <= 12 ? 12 : 24;   // pipeline steps per unrolled block; coefficient ring length
   static_assert(2 * KL <= SUNA && SUNA % 6 == 0 && SUNA % SPB == 0 && SUNA % SPH == 0, "pipeline rings");
   constexpr int H = 2 * KL;
-  constexpr int RS = 3 * KL + 1;          // column-major level rings, odd stride (see k_fws)
+  constexpr int RS = (3 * KL) | 1;        // column-major level rings, odd stride (see k_fws)
   __shared__ float RG[(NT + 4) * RS];     // alpha*lam of levels 0..KL-1
   __shared__ float2 AK[SUNA][NT];         // (alpha, 1 - kappa) of window row u at [u % SUNA], own column
   const unsigned c = threadIdx.x;
@@ -2304,9 +2308,10 @@ const Shape kShapes[SEIS_KIND_COUNT][7] = {
      {128, 128, 1024}}};
 
 // Streamed forward (strips = 2): rx-column strips, one thread per column, k
-// levels per launch, rz rows per segment (0: the whole column).
+// levels per launch, rz rows per segment (0: the whole column); the plain
+// forward also at 5 levels, the saving forward and fused tangent at 3 or 4.
 constexpr int kStreamNT[4] = {64, 128, 192, 256};
-constexpr int kStreamKMin = 3, kStreamKMax = 4;
+constexpr int kStreamKMin = 3, kStreamKMax = 4, kStreamFwdKMax = 5;
 
 // Streamed Born (strips = 2): the same strips, 3 or 4 levels per launch.
 constexpr int kStreamBornKMin = 3, kStreamBornKMax = 4;
@@ -2319,7 +2324,7 @@ bool stream_cfg(int kind, const KCfg& c) {
   bool nt_ok = false;
   for (int nt : kStreamNT) nt_ok = nt_ok || c.rx == nt;
   if (!nt_ok || !(c.rz == 0 || c.rz >= 8)) return false;
-  if (kind == SEIS_KIND_FORWARD) return c.k >= kStreamKMin && c.k <= kStreamKMax;
+  if (kind == SEIS_KIND_FORWARD) return c.k >= kStreamKMin && c.k <= kStreamFwdKMax;
   // saving forward and fused forward + tangent: 128- or 192-column strips
   if (kind == SEIS_KIND_RECORD || kind == SEIS_KIND_TANGENT)
     return (c.rx == 128 || c.rx == 192) && c.k >= kStreamKMin && c.k <= kStreamKMax;
@@ -2551,7 +2556,9 @@ const void* fwd_ptr(const KCfg& c) {
 
 template <int NT>
 const void* fws_ptr_nt(int k) {
-  return k == 3 ? (const void*)k_fws<NT, 3, false, false> : (const void*)k_fws<NT, 4, false, false>;
+  return k == 3   ? (const void*)k_fws<NT, 3, false, false>
+         : k == 5 ? (const void*)k_fws<NT, 5, false, false>
+                  : (const void*)k_fws<NT, 4, false, false>;
 }
 
 // the streamed forward variants: plain (64-256 columns), saving forward
@@ -2634,6 +2641,7 @@ void launch_fwd_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
 template <int NT>
 void launch_fws_t(const KCfg& c, int grid, hipStream_t s, const FwdArgs& a) {
   if (c.k == 3) hipLaunchKernelGGL((k_fws<NT, 3, false, false>), dim3(grid), dim3(NT), 0, s, a);
+  else if (c.k == 5) hipLaunchKernelGGL((k_fws<NT, 5, false, false>), dim3(grid), dim3(NT), 0, s, a);
   else hipLaunchKernelGGL((k_fws<NT, 4, false, false>), dim3(grid), dim3(NT), 0, s, a);
 }
 

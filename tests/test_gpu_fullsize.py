@@ -100,7 +100,7 @@ def test_tuned_batch32_plan_golden():
 
 def test_tuned_linesearch_plan_golden():
     """The benchmark's line-search fan-out (1600 forward-only 70x70 models,
-    streamed whole-column strips of 192): the golden model at slot 777, its
+    streamed whole-column strips of 128, 5 levels): the golden model at slot 777, its
     seismograms and its misfit against the fixtures."""
     g = Geometry()
     fwd, gold = _gold70()
@@ -111,7 +111,7 @@ def test_tuned_linesearch_plan_golden():
     P = Propagator([g] * n, caps=CAP_FORWARD)
     applied = autotune.apply(P)
     print("linesearch configs", _configs(P), flush=True)
-    assert applied["forward"] == (192, 0, 4, 2)
+    assert applied["forward"] == (128, 0, 5, 2)
     v = torch.tensor(np.concatenate(vecs))
     seis = P.forward(v)
     sl = slice(int(P.seis_off[at]), int(P.seis_off[at + 1]))

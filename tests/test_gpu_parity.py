@@ -165,7 +165,7 @@ def test_large_forward_windows_bit_identical(small_geoms, cfg):
 
 
 STREAM_CONFIGS = [(128, 0, 4, 2), (128, 0, 3, 2), (192, 0, 4, 2), (256, 0, 3, 2), (128, 8, 4, 2), (64, 0, 3, 2), (64, 13, 4, 2),
-                  (192, 23, 3, 2)]
+                  (192, 23, 3, 2), (192, 0, 5, 2), (128, 11, 5, 2), (64, 0, 5, 2)]
 
 
 @pytest.mark.parametrize("cfg", STREAM_CONFIGS)
@@ -219,6 +219,8 @@ def test_streamed_forward_ragged_batch():
     Ps.set_config("forward", 128, 0, 3, 2)      # 14 padded rows >= 12
     with pytest.raises(ValueError):
         Ps.set_config("forward", 128, 0, 4, 2)  # 14 < 16
+    with pytest.raises(ValueError):
+        P.set_config("forward", 128, 0, 6, 2)   # streamed forward: 3 to 5 levels
     with pytest.raises(ValueError):
         P.set_config("record", 128, 0, 5, 2)    # streamed record: 3 or 4 levels
     with pytest.raises(ValueError):
