@@ -84,7 +84,10 @@ typedef struct {
 /* Create a plan for n_models models.  ckpt_every = 0 stores the full
  * forward history for the adjoint (reference behaviour, :165-176);
  * ckpt_every = K > 0 stores the wavefield every K steps and recomputes each
- * segment during the adjoint pass.  caps is a mask of SEIS_CAP_*. */
+ * segment during the adjoint pass.  caps is a mask of SEIS_CAP_*.  Each
+ * padded grid, (nz + 2 nbc) x (nx + 2 nbc), may hold at most
+ * INT32_MAX / 4 cells (the kernels address a wavefield with 32-bit byte
+ * offsets). */
 int seis_plan_create(const seis_model_desc* models /* host */, int32_t n_models,
                      const seis_time_desc* time /* host */, int32_t ckpt_every,
                      int32_t caps, seis_plan_t** plan_out /* host */);

@@ -54,6 +54,15 @@ def test_plan_bytes_rejects_bad_geometry(bad):
         plan_bytes([g])
 
 
+def test_plan_bytes_rejects_grids_beyond_32bit_offsets():
+    """A padded grid above INT32_MAX / 4 cells is refused (one wavefield is
+    addressed with 32-bit byte offsets); one just below is accepted."""
+    kw = dict(nbc=8, nt=200, n_shots=1, src_x=(5,), rcv_x=(1, 2, 3))
+    with pytest.raises(ValueError, match="model too large"):
+        plan_bytes([Geometry(nz=23200 - 16, nx=23200 - 16, **kw)], caps=_lib.CAP_FORWARD)
+    assert plan_bytes([Geometry(nz=23168 - 16, nx=23168 - 16, **kw)], caps=_lib.CAP_FORWARD) > 0
+
+
 def test_plan_bytes_rejects_bad_options():
     g = Geometry(nz=10, nx=10, nbc=5, nt=200)
     with pytest.raises(ValueError):
